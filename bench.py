@@ -1,0 +1,165 @@
+"""TreePM particle-force-updates/s on one MI355X: FP32 (fixed-point) form vs the plain FP64 form.
+
+Workload: the clustered two-species box (N = 2.2e7, 1.1e7 m+ / 1.1e7 m-, 60 % of m+ in 2000 NFW
+halos, m- rejection-sampled in under-dense regions), L = 250, PM 256^3, r_s = 1.25 cells,
+r_cut = 4.5 r_s, theta = 0.5, leaf 8, m+ = 1, |m-| = 19, eps = L / N^(1/3) / 30.
+A step is one complete force evaluation (sort, tree, deposit, FFT, interpolation, walk) on inputs
+already resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "janus-sim_amd"))
+sys.path.insert(0, ROOT)
+
+import janus_treepm as j  # noqa: E402
+from janus_treepm import synth  # noqa: E402
+
+METRIC = "TreePM particle-force-updates/s at N=2.2e7 on 1 GPU, FP32 vs FP64 at matched error"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+FLOPS_PER_INTERACTION = 38
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_ctx(w, precision):
+    p = j.default_params(w.box_size, w.n_cells)
+    p.r_s, p.r_cut, p.theta, p.softening, p.leaf_size = w.r_s, w.r_cut, w.theta, w.softening, w.leaf_size
+    p.precision = precision
+    return j.ForceContext(p, w.m_plus, w.m_minus_abs)
+
+
+def run_form(w, d_pos, d_sp, precision, warmup, steps):
+    ctx = make_ctx(w, precision)
+    rt = np.float64 if precision == j.FP64 else np.float32
+    d_acc = j.DeviceArray((w.n, 3), rt)
+    for i in range(warmup):
+        ctx.compute_forces_device(w.n, d_pos, d_sp, d_acc)
+        log(f"  warmup {i + 1}/{warmup} precision={'fp64' if precision else 'fp32'}")
+    j.load_library().jtp_device_synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ctx.compute_forces_device(w.n, d_pos, d_sp, d_acc)
+        stats.append(ctx.stats())
+    j.load_library().jtp_device_synchronize()
+    t = time.perf_counter() - t0
+    log(f"  timed {steps} steps precision={'fp64' if precision else 'fp32'}: {t / steps * 1e3:.2f} ms/step")
+    ctx.close()
+    return t, stats, d_acc
+
+
+def phase_summary(stats):
+    keys = ["ms_sort", "ms_build", "ms_deposit", "ms_fft", "ms_interp", "ms_walk", "ms_total"]
+    return {k: float(np.mean([s[k] for s in stats])) for k in keys}
+
+
+def cpu_baseline(w_full):
+    """Reference CPU TreePM (oracle restatement of treepm_force.rs / tree_short.rs / pm_grid.rs: leaf-1 octree,
+    x^4 split, r_s = r_cut/3, two-grid PM) timed on a bounded sample: the same recipe in a box of 1/64 the
+    volume at the same number density, cell size, r_s, theta and softening."""
+    from oracle.oracle_lib import RefCpu, load
+
+    orc = load()
+    n_s = w_full.n // 64
+    Ls = w_full.box_size / 4
+    Gs = w_full.n_cells // 4
+    ws = synth.clustered(n_total=n_s, box_size=Ls, n_cells=Gs, n_halos=max(1, 2000 // 64), seed=99)
+    h = Ls / Gs
+    r_s = w_full.r_s
+    r_cut = 3.0 * r_s
+    mass = np.where(ws.species > 0, ws.m_plus, ws.m_minus_abs)
+    ref = RefCpu(orc, r_cut, Gs, Ls, w_full.theta, w_full.softening, r_s, 1.0 / h ** 3, 1.0)
+    t0 = time.perf_counter()
+    ref.update(ws.pos, mass, ws.species)
+    ref.forces()
+    t = time.perf_counter() - t0
+    return {
+        "value": n_s / t, "unit": "particle-force-updates/s", "cores": int(orc.lib.orc_num_threads()),
+        "kind": "port",
+        "sample": f"C++ restatement of the reference CPU TreePM (leaf-1 octree, x^4 split, r_s=r_cut/3, two-grid "
+                  f"FP64 PM) on the same recipe at 1/64 volume: N={n_s}, L={Ls:g}, PM {Gs}^3, theta={w_full.theta}, "
+                  f"eps={w_full.softening:.4g}; one update + all forces in {t:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=22_000_000)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the TreePM step is single-GPU; "
+              "multi-GPU sharding is design-only) -- run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if j.device_count() <= 0:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.perf_counter()
+    w = synth.clustered(n_total=args.n, box_size=250.0, n_cells=256, n_halos=2000, seed=7)
+    log(f"workload generated in {time.perf_counter() - t0:.1f} s (N={w.n})")
+    pos = j.to_fixed(w.pos, w.box_size)
+    d_pos = j.DeviceArray.from_host(pos)
+    d_sp = j.DeviceArray.from_host(w.species)
+    del pos
+
+    t32, st32, _ = run_form(w, d_pos, d_sp, j.FP32, args.warmup, args.steps)
+    ms32 = t32 / args.steps * 1e3
+    value = w.n * args.steps / t32
+    ph32 = phase_summary(st32)
+    inter = int(np.mean([s["interactions"] for s in st32]))
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-force-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"clustered two-species box N={w.n} (2000 NFW halos + uniform m+, void-filling m-), "
+                               f"L=250, PM 256^3, r_s=1.25 cells, r_cut=4.5 r_s, theta=0.5, leaf 8, "
+                               f"m+=1, |m-|=19; FP32 fixed-point form",
+                   "n_particles": w.n, "pm_grid": 256, "theta": w.theta, "r_s": w.r_s, "r_cut": w.r_cut,
+                   "softening": w.softening, "parallelism": "single GPU"},
+    }
+    walk_ms = ph32["ms_walk"]
+    achieved = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "valu", "kernel": "k_walk (group walk, FP32)", "achieved": achieved,
+                       "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                       "traffic": None, "interactions": inter, "flops_per_interaction": FLOPS_PER_INTERACTION,
+                       "walk_ms": walk_ms}
+    G3 = 256 ** 3
+    hbm = {
+        "deposit": (w.n * 13 + 4 * G3) / (ph32["ms_deposit"] * 1e-3) / 1e9,
+        "fft": (40 * G3) / (ph32["ms_fft"] * 1e-3) / 1e9,
+        "interp": (w.n * 25 + 4 * G3) / (ph32["ms_interp"] * 1e-3) / 1e9,
+    }
+    out["phases_ms"] = ph32
+    out["phase_hbm_gbs"] = {k: {"achieved": v, "frac": v / PEAK_HBM_GBS} for k, v in hbm.items()}
+
+    if not args.no_fp64:
+        t64, st64, _ = run_form(w, d_pos, d_sp, j.FP64, 1, args.fp64_steps)
+        out["fp64"] = {"value": w.n * args.fp64_steps / t64, "ms_per_step": t64 / args.fp64_steps * 1e3,
+                       "steps": args.fp64_steps, "phases_ms": phase_summary(st64)}
+        out["fp32_over_fp64"] = out["fp64"]["ms_per_step"] / ms32
+
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

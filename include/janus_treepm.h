@@ -1,0 +1,152 @@
+/*
+ * janus_treepm.h — C ABI of the MI355X (gfx950) bimetric TreePM force step.
+ *
+ * One context owns every plan, table and scratch buffer of one force
+ * evaluation on one GPU; nothing is process-global, so distinct contexts may
+ * be driven from distinct host threads.  Every function returns an int:
+ * JTP_OK (0) on success or a negative JTP_ERR_* code, with the message
+ * available from jtp_last_error().  No function aborts the process.
+ *
+ * Particle arrays are caller-owned DEVICE pointers.  Positions are 32-bit
+ * fixed point per axis, u = floor((x/L + 1/2) * 2^32) mod 2^32 for x in
+ * [-L/2, L/2) (the reference's box convention, src/treepm/pm_grid.rs:64-70),
+ * stored xyz-interleaved (u[3*i+0..2]) like the reference's f32 particle
+ * arrays (src/nbody_gpu_twopass.rs:1817-1823).  Species are int8 +1 / -1
+ * (MassSign, src/lib.rs:82-86).  Same species attract, opposite species repel.
+ *
+ * Streams are passed as `void*` holding a hipStream_t (NULL = null stream).
+ */
+#ifndef JANUS_TREEPM_H
+#define JANUS_TREEPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (reference: cufft_wrapper.cu:13-27 returns -1 on any error) */
+#define JTP_OK            0
+#define JTP_ERR_ARG      -1   /* invalid argument (null pointer, bad size, bad parameter) */
+#define JTP_ERR_HIP      -2   /* a HIP runtime call failed                               */
+#define JTP_ERR_ALLOC    -3   /* device allocation failed                                */
+#define JTP_ERR_OVERFLOW -4   /* a bounded on-device structure overflowed (walk stack)   */
+#define JTP_ERR_STATE    -5   /* call out of order (e.g. stage read before a compute)    */
+
+/* ---- enumerations */
+#define JTP_FP32 0            /* fixed-point positions, FP32 walk/PM, Kahan accumulation */
+#define JTP_FP64 1            /* plain FP64 form of the same kernels                     */
+
+#define JTP_SPLIT_ERFC_GAUSS 0 /* S = erfc(x) + 2x/sqrt(pi) e^{-x^2}, x = rho/(2 r_s): force-consistent with PM e^{-k^2 r_s^2} */
+#define JTP_SPLIT_ERFC       1 /* S = erfc(x) only: reference GPU walk (nbody_gpu_twopass.rs:1776-1786) */
+#define JTP_SPLIT_POLY4      2 /* S = 1-(r/r_cut)^4: reference CPU tree (tree_short.rs:17-31, splitting.rs:20-38) */
+
+#define JTP_GRAD_4PT 0        /* (8(f[+1]-f[-1]) - (f[+2]-f[-2])) / 12h */
+#define JTP_GRAD_2PT 1        /* (f[+1]-f[-1]) / 2h: reference pm_grid.rs:329-348, nbody_gpu_twopass.rs:233-244 */
+
+/* ---- parameters (YAML drop-in: simulation.box_size_mpc config.rs:23, pm_grid.n_cells :98,
+ *      pm_grid.k_min :101, simulation.theta :36, simulation.softening_mpc :39) */
+typedef struct jtp_params {
+    double  box_size;    /* L                                                        */
+    int32_t n_cells;     /* G, PM grid per axis; power of two in [16, 512]           */
+    int32_t k_min;       /* zero modes with max|k_i| < k_min (0 = only DC); cufft_wrapper.cu:170-181 */
+    double  r_s;         /* Gaussian split scale (length units)                      */
+    double  r_cut;       /* short-range cutoff (length units), 0 < r_cut <= L/5      */
+    double  theta;       /* opening angle, 0 <= theta <= 1 (0 = direct sum)          */
+    double  softening;   /* Plummer epsilon > 0                                      */
+    double  g_constant;  /* G                                                        */
+    int32_t leaf_size;   /* particles per leaf bucket, 1..16                         */
+    int32_t group_size;  /* targets per walk group, 1..64                            */
+    int32_t precision;   /* JTP_FP32 / JTP_FP64                                      */
+    int32_t split_mode;  /* JTP_SPLIT_*                                              */
+    int32_t gradient;    /* JTP_GRAD_*                                               */
+    int32_t deconvolve;  /* 1: divide the Green function by the CIC window W(k)^2    */
+    int32_t pm_enabled;  /* 0: skip the long-range part                              */
+    int32_t tree_enabled;/* 0: skip the short-range part                             */
+} jtp_params;
+
+/* Per-phase timings of the last jtp_compute_forces (HIP events on the call's stream). */
+typedef struct jtp_stats {
+    double  ms_total;
+    double  ms_sort;      /* Morton keys + radix sort + particle gather        */
+    double  ms_build;     /* Karras hierarchy + bottom-up monopoles + groups   */
+    double  ms_deposit;   /* cell offsets + deterministic CIC gather deposit   */
+    double  ms_fft;       /* R2C, Green, C2R                                   */
+    double  ms_interp;    /* gradient + CIC interpolation                      */
+    double  ms_walk;      /* group walk (one kernel launch)                    */
+    uint64_t interactions;/* pair + monopole interactions evaluated by the walk */
+    int64_t n_groups;
+    int64_t n_particles;
+} jtp_stats;
+
+/* Stages readable with jtp_stage_copy after a jtp_compute_forces (parity tests). */
+#define JTP_STAGE_KEYS      1 /* uint64[n] sorted Morton keys                                   */
+#define JTP_STAGE_PERM      2 /* int32[n]  sorted slot -> original particle index               */
+#define JTP_STAGE_NODES     3 /* int32[n-1][4] internal nodes: first, last, left, right
+                                 (child >= 0: internal node; child < 0: leaf ~sorted_index)     */
+#define JTP_STAGE_MONO      4 /* double[n-1][8]: q+, com+ xyz, q-, com- xyz as used by the walk
+                                 (q = sigma G |m| count; com in fixed-point units)              */
+#define JTP_STAGE_RHO       5 /* real[G^3]  deposited mass per cell sum_p q_p W, x fastest      */
+#define JTP_STAGE_PHI       6 /* real[G^3]  long-range potential of rho+ - rho-                 */
+#define JTP_STAGE_ACC_PM    7 /* real[n][3] long-range acceleration, ORIGINAL particle order    */
+#define JTP_STAGE_SUMS      8 /* uint64[n-1][8]: n+, sum x+ y+ z+, n-, sum x- y- z- (exact, fixed units) */
+#define JTP_STAGE_BOXES     9 /* uint32[n-1][6] node bounding boxes bmin xyz, bmax xyz (fixed)  */
+
+/* ---- lifecycle (reference: CuFFTPoisson::new / drop, cufft_ffi.rs:85-118,339-353;
+ *      TreePMForce::new, treepm_force.rs:35-51) */
+typedef struct jtp_ctx jtp_ctx;
+/* Defaults: r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, softening = L/G/30,
+ * G = 1, leaf 8, group 64, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on. */
+void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells);
+int  jtp_create(const jtp_params* p, jtp_ctx** out);
+int  jtp_destroy(jtp_ctx* ctx);
+const char* jtp_last_error(const jtp_ctx* ctx);
+const char* jtp_version(void);
+
+/* Species masses: m_plus > 0 for species +1, m_minus_abs = |m-| > 0 for species -1.
+ * Replaces the reference's per-particle mass (Particle.mass, nbody.rs:79-91) and
+ * mass_factor (nbody_gpu_twopass.rs:2319-2322). */
+int  jtp_set_species_mass(jtp_ctx* ctx, double m_plus, double m_minus_abs);
+
+/* ---- the integrator's force call.
+ * Replaces TreePMForce::update + compute_all_forces (treepm_force.rs:66-87,124-131) and the
+ * force part of GpuNBodyTwoPass::step_treepm_gpu (nbody_gpu_twopass.rs:4529-4616, 3961-4138).
+ * d_acc receives float[n*3] (JTP_FP32) or double[n*3] (JTP_FP64) in the caller's order:
+ * a_i = sigma_i * sum_j sigma_j G |m_j| (x_j - x_i)/|.|^3, periodic, mean density removed. */
+int  jtp_compute_forces(jtp_ctx* ctx, int64_t n, const uint32_t* d_pos_fixed,
+                        const int8_t* d_species, void* d_acc, void* stream);
+
+/* Position conversion to fixed point on the device (x in any period; wrapped). */
+int  jtp_pos_to_fixed_f64(jtp_ctx* ctx, int64_t n, const double* d_pos, uint32_t* d_fixed, void* stream);
+int  jtp_pos_to_fixed_f32(jtp_ctx* ctx, int64_t n, const float* d_pos, uint32_t* d_fixed, void* stream);
+
+/* ---- PM-only Poisson solve on caller grids (reference: cufft_solve_device[_filtered],
+ * cufft_wrapper.cu:336-398, bound at cufft_ffi.rs:291-337).  d_rho: real[G^3] mass per cell
+ * (x fastest); d_phi: real[G^3] potential of that mass with the context's Green function
+ * (split r_s, k_min, deconvolution).  real = float or double per the context precision. */
+int  jtp_pm_solve(jtp_ctx* ctx, const void* d_rho, void* d_phi, void* stream);
+
+/* ---- reference-compatible Morton keys of f32 positions (compute_morton_all,
+ * nbody_gpu_twopass.rs:1447-1487): 21 bits/axis of (x + box_half) * inv_cell_size. */
+int  jtp_morton_keys_f32(int64_t n, const float* d_pos, float box_half, float inv_cell_size,
+                         uint64_t* d_keys, void* stream);
+
+/* ---- diagnostics */
+int  jtp_get_stats(jtp_ctx* ctx, jtp_stats* out);            /* synchronises the last call */
+int  jtp_stage_size(jtp_ctx* ctx, int32_t stage, size_t* bytes);
+int  jtp_stage_copy(jtp_ctx* ctx, int32_t stage, void* host_dst, size_t bytes);
+
+/* ---- device memory helpers (reference: cufft_alloc/free/copy_*, cufft_wrapper.cu:251-288) */
+void* jtp_alloc(size_t bytes);
+void  jtp_free(void* p);
+int   jtp_copy_h2d(void* d_dst, const void* h_src, size_t bytes);
+int   jtp_copy_d2h(void* h_dst, const void* d_src, size_t bytes);
+int   jtp_copy_d2d(void* d_dst, const void* d_src, size_t bytes);
+int   jtp_device_synchronize(void);
+int   jtp_device_count(int* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JANUS_TREEPM_H */

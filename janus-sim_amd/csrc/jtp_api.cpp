@@ -1,0 +1,526 @@
+// C ABI of the TreePM force step: context lifecycle, buffer management and
+// the phase sequence of one force evaluation.
+//
+// Phase order on the caller's stream (each phase bracketed by a HIP event):
+//   keys + radix sort + particle gather | Karras tree + monopoles + groups |
+//   CIC deposit | FFT Poisson solve | interpolation | group walk (+ PM add, scatter to caller order)
+// The reference runs the same pieces with a device synchronisation after
+// every launch (src/nbody_gpu_twopass.rs:4486-4649); here only the final
+// status read synchronises.
+#include "jtp_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace jtp {
+
+static int fail(Ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+static int hip_fail(Ctx* c, hipError_t e, const char* where) {
+    return fail(c, JTP_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+#define JTP_HIP(c, call)                                    \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T*& p, size_t count) {
+    if (p) {
+        (void)hipFree((void*)p);
+        p = nullptr;
+    }
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e == hipSuccess) p = (T*)q;
+    return e;
+}
+static hipError_t dalloc_bytes(void*& p, size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    if (bytes == 0) bytes = 1;
+    return hipMalloc(&p, bytes);
+}
+
+// Chebyshev interpolant of P on [0, X] (degree kPolyDeg), converted to a
+// polynomial in t = 2x/X - 1.  P(x) = erfcx(x) [+ 2x/sqrt(pi)], so that
+// S(x) = exp(-x^2) P(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)].
+void make_split_poly(Ctx& c) {
+    const int N = kPolyDeg + 1;
+    const double rho_max = std::sqrt(c.p.r_cut * c.p.r_cut + c.p.softening * c.p.softening);
+    const double X = (c.p.r_s > 0 ? rho_max / (2.0 * c.p.r_s) : 1.0) * 1.0001 + 1e-12;
+    const bool gauss = c.p.split_mode == JTP_SPLIT_ERFC_GAUSS;
+    std::vector<double> f(N), cheb(N, 0.0);
+    for (int k = 0; k < N; ++k) {
+        const double t = std::cos(M_PI * (k + 0.5) / N);
+        const double x = 0.5 * X * (t + 1.0);
+        double v = std::erfc(x) * std::exp(x * x);
+        if (gauss) v += 2.0 * x / std::sqrt(M_PI);
+        f[k] = v;
+    }
+    for (int j = 0; j < N; ++j) {
+        double s = 0;
+        for (int k = 0; k < N; ++k) s += f[k] * std::cos(M_PI * j * (k + 0.5) / N);
+        cheb[j] = 2.0 * s / N;
+    }
+    cheb[0] *= 0.5;
+    // T_j(t) as monomials: T_0 = 1, T_1 = t, T_{j+1} = 2t T_j - T_{j-1}
+    std::vector<std::vector<double>> T(N, std::vector<double>(N, 0.0));
+    T[0][0] = 1.0;
+    if (N > 1) T[1][1] = 1.0;
+    for (int j = 1; j + 1 < N; ++j)
+        for (int d = 0; d < N; ++d) {
+            double v = -T[j - 1][d];
+            if (d > 0) v += 2.0 * T[j][d - 1];
+            T[j + 1][d] = v;
+        }
+    for (int d = 0; d < N; ++d) {
+        double s = 0;
+        for (int j = 0; j < N; ++j) s += cheb[j] * T[j][d];
+        c.poly.c[d] = (float)s;
+    }
+    c.poly_tscale = (float)(c.p.r_s > 0 ? 1.0 / (c.p.r_s * X) : 0.0);
+}
+
+static void free_particle_buffers(Ctx& c) {
+    void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
+                    c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c.keys0 = c.keys1 = nullptr;
+    c.vals0 = c.vals1 = nullptr;
+    c.hist = c.hist_blk = nullptr;
+    c.parts = nullptr;
+    c.node_geo = nullptr;
+    c.node_kids = nullptr;
+    c.parent_int = c.parent_leaf = c.flags = nullptr;
+    c.node_sum = c.node_mono = nullptr;
+    c.groups = nullptr;
+    c.pm_acc = nullptr;
+    c.cap_n = 0;
+}
+
+static int ensure_capacity(Ctx& c, int64_t n) {
+    if (n <= c.cap_n) return JTP_OK;
+    free_particle_buffers(c);
+    const int64_t cap = n < 1024 ? 1024 : n + n / 8;
+    const int64_t nblk = (cap + kSortTile - 1) / kSortTile;
+    const int64_t m = nblk * 256;
+    const size_t rs = c.real_size();
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) {
+        if (e == hipSuccess && x != hipSuccess) e = x;
+    };
+    chk(dalloc(c.keys0, cap));
+    chk(dalloc(c.keys1, cap));
+    chk(dalloc(c.vals0, cap));
+    chk(dalloc(c.vals1, cap));
+    chk(dalloc(c.hist, m));
+    chk(dalloc(c.hist_blk, (m + 2047) / 2048 + 1));
+    chk(dalloc(c.parts, cap));
+    chk(dalloc(c.node_geo, 2 * cap));
+    chk(dalloc(c.node_kids, cap));
+    chk(dalloc(c.parent_int, cap));
+    chk(dalloc(c.parent_leaf, cap));
+    chk(dalloc(c.flags, cap));
+    chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));
+    chk(dalloc_bytes(c.node_mono, (size_t)cap * (rs == 8 ? 64 : 32)));
+    chk(dalloc(c.groups, cap + 4096 + 64));
+    chk(dalloc_bytes(c.pm_acc, (size_t)cap * 3 * rs));
+    if (e != hipSuccess) {
+        free_particle_buffers(c);
+        (void)hipGetLastError();
+        return fail(&c, JTP_ERR_ALLOC, std::string("device allocation failed: ") + hipGetErrorString(e));
+    }
+    c.cap_n = cap;
+    return JTP_OK;
+}
+
+static int check_params(const jtp_params* p, std::string& msg) {
+    if (!p) { msg = "params is null"; return JTP_ERR_ARG; }
+    if (!(p->box_size > 0) || !std::isfinite(p->box_size)) { msg = "box_size must be > 0"; return JTP_ERR_ARG; }
+    const int G = p->n_cells;
+    if (G < 16 || G > 512 || (G & (G - 1))) { msg = "n_cells must be a power of two in [16, 512]"; return JTP_ERR_ARG; }
+    if (!(p->r_s >= 0) || !std::isfinite(p->r_s)) { msg = "r_s must be >= 0"; return JTP_ERR_ARG; }
+    if (!(p->r_cut > 0) || p->r_cut > p->box_size / 5.0) { msg = "r_cut must be in (0, L/5]"; return JTP_ERR_ARG; }
+    if (!(p->theta >= 0) || p->theta > 1.0) { msg = "theta must be in [0, 1]"; return JTP_ERR_ARG; }
+    if (!(p->softening > 0) || !std::isfinite(p->softening)) { msg = "softening must be > 0"; return JTP_ERR_ARG; }
+    if (!std::isfinite(p->g_constant)) { msg = "g_constant must be finite"; return JTP_ERR_ARG; }
+    if (p->leaf_size < 1 || p->leaf_size > 16) { msg = "leaf_size must be in [1, 16]"; return JTP_ERR_ARG; }
+    if (p->group_size < 1 || p->group_size > 64) { msg = "group_size must be in [1, 64]"; return JTP_ERR_ARG; }
+    if (p->precision != JTP_FP32 && p->precision != JTP_FP64) { msg = "precision must be JTP_FP32 or JTP_FP64"; return JTP_ERR_ARG; }
+    if (p->split_mode < JTP_SPLIT_ERFC_GAUSS || p->split_mode > JTP_SPLIT_POLY4) { msg = "bad split_mode"; return JTP_ERR_ARG; }
+    if (p->split_mode != JTP_SPLIT_POLY4 && !(p->r_s > 0)) { msg = "erfc split modes need r_s > 0"; return JTP_ERR_ARG; }
+    if (p->gradient != JTP_GRAD_4PT && p->gradient != JTP_GRAD_2PT) { msg = "bad gradient"; return JTP_ERR_ARG; }
+    if (p->k_min < 0) { msg = "k_min must be >= 0"; return JTP_ERR_ARG; }
+    return JTP_OK;
+}
+
+}  // namespace jtp
+
+using namespace jtp;
+
+struct jtp_ctx {
+    Ctx c;
+};
+
+extern "C" {
+
+const char* jtp_version(void) { return "janus_treepm 0.1 (gfx950)"; }
+
+void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->box_size = box_size;
+    p->n_cells = n_cells;
+    p->k_min = 0;
+    const double h = n_cells > 0 ? box_size / n_cells : 1.0;
+    p->r_s = 1.25 * h;
+    p->r_cut = 4.5 * p->r_s;
+    p->theta = 0.5;
+    p->softening = h / 30.0;
+    p->g_constant = 1.0;
+    p->leaf_size = 8;
+    p->group_size = 64;
+    p->precision = JTP_FP32;
+    p->split_mode = JTP_SPLIT_ERFC_GAUSS;
+    p->gradient = JTP_GRAD_4PT;
+    p->deconvolve = 1;
+    p->pm_enabled = 1;
+    p->tree_enabled = 1;
+}
+
+int jtp_create(const jtp_params* p, jtp_ctx** out) {
+    if (!out) return JTP_ERR_ARG;
+    *out = nullptr;
+    std::string msg;
+    int rc = check_params(p, msg);
+    if (rc != JTP_OK) {
+        std::fprintf(stderr, "jtp_create: %s\n", msg.c_str());
+        return rc;
+    }
+    jtp_ctx* h = new (std::nothrow) jtp_ctx();
+    if (!h) return JTP_ERR_ALLOC;
+    Ctx& c = h->c;
+    c.p = *p;
+    int G = p->n_cells;
+    while ((1 << c.logG) < G) ++c.logG;
+    hipError_t e = hipGetDevice(&c.device);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "jtp_create: hipGetDevice: %s\n", hipGetErrorString(e));
+        delete h;
+        return JTP_ERR_HIP;
+    }
+    const size_t rs = c.real_size();
+    const int64_t G3 = c.G3();
+    e = dalloc_bytes(c.rho, G3 * rs);
+    if (e == hipSuccess) e = dalloc_bytes(c.phi, G3 * rs);
+    if (e == hipSuccess) e = dalloc_bytes(c.spec, (size_t)G * G * c.Hc() * 2 * rs);
+    if (e == hipSuccess) e = dalloc_bytes(c.twiddle, (size_t)2 * G * rs);
+    if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)2 * G * rs);
+    if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
+    if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
+    if (e == hipSuccess) e = dalloc(c.dev_counters, 8);
+    if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_pinned, 4 * sizeof(unsigned long long), 0);
+    if (e == hipSuccess) e = pm_tables(c);
+    if (e == hipSuccess) {
+        for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c.ev[i]);
+        c.events_ok = (e == hipSuccess);
+    }
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "jtp_create: %s\n", hipGetErrorString(e));
+        (void)hipGetLastError();
+        jtp_destroy(h);
+        return JTP_ERR_ALLOC;
+    }
+    make_split_poly(c);
+    *out = h;
+    return JTP_OK;
+}
+
+int jtp_destroy(jtp_ctx* h) {
+    if (!h) return JTP_OK;
+    Ctx& c = h->c;
+    (void)hipDeviceSynchronize();
+    free_particle_buffers(c);
+    void* ptrs[] = {c.rho, c.phi, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4, c.dev_counters, c.dev_inter};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c.host_pinned) (void)hipHostFree(c.host_pinned);
+    for (int i = 0; i < 8; ++i)
+        if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
+    delete h;
+    return JTP_OK;
+}
+
+const char* jtp_last_error(const jtp_ctx* h) { return h ? h->c.err.c_str() : "null context"; }
+
+int jtp_set_species_mass(jtp_ctx* h, double m_plus, double m_minus_abs) {
+    if (!h) return JTP_ERR_ARG;
+    if (!(m_plus >= 0) || !(m_minus_abs >= 0) || !std::isfinite(m_plus) || !std::isfinite(m_minus_abs))
+        return fail(&h->c, JTP_ERR_ARG, "species masses must be finite and >= 0");
+    h->c.m_plus = m_plus;
+    h->c.m_minus = m_minus_abs;
+    return JTP_OK;
+}
+
+int jtp_compute_forces(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_t* d_species, void* d_acc,
+                       void* stream) {
+    if (!h) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    if (n < 0 || n > (int64_t)0x7fffffff - 8192) return fail(&c, JTP_ERR_ARG, "n out of range");
+    if (n > 0 && (!d_pos || !d_species || !d_acc)) return fail(&c, JTP_ERR_ARG, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    c.have_result = false;
+    c.last_n = n;
+    c.last_stream = s;
+    if (n == 0) {
+        c.have_result = true;
+        return JTP_OK;
+    }
+    int rc = ensure_capacity(c, n);
+    if (rc != JTP_OK) return rc;
+    c.n_groups_max = (n + c.p.group_size - 1) / c.p.group_size + 4096;
+
+    JTP_HIP(&c, hipEventRecord(c.ev[0], s));
+    JTP_HIP(&c, launch_fill_i32((int32_t*)c.dev_inter, 4, 0, s));
+    JTP_HIP(&c, launch_morton_keys(d_pos, c.keys0, c.vals0, n, s));
+    JTP_HIP(&c, radix_sort_pairs(c, n, s));
+    JTP_HIP(&c, launch_gather_parts(d_pos, d_species, c.vals0, c.parts, n, s));
+    JTP_HIP(&c, hipEventRecord(c.ev[1], s));
+    if (c.p.tree_enabled) JTP_HIP(&c, build_tree(c, n, s));
+    JTP_HIP(&c, hipEventRecord(c.ev[2], s));
+    if (c.p.pm_enabled) JTP_HIP(&c, pm_deposit(c, n, s));
+    JTP_HIP(&c, hipEventRecord(c.ev[3], s));
+    if (c.p.pm_enabled) JTP_HIP(&c, pm_fft_solve(c, c.rho, c.phi, s));
+    JTP_HIP(&c, hipEventRecord(c.ev[4], s));
+    if (c.p.pm_enabled) JTP_HIP(&c, pm_interp(c, n, s));
+    JTP_HIP(&c, hipEventRecord(c.ev[5], s));
+    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s));
+    else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
+    JTP_HIP(&c, hipEventRecord(c.ev[6], s));
+    // status: interaction count, group count, walk error flag
+    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned, c.dev_inter, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 1, c.dev_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipStreamSynchronize(s));
+    c.have_result = true;
+    if (c.p.tree_enabled) {
+        const int32_t* cnt = (const int32_t*)(c.host_pinned + 1);
+        if (cnt[1] == 1) return fail(&c, JTP_ERR_OVERFLOW, "walk stack overflow: a group's traversal exceeded the stack bound");
+        if (cnt[1] != 0) return fail(&c, JTP_ERR_STATE, "walk found a malformed hierarchy (code " + std::to_string(cnt[1]) + ")");
+    }
+    return JTP_OK;
+}
+
+int jtp_pos_to_fixed_f64(jtp_ctx* h, int64_t n, const double* d_pos, uint32_t* d_fixed, void* stream) {
+    if (!h || n < 0 || (n > 0 && (!d_pos || !d_fixed))) return JTP_ERR_ARG;
+    JTP_HIP(&h->c, launch_fixed_from_f64(d_pos, d_fixed, 3 * n, h->c.p.box_size, (hipStream_t)stream));
+    return JTP_OK;
+}
+int jtp_pos_to_fixed_f32(jtp_ctx* h, int64_t n, const float* d_pos, uint32_t* d_fixed, void* stream) {
+    if (!h || n < 0 || (n > 0 && (!d_pos || !d_fixed))) return JTP_ERR_ARG;
+    JTP_HIP(&h->c, launch_fixed_from_f32(d_pos, d_fixed, 3 * n, h->c.p.box_size, (hipStream_t)stream));
+    return JTP_OK;
+}
+
+int jtp_pm_solve(jtp_ctx* h, const void* d_rho, void* d_phi, void* stream) {
+    if (!h) return JTP_ERR_ARG;
+    if (!d_rho || !d_phi) return fail(&h->c, JTP_ERR_ARG, "null grid pointer");
+    hipStream_t s = (hipStream_t)stream;
+    JTP_HIP(&h->c, pm_fft_solve(h->c, d_rho, d_phi, s));
+    JTP_HIP(&h->c, hipStreamSynchronize(s));
+    return JTP_OK;
+}
+
+int jtp_morton_keys_f32(int64_t n, const float* d_pos, float box_half, float inv_cell_size, uint64_t* d_keys,
+                        void* stream) {
+    if (n < 0 || (n > 0 && (!d_pos || !d_keys))) return JTP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (launch_morton_keys_f32(d_pos, d_keys, n, box_half, inv_cell_size, s) != hipSuccess) return JTP_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return JTP_ERR_HIP;
+    return JTP_OK;
+}
+
+int jtp_get_stats(jtp_ctx* h, jtp_stats* out) {
+    if (!h || !out) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    std::memset(out, 0, sizeof(*out));
+    if (!c.have_result) return fail(&c, JTP_ERR_STATE, "no completed force evaluation");
+    out->n_particles = c.last_n;
+    if (c.last_n <= 0) return JTP_OK;
+    JTP_HIP(&c, hipEventSynchronize(c.ev[6]));
+    float ms[6];
+    for (int i = 0; i < 6; ++i) JTP_HIP(&c, hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]));
+    out->ms_sort = ms[0];
+    out->ms_build = ms[1];
+    out->ms_deposit = ms[2];
+    out->ms_fft = ms[3];
+    out->ms_interp = ms[4];
+    out->ms_walk = ms[5];
+    float tot;
+    JTP_HIP(&c, hipEventElapsedTime(&tot, c.ev[0], c.ev[6]));
+    out->ms_total = tot;
+    out->interactions = c.host_pinned[0];
+    out->n_groups = ((const int32_t*)(c.host_pinned + 1))[0];
+    return JTP_OK;
+}
+
+int jtp_stage_size(jtp_ctx* h, int32_t stage, size_t* bytes) {
+    if (!h || !bytes) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    const int64_t n = c.last_n < 0 ? 0 : c.last_n;
+    const int64_t nI = n > 1 ? n - 1 : 0;
+    const size_t rs = c.real_size();
+    switch (stage) {
+        case JTP_STAGE_KEYS: *bytes = n * 8; break;
+        case JTP_STAGE_PERM: *bytes = n * 4; break;
+        case JTP_STAGE_NODES: *bytes = nI * 16; break;
+        case JTP_STAGE_MONO: *bytes = nI * 64; break;
+        case JTP_STAGE_SUMS: *bytes = nI * 64; break;
+        case JTP_STAGE_RHO:
+        case JTP_STAGE_PHI: *bytes = c.G3() * rs; break;
+        case JTP_STAGE_ACC_PM: *bytes = n * 3 * rs; break;
+        case JTP_STAGE_BOXES: *bytes = nI * 24; break;
+        default: return fail(&c, JTP_ERR_ARG, "unknown stage");
+    }
+    return JTP_OK;
+}
+
+int jtp_stage_copy(jtp_ctx* h, int32_t stage, void* host_dst, size_t bytes) {
+    if (!h || !host_dst) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    size_t need = 0;
+    int rc = jtp_stage_size(h, stage, &need);
+    if (rc != JTP_OK) return rc;
+    if (bytes < need) return fail(&c, JTP_ERR_ARG, "destination too small");
+    if (!c.have_result) return fail(&c, JTP_ERR_STATE, "no completed force evaluation");
+    if (need == 0) return JTP_OK;
+    JTP_HIP(&c, hipDeviceSynchronize());
+    const int64_t n = c.last_n;
+    const int64_t nI = n - 1;
+    switch (stage) {
+        case JTP_STAGE_KEYS: JTP_HIP(&c, hipMemcpy(host_dst, c.keys0, need, hipMemcpyDeviceToHost)); break;
+        case JTP_STAGE_PERM: JTP_HIP(&c, hipMemcpy(host_dst, c.vals0, need, hipMemcpyDeviceToHost)); break;
+        case JTP_STAGE_RHO: JTP_HIP(&c, hipMemcpy(host_dst, c.rho, need, hipMemcpyDeviceToHost)); break;
+        case JTP_STAGE_PHI: JTP_HIP(&c, hipMemcpy(host_dst, c.phi, need, hipMemcpyDeviceToHost)); break;
+        case JTP_STAGE_NODES:
+        case JTP_STAGE_BOXES: {
+            if (!c.p.tree_enabled) return fail(&c, JTP_ERR_STATE, "tree disabled");
+            std::vector<uint4> geo(2 * nI);
+            std::vector<int2> kids(nI);
+            JTP_HIP(&c, hipMemcpy(geo.data(), c.node_geo, geo.size() * sizeof(uint4), hipMemcpyDeviceToHost));
+            JTP_HIP(&c, hipMemcpy(kids.data(), c.node_kids, kids.size() * sizeof(int2), hipMemcpyDeviceToHost));
+            if (stage == JTP_STAGE_NODES) {
+                int32_t* o = (int32_t*)host_dst;
+                for (int64_t i = 0; i < nI; ++i) {
+                    o[4 * i] = (int32_t)geo[2 * i].w;
+                    o[4 * i + 1] = (int32_t)geo[2 * i + 1].w;
+                    o[4 * i + 2] = kids[i].x;
+                    o[4 * i + 3] = kids[i].y;
+                }
+            } else {
+                uint32_t* o = (uint32_t*)host_dst;
+                for (int64_t i = 0; i < nI; ++i) {
+                    o[6 * i] = geo[2 * i].x; o[6 * i + 1] = geo[2 * i].y; o[6 * i + 2] = geo[2 * i].z;
+                    o[6 * i + 3] = geo[2 * i + 1].x; o[6 * i + 4] = geo[2 * i + 1].y; o[6 * i + 5] = geo[2 * i + 1].z;
+                }
+            }
+            break;
+        }
+        case JTP_STAGE_SUMS: {
+            if (!c.p.tree_enabled) return fail(&c, JTP_ERR_STATE, "tree disabled");
+            std::vector<uint64_t> raw(nI * 8);
+            JTP_HIP(&c, hipMemcpy(raw.data(), c.node_sum, nI * 64, hipMemcpyDeviceToHost));
+            uint64_t* o = (uint64_t*)host_dst;
+            for (int64_t i = 0; i < nI; ++i) {
+                const uint64_t* r = &raw[8 * i];
+                const uint32_t* cnt = (const uint32_t*)(r + 6);
+                o[8 * i] = cnt[0];
+                o[8 * i + 1] = r[0]; o[8 * i + 2] = r[1]; o[8 * i + 3] = r[2];
+                o[8 * i + 4] = cnt[1];
+                o[8 * i + 5] = r[3]; o[8 * i + 6] = r[4]; o[8 * i + 7] = r[5];
+            }
+            break;
+        }
+        case JTP_STAGE_MONO: {
+            if (!c.p.tree_enabled) return fail(&c, JTP_ERR_STATE, "tree disabled");
+            double* o = (double*)host_dst;
+            if (c.p.precision == JTP_FP64) {
+                std::vector<double> raw(nI * 8);
+                JTP_HIP(&c, hipMemcpy(raw.data(), c.node_mono, nI * 64, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < nI; ++i) {
+                    o[8 * i] = raw[8 * i + 3];
+                    o[8 * i + 1] = raw[8 * i]; o[8 * i + 2] = raw[8 * i + 1]; o[8 * i + 3] = raw[8 * i + 2];
+                    o[8 * i + 4] = raw[8 * i + 7];
+                    o[8 * i + 5] = raw[8 * i + 4]; o[8 * i + 6] = raw[8 * i + 5]; o[8 * i + 7] = raw[8 * i + 6];
+                }
+            } else {
+                std::vector<uint32_t> raw(nI * 8);
+                JTP_HIP(&c, hipMemcpy(raw.data(), c.node_mono, nI * 32, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < nI; ++i) {
+                    const uint32_t* r = &raw[8 * i];
+                    float qp, qm;
+                    std::memcpy(&qp, &r[3], 4);
+                    std::memcpy(&qm, &r[7], 4);
+                    o[8 * i] = qp;
+                    o[8 * i + 1] = r[0]; o[8 * i + 2] = r[1]; o[8 * i + 3] = r[2];
+                    o[8 * i + 4] = qm;
+                    o[8 * i + 5] = r[4]; o[8 * i + 6] = r[5]; o[8 * i + 7] = r[6];
+                }
+            }
+            break;
+        }
+        case JTP_STAGE_ACC_PM: {
+            if (!c.p.pm_enabled) return fail(&c, JTP_ERR_STATE, "PM disabled");
+            const size_t rs = c.real_size();
+            std::vector<char> raw(n * 3 * rs);
+            std::vector<int32_t> perm(n);
+            JTP_HIP(&c, hipMemcpy(raw.data(), c.pm_acc, raw.size(), hipMemcpyDeviceToHost));
+            JTP_HIP(&c, hipMemcpy(perm.data(), c.vals0, n * 4, hipMemcpyDeviceToHost));
+            char* o = (char*)host_dst;
+            for (int64_t i = 0; i < n; ++i) std::memcpy(o + perm[i] * 3 * rs, raw.data() + i * 3 * rs, 3 * rs);
+            break;
+        }
+        default: return fail(&c, JTP_ERR_ARG, "unknown stage");
+    }
+    return JTP_OK;
+}
+
+void* jtp_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+void jtp_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+int jtp_copy_h2d(void* d, const void* h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice) == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
+int jtp_copy_d2h(void* h, const void* d, size_t b) { return hipMemcpy(h, d, b, hipMemcpyDeviceToHost) == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
+int jtp_copy_d2d(void* d, const void* s, size_t b) { return hipMemcpy(d, s, b, hipMemcpyDeviceToDevice) == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
+int jtp_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
+int jtp_device_count(int* out) {
+    if (!out) return JTP_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = 0;
+        return JTP_ERR_HIP;
+    }
+    *out = n;
+    return JTP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Device-side helpers shared by the kernels (wavefront scans and masks).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace jtp {
+
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
+__device__ inline uint64_t lanemask_lt() {
+    const int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive prefix sum across the 64 lanes of a wavefront.
+template <typename T>
+__device__ inline T wave_incl_scan(T v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T o = __shfl_up(v, off, 64);
+        if (l >= off) v += o;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+// Exclusive prefix sum over a 256-thread block, one value per thread.
+// `lds` must hold 4 uint32 scratch entries; `total` receives the block sum.
+__device__ inline uint32_t block256_excl_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+    const int w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane_id() == 63) lds[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint32_t t = lds[i];
+        if (i < w) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + inc - v;
+}
+
+}  // namespace jtp

@@ -1,0 +1,138 @@
+// Internal declarations shared by the HIP translation units of libjanus_treepm.
+// Device helpers, launch wrappers and the context structure.  Nothing here is
+// part of the C ABI (include/janus_treepm.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <type_traits>
+
+#include "../../include/janus_treepm.h"
+
+namespace jtp {
+
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int kBlock = 256;          // threads per workgroup for streaming kernels
+constexpr int kSortIPT = 8;          // keys per thread per radix-sort tile
+constexpr int kSortTile = kBlock * kSortIPT;
+constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Morton cell (L/16)
+constexpr int kPolyDeg = 12;         // FP32 split-function polynomial degree
+constexpr int kFftPoints = 4096;     // complex points held in LDS per FFT workgroup
+
+// ---------------------------------------------------------------- small types
+template <typename R> struct Cplx { R x, y; };
+
+// Morton interleave of a 21-bit integer into every third bit (x in bit 0).
+__host__ __device__ inline uint64_t spread3_21(uint64_t v) {
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8))  & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4))  & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2))  & 0x1249249249249249ULL;
+    return v;
+}
+__host__ __device__ inline uint32_t compact3_21(uint64_t v) {
+    v &= 0x1249249249249249ULL;
+    v = (v ^ (v >> 2))  & 0x10c30c30c30c30c3ULL;
+    v = (v ^ (v >> 4))  & 0x100f00f00f00f00fULL;
+    v = (v ^ (v >> 8))  & 0x1f0000ff0000ffULL;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffULL;
+    v = (v ^ (v >> 32)) & 0x1fffffULL;
+    return (uint32_t)v;
+}
+__host__ __device__ inline uint64_t morton_fixed(uint32_t ux, uint32_t uy, uint32_t uz) {
+    return spread3_21(ux >> 11) | (spread3_21(uy >> 11) << 1) | (spread3_21(uz >> 11) << 2);
+}
+
+// Split-function coefficients for the FP32 walk.  The split factor is
+// S(x) = exp(-x^2) * P(t), t = x * tscale - 1, x = rho / (2 r_s), with P a
+// degree-kPolyDeg polynomial (Horner in t) interpolating erfcx(x) [+ 2x/sqrt(pi)].
+struct SplitPoly {
+    float c[kPolyDeg + 1];
+};
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+    jtp_params p{};
+    double m_plus = 1.0, m_minus = 1.0;   // |m| per species
+    int logG = 0;
+    std::string err;
+    int device = 0;
+
+    // capacities (elements)
+    int64_t cap_n = 0;
+    // particle-side buffers
+    uint64_t *keys0 = nullptr, *keys1 = nullptr;
+    int32_t *vals0 = nullptr, *vals1 = nullptr;
+    uint32_t *hist = nullptr;        // 256 * nblocks
+    uint32_t *hist_blk = nullptr;    // scan partials
+    uint4 *parts = nullptr;          // sorted records: fixed xyz + species
+    // tree
+    uint4 *node_geo = nullptr;       // 2 per internal node
+    int2 *node_kids = nullptr;
+    int32_t *parent_int = nullptr;   // parent of internal node
+    int32_t *parent_leaf = nullptr;  // parent of leaf
+    int32_t *flags = nullptr;        // bottom-up arrival counters
+    void *node_sum = nullptr;        // NodeSum per internal node
+    void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
+    int2 *groups = nullptr;
+    int32_t *cell4 = nullptr;        // level-4 cell offsets (4097)
+    int32_t *dev_counters = nullptr; // [0] n_groups, [1] error flag, [2] walk queue head
+    unsigned long long *dev_inter = nullptr;  // interaction counter
+    // PM
+    int32_t *cell_start = nullptr;   // G^3 + 1
+    void *rho = nullptr;             // real G^3 deposited mass per cell
+    void *phi = nullptr;             // real G^3 potential
+    void *spec = nullptr;            // complex G^2 (G/2+1)
+    void *twiddle = nullptr;         // complex G  e^{-2 pi i m/G}
+    void *gtab = nullptr;            // real 2G: per-axis window/Gaussian factor, k_i^2
+    void *pm_acc = nullptr;          // real 3n (sorted order)
+    SplitPoly poly{};
+    float poly_tscale = 0.0f;
+    double green_norm = 0.0;
+    unsigned long long* host_pinned = nullptr;  // [0] interactions, [1] (n_groups | err << 32)
+
+    int64_t last_n = -1;
+    bool have_result = false;
+    hipEvent_t ev[8]{};
+    bool events_ok = false;
+    hipStream_t last_stream = nullptr;
+    int64_t n_groups_max = 0;
+
+    size_t real_size() const { return p.precision == JTP_FP64 ? 8 : 4; }
+    int64_t G() const { return p.n_cells; }
+    int64_t G3() const { return (int64_t)p.n_cells * p.n_cells * p.n_cells; }
+    int64_t Hc() const { return p.n_cells / 2 + 1; }
+};
+
+// ---------------------------------------------------------------- launchers
+// All return hipError_t from hipGetLastError after the launch(es).
+hipError_t launch_fixed_from_f64(const double* pos, uint32_t* out, int64_t n3, double L, hipStream_t s);
+hipError_t launch_fixed_from_f32(const float* pos, uint32_t* out, int64_t n3, double L, hipStream_t s);
+hipError_t launch_morton_keys(const uint32_t* pos, uint64_t* keys, int32_t* vals, int64_t n, hipStream_t s);
+hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, float box_half,
+                                  float inv_cell, hipStream_t s);
+hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s);   // keys0/vals0 -> keys0/vals0 (sorted)
+hipError_t launch_gather_parts(const uint32_t* pos, const int8_t* species, const int32_t* perm,
+                               uint4* parts, int64_t n, hipStream_t s);
+
+hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s);            // Karras + bottom-up + mono + groups
+hipError_t launch_fill_i32(int32_t* out, int64_t m, int32_t v, hipStream_t s);
+hipError_t launch_cell_start(const uint64_t* keys, int64_t n, int shift, int64_t ncells, int32_t* out,
+                             hipStream_t s);
+
+hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s);           // grid <- sum q W
+hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s);  // phi <- Green * rho
+hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s);            // pm_acc <- -sigma grad phi
+hipError_t pm_tables(Ctx& c);                                      // twiddles + Green tables (sync)
+
+hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
+hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
+
+// host-side: FP32 split polynomial for the context's split mode and x range
+void make_split_poly(Ctx& c);
+
+}  // namespace jtp

@@ -1,0 +1,412 @@
+// Long-range (PM) part: deterministic CIC deposit of rho+ - rho-, 3-D real FFT
+// Poisson solve with the split Green function, gradient + CIC interpolation.
+//
+// Reference behaviour replaced:
+//   cic_scatter (FP64 CAS atomics into two grids), nbody_gpu_twopass.rs:85-158,
+//     and PmGrid::assign_mass, src/treepm/pm_grid.rs:63-103 -> k_deposit: one net grid
+//     rho+ - rho- gathered per node from Morton-sorted particles; no float atomics.
+//   cufft_solve_device_filtered (cuFFT D2Z / apply_green_kernel / Z2D / normalize),
+//     cuda/cufft_wrapper.cu:148-208,290-295,350-398, and the rustfft passes of
+//     PmGrid::solve_poisson_filtered, pm_grid.rs:132-271 -> five LDS Stockham passes
+//     (R2C x, y, z with the Green multiply fused between forward and inverse z, y, C2R x).
+//   cic_gather (nbody_gpu_twopass.rs:162-257) and PmGrid::interpolate_force
+//     (pm_grid.rs:275-359) -> k_interp on the single net potential (Phi- = -Phi+).
+//
+// Grid layout: real[z][y][x] (x fastest; index i + G(j + G k) as in pm_grid.rs:97),
+// half spectrum complex[z][y][kx], kx in [0, G/2].
+#include "jtp_internal.h"
+#include "jtp_device.h"
+
+#include <cmath>
+#include <vector>
+
+namespace jtp {
+
+template <typename R>
+__device__ inline R frac_of(uint32_t u, int logG) {
+    const uint32_t fb = 32 - logG;
+    const uint32_t m = (fb >= 32) ? 0xffffffffu : ((1u << fb) - 1u);
+    return (R)(u & m) * (R)(1.0 / (double)(1ull << fb));
+}
+
+// ------------------------------------------------------------------ deposit
+// Thread per grid node, nodes enumerated in Morton order so that the eight
+// source cells of a wavefront's nodes are contiguous particle runs.
+template <typename R>
+__global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __restrict__ cs, int logG,
+                          R qp, R qm, R* __restrict__ rho) {
+    const int64_t G = 1ll << logG;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= G * G * G) return;
+    const int32_t i = (int32_t)compact3_21((uint64_t)m);
+    const int32_t j = (int32_t)compact3_21((uint64_t)m >> 1);
+    const int32_t k = (int32_t)compact3_21((uint64_t)m >> 2);
+    const int32_t msk = (int32_t)(G - 1);
+    R acc = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 0; ++dz) {
+#pragma unroll
+        for (int dy = -1; dy <= 0; ++dy) {
+#pragma unroll
+            for (int dx = -1; dx <= 0; ++dx) {
+                const uint32_t ci = (uint32_t)((i + dx) & msk), cj = (uint32_t)((j + dy) & msk),
+                               ck = (uint32_t)((k + dz) & msk);
+                const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
+                const int32_t s = cs[cm], e = cs[cm + 1];
+                for (int32_t p = s; p < e; ++p) {
+                    const uint4 r = parts[p];
+                    const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
+                    const R wx = dx == 0 ? (R)1 - fx : fx;
+                    const R wy = dy == 0 ? (R)1 - fy : fy;
+                    const R wz = dz == 0 ? (R)1 - fz : fz;
+                    const R q = (int32_t)r.w > 0 ? qp : qm;
+                    acc += q * (wx * wy * wz);
+                }
+            }
+        }
+    }
+    rho[((int64_t)k * G + j) * G + i] = acc;
+}
+
+// ------------------------------------------------------------------ FFT core
+// In-place radix-2 Stockham over `nlines` lines of length G held in LDS with
+// element (line l, point p) at buf[l * SL + p * SP].  Each stage reads its
+// butterflies into registers, synchronises, and writes them back in autosorted
+// order (Stockham): out[2j - k] = a + w b, out[2j - k + Ns] = a - w b with
+// k = j mod Ns and w = exp(-+2 pi i k / (2 Ns)) from the table tw[m] = e^{-2 pi i m / G}.
+template <typename R, int PER>
+__device__ inline void lds_fft(Cplx<R>* buf, int G, int logG, int nlines, int SL, int SP, bool line_fast,
+                               const Cplx<R>* __restrict__ tw, bool inv) {
+    const int halfG = G >> 1;
+    const int tid = threadIdx.x;
+    for (int ls = 0; ls < logG; ++ls) {
+        const int Ns = 1 << ls;
+        Cplx<R> va[PER], vb[PER];
+        int ln[PER], jj[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = tid + u * 256;
+            int l, j;
+            if (line_fast) {
+                l = q % nlines;
+                j = q / nlines;
+            } else {
+                l = q / halfG;
+                j = q % halfG;
+            }
+            ln[u] = l;
+            jj[u] = j;
+            Cplx<R> a = buf[l * SL + j * SP];
+            Cplx<R> b = buf[l * SL + (j + halfG) * SP];
+            const int k = j & (Ns - 1);
+            Cplx<R> w = tw[(k << (logG - 1 - ls))];
+            if (inv) w.y = -w.y;
+            const Cplx<R> wb = {w.x * b.x - w.y * b.y, w.x * b.y + w.y * b.x};
+            va[u] = {a.x + wb.x, a.y + wb.y};
+            vb[u] = {a.x - wb.x, a.y - wb.y};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int j = jj[u];
+            const int k = j & (Ns - 1);
+            const int o = 2 * j - k;
+            buf[ln[u] * SL + o * SP] = va[u];
+            buf[ln[u] * SL + (o + Ns) * SP] = vb[u];
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kFftPer = kFftPoints / 2 / 256;  // butterflies per thread per stage
+
+// X forward (R2C): pairs of real lines (y even, y odd) packed as one complex line.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cplx<R>* __restrict__ out, int logG,
+                                                   const Cplx<R>* __restrict__ tw) {
+    __shared__ Cplx<R> buf[kFftPoints];
+    const int G = 1 << logG, H = G / 2 + 1;
+    const int npair = kFftPoints / G;              // complex lines per block
+    const int64_t line0 = (int64_t)blockIdx.x * npair * 2;  // first real line
+    const R* src = in + line0 * G;
+    for (int e = threadIdx.x; e < 2 * kFftPoints; e += 256) {
+        const int line = e / G, x = e % G;
+        const R v = src[e];
+        if (line & 1) buf[(line >> 1) * G + x].y = v; else buf[(line >> 1) * G + x].x = v;
+    }
+    __syncthreads();
+    lds_fft<R, kFftPer>(buf, G, logG, npair, G, 1, false, tw, false);
+    Cplx<R>* dst = out + line0 * H;
+    for (int e = threadIdx.x; e < 2 * npair * H; e += 256) {
+        const int row = e / H, k = e % H;
+        const int p = row >> 1;
+        const Cplx<R> z = buf[p * G + k];
+        const Cplx<R> zc = buf[p * G + ((G - k) & (G - 1))];
+        Cplx<R> o;
+        if ((row & 1) == 0) {  // A = (Z[k] + conj Z[G-k]) / 2
+            o = {(R)0.5 * (z.x + zc.x), (R)0.5 * (z.y - zc.y)};
+        } else {               // B = (Z[k] - conj Z[G-k]) / (2i)
+            o = {(R)0.5 * (z.y + zc.y), (R)0.5 * (zc.x - z.x)};
+        }
+        dst[e] = o;
+    }
+}
+
+// X inverse (C2R): two half-spectrum rows A, B -> Z = A_full + i B_full -> inverse FFT -> (a, b).
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ in, R* __restrict__ out, int logG,
+                                                   const Cplx<R>* __restrict__ tw) {
+    __shared__ Cplx<R> buf[kFftPoints];
+    const int G = 1 << logG, H = G / 2 + 1;
+    const int npair = kFftPoints / G;
+    const int64_t line0 = (int64_t)blockIdx.x * npair * 2;
+    const Cplx<R>* src = in + line0 * H;
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int p = e / G, k = e % G;
+        const Cplx<R>* ra = src + (int64_t)(2 * p) * H;
+        const Cplx<R>* rb = ra + H;
+        Cplx<R> a, b;
+        if (k <= G / 2) {
+            a = ra[k];
+            b = rb[k];
+            if (k == 0 || k == G / 2) {
+                a.y = 0;
+                b.y = 0;
+            }
+        } else {
+            a = ra[G - k];
+            b = rb[G - k];
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        buf[p * G + k] = {a.x - b.y, a.y + b.x};
+    }
+    __syncthreads();
+    lds_fft<R, kFftPer>(buf, G, logG, npair, G, 1, false, tw, true);
+    R* dst = out + line0 * G;
+    for (int e = threadIdx.x; e < 2 * kFftPoints; e += 256) {
+        const int line = e / G, x = e % G;
+        const Cplx<R> z = buf[(line >> 1) * G + x];
+        dst[e] = (line & 1) ? z.y : z.x;
+    }
+}
+
+// Y pass (complex, forward or inverse): block = (z, block of KB consecutive kx).
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, int logG, const Cplx<R>* __restrict__ tw,
+                                               int inv) {
+    __shared__ Cplx<R> buf[kFftPoints];
+    const int G = 1 << logG, H = G / 2 + 1;
+    const int KB = kFftPoints / G;
+    const int nkb = (H + KB - 1) / KB;
+    const int z = blockIdx.x / nkb, kx0 = (blockIdx.x % nkb) * KB;
+    Cplx<R>* base = spec + (int64_t)z * G * H;
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int y = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        buf[y * KB + jb] = (kx < H) ? base[(int64_t)y * H + kx] : Cplx<R>{0, 0};
+    }
+    __syncthreads();
+    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, inv != 0);
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int y = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        if (kx < H) base[(int64_t)y * H + kx] = buf[y * KB + jb];
+    }
+}
+
+// Z pass with the Green function: forward z FFT, multiply, inverse z FFT.
+// green(kx, ky, kz) = norm * T[kx] T[ky] T[kz] / (K2[kx] + K2[ky] + K2[kz]); zero for DC and
+// for max(|kx|,|ky|,|kz|) < k_min (apply_green_kernel, cufft_wrapper.cu:160-207).
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec, int logG,
+                                                     const Cplx<R>* __restrict__ tw, const R* __restrict__ gtab,
+                                                     R norm, int k_min) {
+    __shared__ Cplx<R> buf[kFftPoints];
+    const int G = 1 << logG, H = G / 2 + 1;
+    const int KB = kFftPoints / G;
+    const int nkb = (H + KB - 1) / KB;
+    const int y = blockIdx.x / nkb, kx0 = (blockIdx.x % nkb) * KB;
+    const int64_t zs = (int64_t)G * H;
+    Cplx<R>* base = spec + (int64_t)y * H;
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int z = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        buf[z * KB + jb] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
+    }
+    __syncthreads();
+    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, false);
+    const R* T = gtab;
+    const R* K2 = gtab + G;
+    const int ky = y;
+    const int aky = ky <= G / 2 ? ky : G - ky;
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int kz = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        const int akz = kz <= G / 2 ? kz : G - kz;
+        const int akx = kx;
+        const int kmax = max(akx, max(aky, akz));
+        R f = 0;
+        if (kx < H && !(akx == 0 && aky == 0 && akz == 0) && kmax >= k_min)
+            f = norm * T[kx] * T[ky] * T[kz] / (K2[kx] + K2[ky] + K2[kz]);
+        Cplx<R> v = buf[kz * KB + jb];
+        buf[kz * KB + jb] = {v.x * f, v.y * f};
+    }
+    __syncthreads();
+    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, true);
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int z = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        if (kx < H) base[z * zs + kx] = buf[z * KB + jb];
+    }
+}
+
+template <typename R>
+static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s) {
+    const int G = (int)c.G(), H = G / 2 + 1, logG = c.logG;
+    const int npair = kFftPoints / G;
+    const int KB = kFftPoints / G;
+    const int nkb = (H + KB - 1) / KB;
+    const unsigned nx = (unsigned)((int64_t)G * G / (2 * npair));
+    const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
+    Cplx<R>* spec = (Cplx<R>*)c.spec;
+    k_fft_x_r2c<R><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, logG, tw);
+    k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 0);
+    k_fft_z_green<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, (const R*)c.gtab,
+                                                                     (R)c.green_norm, c.p.k_min);
+    k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 1);
+    k_fft_x_c2r<R><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, logG, tw);
+    return hipGetLastError();
+}
+
+hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s) {
+    if (c.p.precision == JTP_FP64) return fft_solve_t<double>(c, (const double*)rho, (double*)phi, s);
+    return fft_solve_t<float>(c, (const float*)rho, (float*)phi, s);
+}
+
+hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
+    const int64_t G3 = c.G3();
+    hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * c.logG, G3, c.cell_start, s);
+    if (e != hipSuccess) return e;
+    const unsigned nb = (unsigned)((G3 + kBlock - 1) / kBlock);
+    if (c.p.precision == JTP_FP64)
+        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.logG, c.m_plus, -c.m_minus,
+                                                            (double*)c.rho);
+    else
+        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.logG, (float)c.m_plus,
+                                                           (float)-c.m_minus, (float*)c.rho);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ interpolation
+// a_i = -sigma_i sum_corners W * grad(phi)(corner), grad by the 4-point (or 2-point) stencil.
+template <typename R, bool FOUR>
+__device__ inline R dstencil(const R* __restrict__ phi, int64_t b, int64_t st, int c, int G, R inv) {
+    // phi along one axis: b = index with that axis coordinate removed, st = stride, c = coordinate
+    const int msk = G - 1;
+    const R p1 = phi[b + (int64_t)((c + 1) & msk) * st], m1 = phi[b + (int64_t)((c - 1) & msk) * st];
+    if (FOUR) {
+        const R p2 = phi[b + (int64_t)((c + 2) & msk) * st], m2 = phi[b + (int64_t)((c - 2) & msk) * st];
+        return ((R)8 * (p1 - m1) - (p2 - m2)) * inv;
+    }
+    return (p1 - m1) * inv;
+}
+
+template <typename R, bool FOUR>
+__global__ void k_interp(const uint4* __restrict__ parts, int64_t n, const R* __restrict__ phi, int logG, R inv,
+                         R* __restrict__ acc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int G = 1 << logG;
+    const int64_t G2 = (int64_t)G * G;
+    const uint4 r = parts[i];
+    const int sh = 32 - logG;
+    const int ix = (int)(r.x >> sh), iy = (int)(r.y >> sh), iz = (int)(r.z >> sh);
+    const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
+    const R wx[2] = {(R)1 - fx, fx}, wy[2] = {(R)1 - fy, fy}, wz[2] = {(R)1 - fz, fz};
+    R ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz) {
+        const int cz = (iz + dz) & (G - 1);
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const int cy = (iy + dy) & (G - 1);
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = (ix + dx) & (G - 1);
+                const R w = wx[dx] * wy[dy] * wz[dz];
+                const R gx = dstencil<R, FOUR>(phi, (int64_t)cz * G2 + (int64_t)cy * G, 1, cx, G, inv);
+                const R gy = dstencil<R, FOUR>(phi, (int64_t)cz * G2 + cx, G, cy, G, inv);
+                const R gz = dstencil<R, FOUR>(phi, (int64_t)cy * G + cx, G2, cz, G, inv);
+                ax += w * gx;
+                ay += w * gy;
+                az += w * gz;
+            }
+        }
+    }
+    const R sg = (int32_t)r.w > 0 ? (R)-1 : (R)1;
+    acc[3 * i] = sg * ax;
+    acc[3 * i + 1] = sg * ay;
+    acc[3 * i + 2] = sg * az;
+}
+
+hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s) {
+    if (n == 0) return hipGetLastError();
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    const double h = c.p.box_size / (double)c.G();
+    const bool four = c.p.gradient == JTP_GRAD_4PT;
+    const double inv = four ? 1.0 / (12.0 * h) : 1.0 / (2.0 * h);
+    if (c.p.precision == JTP_FP64) {
+        if (four) k_interp<double, true><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const double*)c.phi, c.logG, inv, (double*)c.pm_acc);
+        else k_interp<double, false><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const double*)c.phi, c.logG, inv, (double*)c.pm_acc);
+    } else {
+        if (four) k_interp<float, true><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const float*)c.phi, c.logG, (float)inv, (float*)c.pm_acc);
+        else k_interp<float, false><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const float*)c.phi, c.logG, (float)inv, (float*)c.pm_acc);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ tables (host, FP64)
+// Twiddles e^{-2 pi i m/G}; per-axis Green factors T[i] = exp(-k_i^2 r_s^2) / sinc(k_i h/2)^4
+// (the sinc^4 only with deconvolution) and K2[i] = k_i^2, k_i = 2 pi i'/L with i' the signed
+// frequency; norm = -4 pi G / (V_cell G^3) (density = mass/V_cell, unnormalised inverse FFT).
+hipError_t pm_tables(Ctx& c) {
+    const int G = (int)c.G();
+    const double L = c.p.box_size, h = L / G;
+    std::vector<double> tw(2 * G), tab(2 * G);
+    for (int m = 0; m < G; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)G;
+        tw[2 * m] = std::cos(a);
+        tw[2 * m + 1] = std::sin(a);
+    }
+    const double dk = 2.0 * M_PI / L;
+    for (int i = 0; i < G; ++i) {
+        const int is = i <= G / 2 ? i : i - G;
+        const double k = dk * is;
+        double t = std::exp(-k * k * c.p.r_s * c.p.r_s);
+        if (c.p.deconvolve && is != 0) {
+            const double x = 0.5 * k * h;
+            const double sc = std::sin(x) / x;
+            t /= sc * sc * sc * sc;
+        }
+        tab[i] = t;
+        tab[G + i] = k * k;
+    }
+    c.green_norm = -4.0 * M_PI * c.p.g_constant / (h * h * h * (double)G * (double)G * (double)G);
+    hipError_t e;
+    if (c.p.precision == JTP_FP64) {
+        e = hipMemcpy(c.twiddle, tw.data(), 2 * G * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c.gtab, tab.data(), 2 * G * sizeof(double), hipMemcpyHostToDevice);
+    } else {
+        std::vector<float> twf(2 * G), tabf(2 * G);
+        for (int m = 0; m < 2 * G; ++m) {
+            twf[m] = (float)tw[m];
+            tabf[m] = (float)tab[m];
+        }
+        e = hipMemcpy(c.twiddle, twf.data(), 2 * G * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c.gtab, tabf.data(), 2 * G * sizeof(float), hipMemcpyHostToDevice);
+    }
+    return e;
+}
+
+}  // namespace jtp

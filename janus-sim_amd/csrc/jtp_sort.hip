@@ -1,0 +1,284 @@
+// Fixed-point conversion, Morton keys and a stable LSD radix sort of
+// (63-bit Morton key, int32 particle index) pairs for gfx950.
+//
+// Reference behaviour replaced: compute_morton_all + radix_histogram /
+// radix_prefix_sum / radix_scatter + reorder_by_idx_* (src/nbody_gpu_twopass.rs:
+// 1447-1658).  The reference sort is a stable LSD radix sort, so its output is
+// the unique stable argsort of the keys; this one is too, with a parallel
+// digit-major scan instead of the reference's serial per-bucket prefix loop and
+// an LDS-staged tile reorder so that the scatter writes contiguous runs.
+#include "jtp_internal.h"
+#include "jtp_device.h"
+
+namespace jtp {
+
+// ------------------------------------------------------------------ fixed point
+__device__ inline uint32_t to_fixed(double x, double L) {
+    double t = x / L + 0.5;
+    t -= floor(t);
+    uint64_t w = (uint64_t)(t * 4294967296.0);
+    if (w >= 4294967296ULL) w -= 4294967296ULL;
+    return (uint32_t)w;
+}
+
+__global__ void k_fixed_from_f64(const double* __restrict__ pos, uint32_t* __restrict__ out, int64_t n3,
+                                 double L) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) out[i] = to_fixed(pos[i], L);
+}
+__global__ void k_fixed_from_f32(const float* __restrict__ pos, uint32_t* __restrict__ out, int64_t n3,
+                                 double L) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) out[i] = to_fixed((double)pos[i], L);
+}
+
+hipError_t launch_fixed_from_f64(const double* pos, uint32_t* out, int64_t n3, double L, hipStream_t s) {
+    if (n3 > 0)
+        k_fixed_from_f64<<<dim3((unsigned)((n3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, out, n3, L);
+    return hipGetLastError();
+}
+hipError_t launch_fixed_from_f32(const float* pos, uint32_t* out, int64_t n3, double L, hipStream_t s) {
+    if (n3 > 0)
+        k_fixed_from_f32<<<dim3((unsigned)((n3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, out, n3, L);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Morton keys
+// Native keys: the top 21 bits of each fixed-point axis, interleaved (x in bit 0).
+__global__ void k_morton_keys(const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys,
+                              int32_t* __restrict__ vals, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = morton_fixed(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    vals[i] = (int32_t)i;
+}
+
+// Reference-compatible keys from f32 positions (compute_morton_all, nbody_gpu_twopass.rs:1458-1484):
+// ix = (unsigned) fminf(fmaxf((x + box_half) * inv_cell, 0), 2^21 - 1), same interleave.
+__global__ void k_morton_keys_f32(const float* __restrict__ pos, uint64_t* __restrict__ keys, int64_t n,
+                                  float box_half, float inv_cell) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float g = __fmul_rn(__fadd_rn(pos[3 * i + a], box_half), inv_cell);
+        c[a] = (uint32_t)fminf(fmaxf(g, 0.0f), 2097151.0f);
+    }
+    keys[i] = spread3_21(c[0]) | (spread3_21(c[1]) << 1) | (spread3_21(c[2]) << 2);
+}
+
+hipError_t launch_morton_keys(const uint32_t* pos, uint64_t* keys, int32_t* vals, int64_t n, hipStream_t s) {
+    if (n > 0) k_morton_keys<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, keys, vals, n);
+    return hipGetLastError();
+}
+hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, float box_half, float inv_cell,
+                                  hipStream_t s) {
+    if (n > 0)
+        k_morton_keys_f32<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, keys, n,
+                                                                                              box_half, inv_cell);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ radix sort
+// Pass structure (8 passes x 8 bits, stable):
+//   k_radix_hist    per-tile digit counts, stored digit-major: hist[d * nblk + b]
+//   k_scan_chunks   exclusive scan of 2048-entry chunks of hist, chunk totals -> part[]
+//   k_scan_part     exclusive scan of part[] (one workgroup)
+//   k_radix_scatter stable in-tile ranks (wavefront bit-match + per-wave LDS counters),
+//                   LDS reorder of the tile by digit, contiguous global writes.
+// Digit-major exclusive prefix of the counts is exactly the global start of
+// digit d's run for tile b.
+
+constexpr int kScanChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                    uint32_t* __restrict__ hist, int nblk) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortIPT; ++r) {
+        int64_t idx = base + r * 256 + tid;
+        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)tid * nblk + blockIdx.x] = h[tid];
+}
+
+__global__ __launch_bounds__(256) void k_scan_chunks(uint32_t* __restrict__ data, int64_t m,
+                                                     uint32_t* __restrict__ part) {
+    __shared__ uint32_t lds[4];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + tid * 8;
+    uint32_t v[8], s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        v[i] = (base + i < m) ? data[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t off = block256_excl_scan(s, lds, &tot);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (base + i < m) data[base + i] = off;
+        off += v[i];
+    }
+    if (tid == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_part(uint32_t* __restrict__ part, int64_t np) {
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t b0 = 0; b0 < np; b0 += 256) {
+        int64_t i = b0 + threadIdx.x;
+        uint32_t v = (i < np) ? part[i] : 0u;
+        uint32_t tot;
+        uint32_t off = block256_excl_scan(v, lds, &tot);
+        uint32_t c = carry;
+        if (i < np) part[i] = c + off;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restrict__ kin,
+                                                       const int32_t* __restrict__ vin,
+                                                       uint64_t* __restrict__ kout, int32_t* __restrict__ vout,
+                                                       int64_t n, int shift, const uint32_t* __restrict__ hist,
+                                                       const uint32_t* __restrict__ part, int nblk) {
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t run[256];
+    __shared__ int64_t gbase[256];
+    __shared__ uint32_t scan_lds[4];
+    __shared__ uint64_t skeys[kSortTile];
+    __shared__ int32_t svals[kSortTile];
+
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int b = blockIdx.x;
+    const int64_t base = (int64_t)b * kSortTile;
+    const uint64_t lt = lanemask_lt();
+
+    uint64_t key[kSortIPT];
+    int32_t val[kSortIPT];
+    uint32_t rk[kSortIPT];
+#pragma unroll
+    for (int r = 0; r < kSortIPT; ++r) {
+        int64_t idx = base + r * 256 + tid;
+        if (idx < n) {
+            key[r] = kin[idx];
+            val[r] = vin[idx];
+        } else {
+            key[r] = 0;
+            val[r] = 0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) wcnt[i][tid] = 0;
+    run[tid] = 0;
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kSortIPT; ++r) {
+        const bool valid = base + r * 256 + tid < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool bs = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(bs);
+            peers &= bs ? bb : ~bb;
+        }
+        const uint32_t lr = (uint32_t)__popcll(peers & lt);
+        if (valid && lr == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + lr;
+            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
+            rk[r] = pos;
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        wcnt[0][tid] = 0;
+        wcnt[1][tid] = 0;
+        wcnt[2][tid] = 0;
+        wcnt[3][tid] = 0;
+        __syncthreads();
+    }
+    // tile offsets per digit and global bases
+    uint32_t toff = block256_excl_scan(run[tid], scan_lds, nullptr);
+    {
+        const int64_t hidx = (int64_t)tid * nblk + b;
+        const int64_t g = (int64_t)hist[hidx] + (int64_t)part[hidx / kScanChunk];
+        gbase[tid] = g - (int64_t)toff;
+    }
+    run[tid] = toff;  // reuse as tile offset table
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortIPT; ++r) {
+        if (base + r * 256 + tid < n) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+            const uint32_t l = run[d] + rk[r];
+            skeys[l] = key[r];
+            svals[l] = val[r];
+        }
+    }
+    __syncthreads();
+    const int64_t cnt = (n - base) < kSortTile ? (n - base) : kSortTile;
+    for (int l = tid; l < cnt; l += 256) {
+        const uint64_t k = skeys[l];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const int64_t o = gbase[d] + l;
+        kout[o] = k;
+        vout[o] = svals[l];
+    }
+    (void)lane;
+}
+
+hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s) {
+    if (n <= 1) return hipGetLastError();
+    const int nblk = (int)((n + kSortTile - 1) / kSortTile);
+    const int64_t m = (int64_t)nblk * 256;
+    const int64_t nchunk = (m + kScanChunk - 1) / kScanChunk;
+    uint64_t* ka = c.keys0;
+    uint64_t* kb = c.keys1;
+    int32_t* va = c.vals0;
+    int32_t* vb = c.vals1;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = pass * 8;
+        k_radix_hist<<<dim3(nblk), dim3(256), 0, s>>>(ka, n, shift, c.hist, nblk);
+        k_scan_chunks<<<dim3((unsigned)nchunk), dim3(256), 0, s>>>(c.hist, m, c.hist_blk);
+        k_scan_part<<<dim3(1), dim3(256), 0, s>>>(c.hist_blk, nchunk);
+        k_radix_scatter<<<dim3(nblk), dim3(256), 0, s>>>(ka, va, kb, vb, n, shift, c.hist, c.hist_blk, nblk);
+        uint64_t* tk = ka; ka = kb; kb = tk;
+        int32_t* tv = va; va = vb; vb = tv;
+    }
+    // 8 passes: the result is back in keys0/vals0
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ particle gather
+// parts[i] = (x, y, z, species) of the i-th particle in key order (reorder_by_idx_*,
+// nbody_gpu_twopass.rs:1490-1514, fused into one 16-byte record).
+__global__ void k_gather_parts(const uint32_t* __restrict__ pos, const int8_t* __restrict__ species,
+                               const int32_t* __restrict__ perm, uint4* __restrict__ parts, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = perm[i];
+    parts[i] = make_uint4(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], (uint32_t)(int32_t)species[p]);
+}
+
+hipError_t launch_gather_parts(const uint32_t* pos, const int8_t* species, const int32_t* perm, uint4* parts,
+                               int64_t n, hipStream_t s) {
+    if (n > 0)
+        k_gather_parts<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, species, perm,
+                                                                                           parts, n);
+    return hipGetLastError();
+}
+
+}  // namespace jtp

@@ -1,0 +1,414 @@
+// Short-range group walk.
+//
+// Reference behaviour replaced: forces_treepm_short_range (src/nbody_gpu_twopass.rs:1670-1811),
+// a per-warp shared-stack walk run once per species tree, and the CPU recursion
+// TreePMTree::acc_recursive + pairwise_acc_with_split (src/treepm/tree_short.rs:172-247).
+//
+// One wavefront owns one group of <= 64 Morton-consecutive targets.  The
+// traversal is lane-parallel: each step pops up to 64 nodes from a per-wave LDS
+// stack, every lane classifies one node against the group's bounding box
+// (prune beyond r_cut / accept the two monopoles / take the bucket's particles /
+// open), and a wavefront prefix sum appends the accepted sources to an LDS
+// interaction list while a ballot pushes the opened nodes' children.  When the
+// list is long enough every lane evaluates its target against all entries
+// (broadcast LDS reads).  Sources are stored relative to the group centre, so
+// the pair displacement is a plain subtraction; periodic minimum image comes
+// from wrapping 32-bit fixed-point differences when the entry is built.
+//
+// Pair law (per entry j with signed charge q_j = sigma_j G |m_j|):
+//   a_i += sigma_i q_j S(.) (x_j - x_i) / (r^2 + eps^2)^{3/2},  zero unless r^2 < r_cut^2
+// Same species attract, opposite species repel (tree_short.rs:238, nbody_gpu_twopass.rs:1711).
+#include "jtp_internal.h"
+#include "jtp_device.h"
+
+namespace jtp {
+
+constexpr int kStackCap = 640;          // logical stack bound per wave
+constexpr int kStackPhys = kStackCap + 192;
+constexpr int kListCap = 512;
+constexpr int kFlushAt = 256;
+
+struct WalkArgs {
+    const uint4* parts;
+    const uint4* geo;
+    const int2* kids;
+    const void* mono;
+    const int2* groups;
+    int32_t* counters;  // [0] n_groups, [1] error flag, [2] queue head
+    unsigned long long* inter;
+    const void* pm_acc;  // sorted order, nullptr when the PM part is disabled
+    const int32_t* perm;
+    void* out;
+    int64_t n;
+    int32_t root;
+    int32_t leaf;
+    int64_t max_steps;
+    double scale;   // length per fixed-point unit, L / 2^32
+    double eps2;
+    double rcut2;
+    double theta2;
+    double inv2rs;  // 1 / (2 r_s)
+    double qp, qm;  // signed particle charges sigma G |m| for species +1 / -1
+    SplitPoly poly;
+    float tscale;   // t = rho * tscale - 1
+    float escale;   // exp(-x^2) = exp2(r2s * escale)
+};
+
+__device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+template <typename R> struct EntryT;
+template <> struct EntryT<float> { using T = float4; };
+template <> struct EntryT<double> { using T = double4; };
+
+// Pair evaluation: returns the force factor f such that a += f * d.
+template <typename R, int MODE>
+struct Pair;
+
+template <int MODE>
+struct Pair<float, MODE> {
+    __device__ static inline float f(float q, float r2, const WalkArgs& a, float eps2, float rcut2) {
+        const float r2s = r2 + eps2;
+        const float inv = __builtin_amdgcn_rsqf(r2s);
+        const float inv3 = inv * inv * inv;
+        float S;
+        if (MODE == JTP_SPLIT_POLY4) {
+            const float u = r2 * (1.0f / rcut2);
+            S = 1.0f - u * u;
+        } else {
+            const float rp = r2s * inv;
+            const float t = rp * a.tscale - 1.0f;
+            float P = a.poly.c[kPolyDeg];
+#pragma unroll
+            for (int d = kPolyDeg - 1; d >= 0; --d) P = P * t + a.poly.c[d];
+            S = __builtin_amdgcn_exp2f(r2s * a.escale) * P;
+        }
+        const float fv = q * S * inv3;
+        return r2 < rcut2 ? fv : 0.0f;
+    }
+};
+
+template <int MODE>
+struct Pair<double, MODE> {
+    __device__ static inline double f(double q, double r2, const WalkArgs& a, double eps2, double rcut2) {
+        const double r2s = r2 + eps2;
+        const double inv = rsqrt(r2s);
+        const double inv3 = inv * inv * inv;
+        double S;
+        if (MODE == JTP_SPLIT_POLY4) {
+            const double u = r2 / rcut2;
+            S = 1.0 - u * u;
+        } else {
+            const double x = r2s * inv * a.inv2rs;
+            S = erfc(x);
+            if (MODE == JTP_SPLIT_ERFC_GAUSS) S += 1.1283791670955126 * x * exp(-x * x);
+        }
+        const double fv = q * S * inv3;
+        return r2 < rcut2 ? fv : 0.0;
+    }
+};
+
+template <typename R>
+__device__ inline R fixed_rel(uint32_t u, uint32_t c, R scale) {
+    return (R)(int32_t)(u - c) * scale;
+}
+
+// Monopole entry (relative to the group centre gc).
+__device__ inline float4 mono_entry(const uint4* mono, int32_t node, int s, const uint32_t gc[3], float scale) {
+    const uint4 m = mono[2 * node + s];
+    return make_float4(fixed_rel<float>(m.x, gc[0], scale), fixed_rel<float>(m.y, gc[1], scale),
+                       fixed_rel<float>(m.z, gc[2], scale), __uint_as_float(m.w));
+}
+__device__ inline double wrap_fixed(double d) {
+    // d in fixed units, bring into [-2^31, 2^31)
+    return d - 4294967296.0 * rint(d * (1.0 / 4294967296.0));
+}
+__device__ inline double4 mono_entry(const double4* mono, int32_t node, int s, const uint32_t gc[3], double scale) {
+    const double4 m = mono[2 * node + s];
+    return make_double4(wrap_fixed(m.x - (double)gc[0]) * scale, wrap_fixed(m.y - (double)gc[1]) * scale,
+                        wrap_fixed(m.z - (double)gc[2]) * scale, m.w);
+}
+__device__ inline float mono_q(const uint4* mono, int32_t node, int s) { return __uint_as_float(mono[2 * node + s].w); }
+__device__ inline double mono_q(const double4* mono, int32_t node, int s) { return mono[2 * node + s].w; }
+
+template <typename R, int MODE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
+    using E = typename EntryT<R>::T;
+    using M = typename std::conditional<sizeof(R) == 4, uint4, double4>::type;
+    __shared__ E list_s[WAVES][kListCap];
+    __shared__ int32_t stack_s[WAVES][kStackPhys];
+
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    E* L = list_s[w];
+    int32_t* S = stack_s[w];
+    const M* mono = (const M*)a.mono;
+    const R scale = (R)a.scale;
+    const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2;
+    const R qpart[2] = {(R)a.qm, (R)a.qp};
+    const int maxapp = a.leaf > 2 ? a.leaf : 2;
+    const uint64_t lt = lanemask_lt();
+    unsigned long long my_inter = 0;
+    const int32_t n_groups = a.counters[0];
+
+    while (true) {
+        int g = 0;
+        if (lane == 0) g = atomicAdd(&a.counters[2], 1);
+        g = __shfl(g, 0, 64);
+        if (g >= n_groups) break;
+        const int2 gr = a.groups[g];
+        const bool valid = lane < gr.y;
+        const int64_t ti = (int64_t)gr.x + (valid ? lane : 0);
+        const uint4 rec = a.parts[ti];
+        const uint64_t vmask = __ballot(valid);
+        const int nvalid = __popcll(vmask);
+
+        uint32_t gmn[3], gmx[3], gc[3], gh[3];
+        {
+            const uint32_t p[3] = {rec.x, rec.y, rec.z};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                gmn[ax] = wave_min_u32(valid ? p[ax] : 0xffffffffu);
+                gmx[ax] = wave_max_u32(valid ? p[ax] : 0u);
+                gc[ax] = gmn[ax] + (gmx[ax] - gmn[ax]) / 2;
+                gh[ax] = (gmx[ax] - gmn[ax]) / 2 + ((gmx[ax] - gmn[ax]) & 1u);
+            }
+        }
+        const R tx = fixed_rel<R>(rec.x, gc[0], scale), ty = fixed_rel<R>(rec.y, gc[1], scale),
+                tz = fixed_rel<R>(rec.z, gc[2], scale);
+        R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
+
+        int sp = 1, lc = 0;
+        int64_t steps = 0;
+        if (lane == 0) S[0] = a.root;
+        wave_lds_fence();
+
+        auto flush = [&](int cnt) {
+            R px = 0, py = 0, pz = 0;
+            for (int j = 0; j < cnt; ++j) {
+                const E e = L[j];
+                const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const R f = Pair<R, MODE>::f(e.w, r2, a, eps2, rcut2);
+                px += f * dx;
+                py += f * dy;
+                pz += f * dz;
+            }
+            if (sizeof(R) == 4) {  // compensated accumulation of the chunk partials
+                R y = px - cx, t = sx + y;
+                cx = (t - sx) - y;
+                sx = t;
+                y = py - cy; t = sy + y; cy = (t - sy) - y; sy = t;
+                y = pz - cz; t = sz + y; cz = (t - sz) - y; sz = t;
+            } else {
+                sx += px;
+                sy += py;
+                sz += pz;
+            }
+            my_inter += (unsigned long long)cnt * (unsigned long long)nvalid;
+        };
+
+        while (sp > 0) {
+            if (++steps > a.max_steps) {  // a finite hierarchy never gets here; report instead of spinning
+                if (lane == 0) atomicExch(&a.counters[1], 2);
+                break;
+            }
+            int k = sp < 64 ? sp : 64;
+            const int kl = (kListCap - lc) / maxapp;
+            k = k < kl ? k : kl;
+            const int ks = kStackCap - sp > 1 ? kStackCap - sp : 1;
+            k = k < ks ? k : ks;
+            sp -= k;
+            const bool has = lane < k;
+            const int32_t node = has ? S[sp + lane] : 0;
+            wave_lds_fence();
+
+            int act = 0, napp = 0;
+            int32_t first = 0;
+            if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
+                if (lane == 0) atomicExch(&a.counters[1], 3);  // malformed child index: skip it
+            } else if (has) {
+                uint32_t nmn[3], nmx[3];
+                int32_t cnt;
+                if (node >= 0) {
+                    const uint4 g0 = a.geo[2 * node], g1 = a.geo[2 * node + 1];
+                    nmn[0] = g0.x; nmn[1] = g0.y; nmn[2] = g0.z;
+                    nmx[0] = g1.x; nmx[1] = g1.y; nmx[2] = g1.z;
+                    first = (int32_t)g0.w;
+                    cnt = (int32_t)g1.w - first + 1;
+                } else {
+                    first = ~node;
+                    const uint4 pr = a.parts[first];
+                    nmn[0] = nmx[0] = pr.x;
+                    nmn[1] = nmx[1] = pr.y;
+                    nmn[2] = nmx[2] = pr.z;
+                    cnt = 1;
+                }
+                R gap2 = 0, ext = 0;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const uint32_t e = nmx[ax] - nmn[ax];
+                    const uint32_t nc = nmn[ax] + e / 2;
+                    const uint32_t nh = e / 2 + (e & 1u);
+                    const int32_t dc = (int32_t)(nc - gc[ax]);
+                    const int64_t adc = dc < 0 ? -(int64_t)dc : (int64_t)dc;
+                    int64_t gap = adc - (int64_t)gh[ax] - (int64_t)nh;
+                    gap = gap > 0 ? gap : 0;
+                    const R gr_ = (R)gap * scale;
+                    gap2 += gr_ * gr_;
+                    const R er = (R)e * scale;
+                    ext = er > ext ? er : ext;
+                }
+                if (gap2 < rcut2) {
+                    if (node >= 0 && ext * ext < theta2 * gap2) {
+                        act = 1;
+                        napp = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
+                    } else if (cnt <= a.leaf) {
+                        act = 2;
+                        napp = cnt;
+                    } else {
+                        act = 3;
+                    }
+                }
+            }
+            // appends
+            const int incl = wave_incl_scan(napp);
+            const int off = lc + incl - napp;
+            const int total = __shfl(incl, 63, 64);
+            if (act == 1) {
+                int o = off;
+                if (mono_q(mono, node, 0) != (R)0) L[o++] = mono_entry(mono, node, 0, gc, scale);
+                if (mono_q(mono, node, 1) != (R)0) L[o] = mono_entry(mono, node, 1, gc, scale);
+            } else if (act == 2) {
+                for (int t = 0; t < napp; ++t) {
+                    const uint4 pr = a.parts[first + t];
+                    E e;
+                    e.x = fixed_rel<R>(pr.x, gc[0], scale);
+                    e.y = fixed_rel<R>(pr.y, gc[1], scale);
+                    e.z = fixed_rel<R>(pr.z, gc[2], scale);
+                    e.w = qpart[(int32_t)pr.w > 0 ? 1 : 0];
+                    L[off + t] = e;
+                }
+            }
+            // pushes
+            const uint64_t om = __ballot(act == 3);
+            const int nopen = __popcll(om);
+            if (sp + 2 * nopen > kStackPhys) {
+                if (lane == 0) atomicExch(&a.counters[1], 1);
+            } else if (act == 3) {
+                const int rank = __popcll(om & lt);
+                const int2 kd = a.kids[node];
+                S[sp + 2 * rank] = kd.y;
+                S[sp + 2 * rank + 1] = kd.x;
+            }
+            if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
+            lc += total;
+            wave_lds_fence();
+            if (lc >= kFlushAt) {
+                flush(lc);
+                lc = 0;
+                wave_lds_fence();
+            }
+        }
+        if (lc > 0) flush(lc);
+        wave_lds_fence();
+
+        if (valid) {
+            const R sg = (int32_t)rec.w > 0 ? (R)1 : (R)-1;
+            R ox = sg * sx, oy = sg * sy, oz = sg * sz;
+            if (a.pm_acc) {
+                const R* pm = (const R*)a.pm_acc;
+                ox += pm[3 * ti];
+                oy += pm[3 * ti + 1];
+                oz += pm[3 * ti + 2];
+            }
+            const int64_t o = (int64_t)a.perm[ti];
+            R* out = (R*)a.out;
+            out[3 * o] = ox;
+            out[3 * o + 1] = oy;
+            out[3 * o + 2] = oz;
+        }
+    }
+    if (lane == 0 && my_inter) atomicAdd(a.inter, my_inter);
+}
+
+template <typename R>
+__global__ void k_pm_only(const R* __restrict__ pm, const int32_t* __restrict__ perm, int64_t n, R* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = perm[i];
+    R v0 = 0, v1 = 0, v2 = 0;
+    if (pm) {
+        v0 = pm[3 * i];
+        v1 = pm[3 * i + 1];
+        v2 = pm[3 * i + 2];
+    }
+    out[3 * o] = v0;
+    out[3 * o + 1] = v1;
+    out[3 * o + 2] = v2;
+}
+
+hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
+    if (n == 0) return hipGetLastError();
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    if (c.p.precision == JTP_FP64)
+        k_pm_only<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.p.pm_enabled ? (const double*)c.pm_acc : nullptr,
+                                                            c.vals0, n, (double*)acc_out);
+    else
+        k_pm_only<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.p.pm_enabled ? (const float*)c.pm_acc : nullptr,
+                                                           c.vals0, n, (float*)acc_out);
+    return hipGetLastError();
+}
+
+template <typename R, int MODE, int WAVES>
+static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int64_t blocks = (gmax + WAVES - 1) / WAVES;
+    const int64_t cap = (int64_t)cus * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    k_walk<R, MODE, WAVES><<<dim3((unsigned)blocks), dim3(64 * WAVES), 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
+    WalkArgs a{};
+    a.parts = c.parts;
+    a.geo = c.node_geo;
+    a.kids = c.node_kids;
+    a.mono = c.node_mono;
+    a.groups = c.groups;
+    a.counters = c.dev_counters;
+    a.inter = c.dev_inter;
+    a.pm_acc = c.p.pm_enabled ? c.pm_acc : nullptr;
+    a.perm = c.vals0;
+    a.out = acc_out;
+    a.n = n;
+    a.root = n >= 2 ? 0 : ~0;
+    a.leaf = c.p.leaf_size;
+    a.max_steps = 4 * n + 1024;  // each step pops >= 1 of the <= 2n-1 nodes
+    a.scale = c.p.box_size / 4294967296.0;
+    a.eps2 = c.p.softening * c.p.softening;
+    a.rcut2 = c.p.r_cut * c.p.r_cut;
+    a.theta2 = c.p.theta * c.p.theta;
+    a.inv2rs = c.p.r_s > 0 ? 0.5 / c.p.r_s : 0.0;
+    a.qp = c.p.g_constant * c.m_plus;
+    a.qm = -c.p.g_constant * c.m_minus;
+    a.poly = c.poly;
+    // t = x * 2/X - 1 with x = rho/(2 r_s) and X the fit range (make_split_poly)
+    a.tscale = c.poly_tscale;
+    a.escale = (float)(c.p.r_s > 0 ? -1.4426950408889634 / (4.0 * c.p.r_s * c.p.r_s) : 0.0);
+    const int64_t gmax = c.n_groups_max;
+    const int mode = c.p.split_mode;
+    if (c.p.precision == JTP_FP64) {
+        if (mode == JTP_SPLIT_ERFC_GAUSS) return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2>(a, gmax, s);
+        if (mode == JTP_SPLIT_ERFC) return launch_walk<double, JTP_SPLIT_ERFC, 2>(a, gmax, s);
+        return launch_walk<double, JTP_SPLIT_POLY4, 2>(a, gmax, s);
+    }
+    if (mode == JTP_SPLIT_ERFC_GAUSS) return launch_walk<float, JTP_SPLIT_ERFC_GAUSS, 4>(a, gmax, s);
+    if (mode == JTP_SPLIT_ERFC) return launch_walk<float, JTP_SPLIT_ERFC, 4>(a, gmax, s);
+    return launch_walk<float, JTP_SPLIT_POLY4, 4>(a, gmax, s);
+}
+
+}  // namespace jtp

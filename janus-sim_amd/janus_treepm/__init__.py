@@ -1,0 +1,344 @@
+"""Python binding of libjanus_treepm (the MI355X TreePM force step).
+
+The library is a C ABI (include/janus_treepm.h); this module loads it with
+ctypes and mirrors the reference's Rust-side interface for this path:
+
+* ``TreePMForce``      - treepm_force.rs:17-146 (new / update / compute_all_forces),
+                         particle arrays in host memory, FP64 results.
+* ``PoissonSolver``    - cufft_ffi.rs:71-354 (CuFFTPoisson / solve_device), device grids.
+* ``ForceContext``     - the device-pointer force call of GpuNBodyTwoPass
+                         (nbody_gpu_twopass.rs:3961-4138, 4486-4649) without the integrator.
+
+Errors from the library raise ``JtpError`` (the reference maps a -1 status to
+``Err(String)``, cufft_ffi.rs:97-99).  There is no CPU fallback: if the shared
+library is missing the import fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libjanus_treepm.so")
+
+JTP_OK, JTP_ERR_ARG, JTP_ERR_HIP, JTP_ERR_ALLOC, JTP_ERR_OVERFLOW, JTP_ERR_STATE = 0, -1, -2, -3, -4, -5
+FP32, FP64 = 0, 1
+SPLIT_ERFC_GAUSS, SPLIT_ERFC, SPLIT_POLY4 = 0, 1, 2
+GRAD_4PT, GRAD_2PT = 0, 1
+STAGE_KEYS, STAGE_PERM, STAGE_NODES, STAGE_MONO, STAGE_RHO, STAGE_PHI, STAGE_ACC_PM, STAGE_SUMS, STAGE_BOXES = (
+    1, 2, 3, 4, 5, 6, 7, 8, 9)
+
+# every symbol declared in include/janus_treepm.h
+EXPORTED = [
+    "jtp_default_params", "jtp_create", "jtp_destroy", "jtp_last_error", "jtp_version", "jtp_set_species_mass",
+    "jtp_compute_forces", "jtp_pos_to_fixed_f64", "jtp_pos_to_fixed_f32", "jtp_pm_solve", "jtp_morton_keys_f32",
+    "jtp_get_stats", "jtp_stage_size", "jtp_stage_copy", "jtp_alloc", "jtp_free", "jtp_copy_h2d",
+    "jtp_copy_d2h", "jtp_copy_d2d", "jtp_device_synchronize", "jtp_device_count",
+]
+
+
+class JtpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"janus_treepm error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("box_size", C.c_double), ("n_cells", C.c_int32), ("k_min", C.c_int32),
+        ("r_s", C.c_double), ("r_cut", C.c_double), ("theta", C.c_double), ("softening", C.c_double),
+        ("g_constant", C.c_double), ("leaf_size", C.c_int32), ("group_size", C.c_int32),
+        ("precision", C.c_int32), ("split_mode", C.c_int32), ("gradient", C.c_int32),
+        ("deconvolve", C.c_int32), ("pm_enabled", C.c_int32), ("tree_enabled", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double), ("ms_sort", C.c_double), ("ms_build", C.c_double), ("ms_deposit", C.c_double),
+        ("ms_fft", C.c_double), ("ms_interp", C.c_double), ("ms_walk", C.c_double),
+        ("interactions", C.c_uint64), ("n_groups", C.c_int64), ("n_particles", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libjanus_treepm.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} not built: run `make -C janus-sim_amd` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp, i64, i32, sz, dbl = C.c_void_p, C.c_int64, C.c_int32, C.c_size_t, C.c_double
+    sig = {
+        "jtp_default_params": (None, [C.POINTER(Params), dbl, i32]),
+        "jtp_create": (C.c_int, [C.POINTER(Params), C.POINTER(vp)]),
+        "jtp_destroy": (C.c_int, [vp]),
+        "jtp_last_error": (C.c_char_p, [vp]),
+        "jtp_version": (C.c_char_p, []),
+        "jtp_set_species_mass": (C.c_int, [vp, dbl, dbl]),
+        "jtp_compute_forces": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+        "jtp_pos_to_fixed_f64": (C.c_int, [vp, i64, vp, vp, vp]),
+        "jtp_pos_to_fixed_f32": (C.c_int, [vp, i64, vp, vp, vp]),
+        "jtp_pm_solve": (C.c_int, [vp, vp, vp, vp]),
+        "jtp_morton_keys_f32": (C.c_int, [i64, vp, C.c_float, C.c_float, vp, vp]),
+        "jtp_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
+        "jtp_stage_size": (C.c_int, [vp, i32, C.POINTER(sz)]),
+        "jtp_stage_copy": (C.c_int, [vp, i32, vp, sz]),
+        "jtp_alloc": (vp, [sz]),
+        "jtp_free": (None, [vp]),
+        "jtp_copy_h2d": (C.c_int, [vp, vp, sz]),
+        "jtp_copy_d2h": (C.c_int, [vp, vp, sz]),
+        "jtp_copy_d2d": (C.c_int, [vp, vp, sz]),
+        "jtp_device_synchronize": (C.c_int, []),
+        "jtp_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = C.c_int(0)
+    rc = lib.jtp_device_count(C.byref(n))
+    return n.value if rc == JTP_OK else 0
+
+
+class DeviceArray:
+    """A device allocation holding a numpy-described array (jtp_alloc / jtp_free)."""
+
+    def __init__(self, shape, dtype):
+        self.lib = load_library()
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = self.lib.jtp_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise JtpError(JTP_ERR_ALLOC, f"jtp_alloc({self.nbytes}) failed")
+
+    @classmethod
+    def from_host(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        if self.nbytes and self.lib.jtp_copy_h2d(self.ptr, a.ctypes.data, self.nbytes) != JTP_OK:
+            raise JtpError(JTP_ERR_HIP, "copy to device failed")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes and self.lib.jtp_copy_d2h(out.ctypes.data, self.ptr, self.nbytes) != JTP_OK:
+            raise JtpError(JTP_ERR_HIP, "copy to host failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.lib.jtp_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def default_params(box_size: float, n_cells: int, **kw) -> Params:
+    lib = load_library()
+    p = Params()
+    lib.jtp_default_params(C.byref(p), float(box_size), int(n_cells))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class ForceContext:
+    """One jtp context: the device force call on fixed-point particle arrays."""
+
+    def __init__(self, params: Params, m_plus: float = 1.0, m_minus_abs: float = 1.0):
+        self.lib = load_library()
+        self.params = params
+        h = C.c_void_p()
+        rc = self.lib.jtp_create(C.byref(params), C.byref(h))
+        if rc != JTP_OK:
+            raise JtpError(rc, "jtp_create failed (invalid parameters or device error)")
+        self.h = h
+        self.set_species_mass(m_plus, m_minus_abs)
+
+    @property
+    def real(self):
+        return np.float64 if self.params.precision == FP64 else np.float32
+
+    def _check(self, rc):
+        if rc != JTP_OK:
+            raise JtpError(rc, self.lib.jtp_last_error(self.h).decode())
+
+    def set_species_mass(self, m_plus: float, m_minus_abs: float):
+        self._check(self.lib.jtp_set_species_mass(self.h, float(m_plus), float(m_minus_abs)))
+
+    def compute_forces_device(self, n: int, d_pos_fixed, d_species, d_acc, stream=None):
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        self._check(self.lib.jtp_compute_forces(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), ptr(d_acc), stream))
+
+    def compute_forces(self, pos_fixed: np.ndarray, species: np.ndarray) -> np.ndarray:
+        """Host convenience: upload, compute, download (float32 or float64 [n,3])."""
+        n = len(species)
+        dp = DeviceArray.from_host(np.ascontiguousarray(pos_fixed, np.uint32).reshape(n, 3))
+        ds = DeviceArray.from_host(np.ascontiguousarray(species, np.int8))
+        da = DeviceArray((n, 3), self.real)
+        self.compute_forces_device(n, dp, ds, da)
+        out = da.download()
+        self._last = (dp, ds, da)
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.lib.jtp_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def stage(self, stage: int) -> np.ndarray:
+        nbytes = C.c_size_t(0)
+        self._check(self.lib.jtp_stage_size(self.h, stage, C.byref(nbytes)))
+        buf = np.empty(nbytes.value, np.uint8)
+        self._check(self.lib.jtp_stage_copy(self.h, stage, buf.ctypes.data, nbytes.value))
+        rt = self.real
+        view = {
+            STAGE_KEYS: (np.uint64, (-1,)), STAGE_PERM: (np.int32, (-1,)), STAGE_NODES: (np.int32, (-1, 4)),
+            STAGE_MONO: (np.float64, (-1, 8)), STAGE_RHO: (rt, (-1,)), STAGE_PHI: (rt, (-1,)),
+            STAGE_ACC_PM: (rt, (-1, 3)), STAGE_SUMS: (np.uint64, (-1, 8)), STAGE_BOXES: (np.uint32, (-1, 6)),
+        }[stage]
+        return buf.view(view[0]).reshape(view[1])
+
+    def pm_solve_device(self, d_rho, d_phi, stream=None):
+        self._check(self.lib.jtp_pm_solve(self.h, d_rho.ptr, d_phi.ptr, stream))
+
+    def to_fixed_device(self, n: int, d_pos, d_fixed, stream=None):
+        if d_pos.dtype == np.float64:
+            self._check(self.lib.jtp_pos_to_fixed_f64(self.h, int(n), d_pos.ptr, d_fixed.ptr, stream))
+        else:
+            self._check(self.lib.jtp_pos_to_fixed_f32(self.h, int(n), d_pos.ptr, d_fixed.ptr, stream))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.jtp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_fixed(x: np.ndarray, box_size: float) -> np.ndarray:
+    """Host conversion of positions in [-L/2, L/2) (any period) to 32-bit fixed point."""
+    t = np.asarray(x, np.float64) / box_size + 0.5
+    t = t - np.floor(t)
+    w = (t * 4294967296.0).astype(np.uint64)
+    w[w >= 4294967296] -= 4294967296
+    return w.astype(np.uint32)
+
+
+def from_fixed(u: np.ndarray, box_size: float) -> np.ndarray:
+    return (np.asarray(u, np.float64) / 4294967296.0 - 0.5) * box_size
+
+
+# ---------------------------------------------------------------- reference-shaped facades
+@dataclass
+class Particle:
+    """nbody.rs Particle (pos, vel, |mass|, sign); only pos / mass / sign matter for forces."""
+    pos: tuple
+    mass: float
+    sign: int
+
+
+class TreePMForce:
+    """Mirror of TreePMForce (treepm_force.rs:17-146) on the GPU.
+
+    ``TreePMForce(r_cut, grid_size, box_size, theta, softening)`` keeps the
+    reference constructor; the split scale defaults to r_cut/3 like the
+    reference (treepm_force.rs:36) unless ``r_s`` is given.  Positions are in
+    [-L/2, L/2) like the reference; species masses are per species (the
+    reference's per-particle |m| must be uniform within a species).
+    """
+
+    def __init__(self, r_cut, grid_size, box_size, theta, softening, r_s=None, precision=FP64,
+                 split_mode=SPLIT_ERFC_GAUSS, gradient=GRAD_4PT, deconvolve=1, g_constant=1.0, pm_only=False):
+        p = default_params(box_size, grid_size)
+        p.r_cut = float(r_cut)
+        p.r_s = float(r_cut / 3.0 if r_s is None else r_s)
+        p.theta = float(theta)
+        p.softening = float(softening)
+        p.precision = precision
+        p.split_mode = split_mode
+        p.gradient = gradient
+        p.deconvolve = deconvolve
+        p.g_constant = g_constant
+        p.tree_enabled = 0 if pm_only else 1
+        self.params = p
+        self.ctx = ForceContext(p)
+        self._acc = None
+
+    @classmethod
+    def new_pm_only(cls, grid_size, box_size):
+        return cls(box_size / 8.0, grid_size, box_size, 0.5, 0.5, r_s=0.0, pm_only=True, split_mode=SPLIT_POLY4)
+
+    def update(self, particles):
+        pos = np.array([p.pos for p in particles], np.float64).reshape(-1, 3)
+        sign = np.array([1 if p.sign > 0 else -1 for p in particles], np.int8)
+        mass = np.array([abs(p.mass) for p in particles], np.float64)
+        mp = mass[sign > 0]
+        mm = mass[sign < 0]
+        m_plus = float(mp[0]) if len(mp) else 1.0
+        m_minus = float(mm[0]) if len(mm) else 1.0
+        if (len(mp) and not np.all(mp == m_plus)) or (len(mm) and not np.all(mm == m_minus)):
+            raise JtpError(JTP_ERR_ARG, "masses must be uniform within a species")
+        self.ctx.set_species_mass(m_plus, m_minus)
+        self._acc = self.ctx.compute_forces(to_fixed(pos, self.params.box_size), sign).astype(np.float64)
+
+    def compute_all_forces(self, particles=None):
+        if self._acc is None:
+            raise JtpError(JTP_ERR_STATE, "update() must be called first")
+        return self._acc.copy()
+
+
+class PoissonSolver:
+    """Mirror of CuFFTPoisson / solve_device (cufft_ffi.rs:71-337) for device grids."""
+
+    def __init__(self, grid_size, box_size, g_constant=1.0, r_s=0.0, k_min=0, precision=FP64, deconvolve=0):
+        p = default_params(box_size, grid_size)
+        p.r_s = float(r_s)
+        p.k_min = int(k_min)
+        p.g_constant = float(g_constant)
+        p.precision = precision
+        p.deconvolve = deconvolve
+        p.split_mode = SPLIT_POLY4 if r_s <= 0 else SPLIT_ERFC_GAUSS
+        p.r_cut = box_size / 8.0
+        self.ctx = ForceContext(p)
+        self.grid_size = grid_size
+
+    def solve(self, rho: np.ndarray) -> np.ndarray:
+        """rho: mass per cell, [G,G,G] indexed [z][y][x]; returns the potential."""
+        d_rho = DeviceArray.from_host(np.ascontiguousarray(rho, self.ctx.real))
+        d_phi = DeviceArray(rho.shape, self.ctx.real)
+        self.ctx.pm_solve_device(d_rho, d_phi)
+        return d_phi.download()

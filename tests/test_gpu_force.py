@@ -1,0 +1,98 @@
+"""GPU full force: sign rules of the reference tests, edge cases, accuracy against
+the FP64 Ewald oracle, and the reference-CPU-compatible mode against the
+restatement of TreePMForce (treepm_force.rs)."""
+import numpy as np
+import pytest
+
+from janus_treepm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+
+
+@pytest.mark.parametrize("si,sj,attract", [(1, 1, True), (-1, -1, True), (1, -1, False), (-1, 1, False)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pair_signs(jtp, si, sj, attract, precision):
+    """treepm_force.rs:207-264 / treepm_physics_8p.rs:142-193: (+,+),(-,-) attract, (+,-),(-,+) repel."""
+    print(f"case: pair signs {si},{sj} precision={precision}", flush=True)
+    f = jtp.TreePMForce(20.0, 32, 100.0, 0.5, 0.5, precision=precision)
+    parts = [jtp.Particle((-10.0, 0.0, 0.0), 1.0, si), jtp.Particle((10.0, 0.0, 0.0), 1.0, sj)]
+    f.update(parts)
+    a = f.compute_all_forces(parts)
+    assert (a[0, 0] > 0) == attract
+    assert (a[1, 0] < 0) == attract
+    assert abs(a[0, 0] + a[1, 0]) < 0.01 * abs(a[0, 0])  # Newton's third law (equal masses)
+
+
+def test_edge_cases(jtp):
+    print("case: edge cases", flush=True)
+    p = jtp.default_params(100.0, 32)
+    ctx = jtp.ForceContext(p)
+    # n = 0 is a no-op
+    ctx.compute_forces(np.zeros((0, 3), np.uint32), np.zeros(0, np.int8))
+    # n = 1: no short-range force, PM force of a lone particle vanishes by symmetry
+    a = ctx.compute_forces(np.array([[123, 456, 789]], np.uint32), np.array([1], np.int8))
+    assert np.all(np.abs(a) < 1e-6)
+    # coincident particles: finite (softened) and zero by symmetry
+    pos = np.tile(np.array([[2 ** 31, 2 ** 31, 2 ** 31]], np.uint32), (5, 1))
+    a = ctx.compute_forces(pos, np.array([1, 1, -1, -1, 1], np.int8))
+    assert np.all(np.isfinite(a)) and np.all(np.abs(a) < 1e-4)
+    # particles on the periodic boundary (u = 0 and u = 2^32 - 1 are neighbours)
+    pos = np.array([[0, 0, 0], [2 ** 32 - 1, 0, 0]], np.uint32)
+    a = ctx.compute_forces(pos, np.array([1, 1], np.int8))
+    assert np.all(np.isfinite(a))
+    # invalid arguments are rejected with JTP_ERR_ARG, never a crash
+    lib = jtp.load_library()
+    assert lib.jtp_compute_forces(ctx.h, -1, None, None, None, None) == jtp.JTP_ERR_ARG
+    assert lib.jtp_compute_forces(ctx.h, 10, None, None, None, None) == jtp.JTP_ERR_ARG
+    bad = jtp.default_params(100.0, 48)
+    with pytest.raises(jtp.JtpError):
+        jtp.ForceContext(bad)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_full_force_vs_ewald(jtp, orc, precision):
+    print(f"case: full force vs Ewald precision={precision}", flush=True)
+    w = synth.lattice_zeldovich(n_side=24, n_cells=32, seed=21)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    p = jtp.default_params(w.box_size, w.n_cells)
+    p.r_s, p.r_cut, p.theta, p.softening, p.precision = w.r_s, w.r_cut, 0.5, w.softening, precision
+    ctx = jtp.ForceContext(p, w.m_plus, w.m_minus_abs)
+    acc = ctx.compute_forces(pos, w.species).astype(np.float64)
+    rng = np.random.default_rng(0)
+    tg = rng.choice(w.n, 400, replace=False)
+    ref = orc.ewald(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, targets=tg, rc=w.box_size / 3)
+    e = _rel(acc[tg], ref)
+    print("TreePM vs Ewald (2x24^3, G=32): median %.2e p99 %.2e" % (np.median(e), np.percentile(e, 99)))
+    assert np.median(e) < 2e-2 and np.percentile(e, 99) < 2e-1
+
+
+def test_compat_vs_reference_cpu(jtp, orc):
+    """Reference-CPU-compatible switches (x^4 split, 2-point gradient, no deconvolution, r_s = r_cut/3)
+    at theta = 0 against the restatement of TreePMForce; particles sit in the inner half of the box so the
+    reference's non-periodic tree sees the same pairs as the minimum-image walk."""
+    print("case: compat vs reference CPU", flush=True)
+    from oracle.oracle_lib import RefCpu
+    L, G = 100.0, 32
+    rng = np.random.default_rng(5)
+    n = 3000
+    x = rng.uniform(-L / 4, L / 4, (n, 3))
+    sp = np.where(rng.random(n) < 0.5, 1, -1).astype(np.int8)
+    pos = jtp.to_fixed(x, L)
+    xq = jtp.from_fixed(pos, L)
+    r_cut, eps = 8.0, 0.3
+    p = jtp.default_params(L, G)
+    p.r_cut, p.r_s, p.theta, p.softening = r_cut, r_cut / 3, 0.0, eps
+    p.precision, p.split_mode, p.gradient, p.deconvolve = 1, 2, 1, 0
+    ctx = jtp.ForceContext(p, 1.0, 1.0)
+    acc = ctx.compute_forces(pos, sp)
+    h = L / G
+    ref = RefCpu(orc, r_cut, G, L, 0.0, eps, r_cut / 3, 1.0 / h ** 3, 1.0)
+    ref.update(xq, np.ones(n), sp)
+    a_ref = ref.forces()
+    e = _rel(acc, a_ref)
+    print("compat vs reference CPU: median %.2e max %.2e" % (np.median(e), np.max(e)))
+    assert np.median(e) < 1e-9 and np.max(e) < 1e-6

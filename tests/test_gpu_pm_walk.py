@@ -1,0 +1,138 @@
+"""GPU parity of the long-range and short-range parts through the C ABI.
+
+PM: deposit vs orc_deposit (pm_grid.rs:63-103 on one net grid), Poisson solve vs
+orc_poisson (pm_grid.rs:132-271 / cufft_wrapper.cu:148-208), interpolation vs
+orc_interp (pm_grid.rs:275-359); numpy FFT as the independent check of jtp_pm_solve.
+Walk: theta = 0 against the minimum-image direct sum with the same pair law, and
+theta = 0.5 against it within a stated tolerance."""
+import numpy as np
+import pytest
+
+from janus_treepm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(jtp, w, **kw):
+    p = jtp.default_params(w.box_size, w.n_cells)
+    p.r_s, p.r_cut, p.theta, p.softening = w.r_s, w.r_cut, w.theta, w.softening
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return jtp.ForceContext(p, w.m_plus, w.m_minus_abs)
+
+
+def _rel(a, b):
+    den = np.linalg.norm(b, axis=1)
+    return np.linalg.norm(a - b, axis=1) / np.maximum(den, 1e-300)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_deposit_solve_interp(jtp, orc, precision):
+    print(f"case: PM chain precision={precision}", flush=True)
+    w = synth.lattice_zeldovich(n_side=20, n_cells=32, seed=9)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=precision, tree_enabled=0)
+    acc = ctx.compute_forces(pos, w.species)
+    rho = ctx.stage(jtp.STAGE_RHO).astype(np.float64)
+    rho_ref = orc.deposit(pos, w.species, w.m_plus, w.m_minus_abs, w.n_cells)
+    tol = 1e-12 if precision == 1 else 2e-6
+    scale = np.max(np.abs(rho_ref))
+    assert np.max(np.abs(rho - rho_ref)) <= tol * scale * 20
+    phi = ctx.stage(jtp.STAGE_PHI).astype(np.float64)
+    phi_ref = orc.poisson(rho_ref, w.n_cells, w.box_size, 1.0, w.r_s, 0, 1)
+    tol = 1e-10 if precision == 1 else 2e-5
+    assert np.max(np.abs(phi - phi_ref)) <= tol * np.max(np.abs(phi_ref))
+    acc_ref = orc.interp(pos, w.species, w.n_cells, w.box_size, phi_ref, four=True)
+    tol = 1e-9 if precision == 1 else 1e-4
+    assert np.max(np.abs(acc - acc_ref)) <= tol * np.max(np.abs(acc_ref))
+    assert np.array_equal(ctx.stage(jtp.STAGE_ACC_PM), acc)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_species_mass_conservation(jtp, orc, precision):
+    """Per-species deposited mass is conserved (pattern of pm_grid.rs:426-439)."""
+    print(f"case: mass conservation precision={precision}", flush=True)
+    w = synth.clustered(n_total=200_000, n_cells=64, n_halos=50, seed=4)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    for mp, mm in ((1.0, 0.0), (0.0, 19.0)):
+        ctx = _ctx(jtp, w, precision=precision, tree_enabled=0)
+        ctx.set_species_mass(mp, mm)
+        ctx.compute_forces(pos, w.species)
+        tot = np.sum(ctx.stage(jtp.STAGE_RHO).astype(np.float64))
+        want = mp * np.sum(w.species > 0) - mm * np.sum(w.species < 0)
+        tol = 1e-12 if precision == 1 else 1e-5
+        assert abs(tot - want) <= tol * abs(want), (tot, want)
+
+
+def test_deposit_deterministic(jtp):
+    print("case: deposit determinism", flush=True)
+    w = synth.clustered(n_total=300_000, n_cells=64, n_halos=80, seed=8)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=0)
+    a1 = ctx.compute_forces(pos, w.species)
+    r1 = ctx.stage(jtp.STAGE_RHO)
+    a2 = ctx.compute_forces(pos, w.species)
+    r2 = ctx.stage(jtp.STAGE_RHO)
+    assert np.array_equal(r1, r2)
+    assert np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_pm_solve_numpy(jtp, precision):
+    """jtp_pm_solve against numpy's FFT (an independent library) with the closed-form Green function."""
+    print(f"case: pm_solve vs numpy precision={precision}", flush=True)
+    G, L, rs = 32, 100.0, 3.0
+    rng = np.random.default_rng(1)
+    rho = rng.standard_normal((G, G, G))
+    ps = jtp.PoissonSolver(G, L, g_constant=1.0, r_s=rs, precision=precision, deconvolve=1)
+    phi = ps.solve(rho).astype(np.float64)
+    k = 2 * np.pi * np.fft.fftfreq(G, d=L / G)
+    KZ, KY, KX = np.meshgrid(k, k, k, indexing="ij")
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    h = L / G
+    sinc = lambda x: np.sinc(x / np.pi)
+    W = sinc(KX * h / 2) ** 2 * sinc(KY * h / 2) ** 2 * sinc(KZ * h / 2) ** 2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        green = -4 * np.pi / k2 * np.exp(-k2 * rs ** 2) / W ** 2 / h ** 3
+    green[0, 0, 0] = 0
+    ref = np.real(np.fft.ifftn(np.fft.fftn(rho) * green))
+    tol = 1e-11 if precision == 1 else 3e-5
+    assert np.max(np.abs(phi - ref)) <= tol * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("precision,mode", [(1, 0), (0, 0), (1, 1), (0, 1), (1, 2), (0, 2)])
+def test_walk_theta0_direct(jtp, orc, precision, mode):
+    print(f"case: walk theta=0 precision={precision} mode={mode}", flush=True)
+    w = synth.clustered(n_total=40_000, box_size=100.0, n_cells=32, n_halos=30, seed=12)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=precision, pm_enabled=0, theta=0.0, split_mode=mode)
+    acc = ctx.compute_forces(pos, w.species).astype(np.float64)
+    ref = orc.short_direct(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, w.r_cut, w.r_s, mode)
+    e = _rel(acc, ref)
+    if precision == 1:
+        assert np.median(e) < 1e-13 and np.max(e) < 1e-9, (np.median(e), np.max(e))
+    else:
+        assert np.median(e) < 3e-6 and np.percentile(e, 99) < 1e-4, (np.median(e), np.percentile(e, 99))
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_walk_theta_half(jtp, orc, precision):
+    print(f"case: walk theta=0.5 precision={precision}", flush=True)
+    w = synth.clustered(n_total=60_000, box_size=100.0, n_cells=32, n_halos=40, seed=13)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=precision, pm_enabled=0, theta=0.5)
+    acc = ctx.compute_forces(pos, w.species).astype(np.float64)
+    ref = orc.short_direct(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, w.r_cut, w.r_s, 0)
+    e = _rel(acc, ref)
+    print("theta=0.5 short-range rel err median %.2e p99 %.2e" % (np.median(e), np.percentile(e, 99)))
+    assert np.median(e) < 1e-3 and np.percentile(e, 99) < 2e-2
+
+
+def test_walk_deterministic(jtp):
+    print("case: walk determinism", flush=True)
+    w = synth.clustered(n_total=100_000, box_size=100.0, n_cells=32, n_halos=40, seed=14)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=0)
+    a1 = ctx.compute_forces(pos, w.species)
+    a2 = ctx.compute_forces(pos, w.species)
+    assert np.array_equal(a1, a2)

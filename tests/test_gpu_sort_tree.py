@@ -1,0 +1,138 @@
+"""GPU parity: Morton keys, stable radix-sort permutation, Karras topology and
+dual monopoles — bit-exact against the oracle restatements of
+nbody_gpu_twopass.rs:1447-1658 (keys, LSD radix sort) and :356-561 (LBVH),
+with numpy's stable argsort as an independent check of the permutation."""
+import numpy as np
+import pytest
+
+from janus_treepm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(jtp, L=250.0, G=32, **kw):
+    p = jtp.default_params(L, G, **kw)
+    return jtp.ForceContext(p, 1.0, 19.0)
+
+
+def _random_fixed(rng, n, dup_frac=0.0):
+    pos = rng.integers(0, 2 ** 32, (n, 3), dtype=np.uint64).astype(np.uint32)
+    if dup_frac > 0 and n > 4:
+        k = int(n * dup_frac)
+        src = rng.integers(0, n, k)
+        dst = rng.integers(0, n, k)
+        pos[dst] = pos[src]
+    sp = np.where(rng.random(n) < 0.5, 1, -1).astype(np.int8)
+    return pos, sp
+
+
+@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 2047, 2048, 2049, 70001])
+def test_keys_perm_small(jtp, orc, n):
+    print(f"case: keys/perm n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    pos, sp = _random_fixed(rng, n, dup_frac=0.1)
+    ctx = _ctx(jtp, tree_enabled=0, pm_enabled=0)
+    ctx.compute_forces(pos, sp)
+    keys = ctx.stage(jtp.STAGE_KEYS)
+    perm = ctx.stage(jtp.STAGE_PERM)
+    k_ref = orc.morton(pos)
+    perm_ref = orc.radix_sort(k_ref)
+    assert np.array_equal(perm, perm_ref)
+    assert np.array_equal(perm, np.argsort(k_ref, kind="stable"))
+    assert np.array_equal(keys, k_ref[perm_ref])
+
+
+def test_keys_perm_large_clustered(jtp, orc):
+    print("case: keys/perm clustered n=1.5e6", flush=True)
+    w = synth.clustered(n_total=1_500_000, n_cells=64, n_halos=200, seed=3)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, L=w.box_size, G=64, tree_enabled=0, pm_enabled=0)
+    ctx.compute_forces(pos, w.species)
+    perm = ctx.stage(jtp.STAGE_PERM)
+    k_ref = orc.morton(pos)
+    assert np.array_equal(perm, np.argsort(k_ref, kind="stable"))
+    assert np.array_equal(ctx.stage(jtp.STAGE_KEYS), k_ref[perm])
+
+
+def test_reference_f32_keys(jtp, orc):
+    """jtp_morton_keys_f32 reproduces compute_morton_all's f32 arithmetic bit for bit."""
+    print("case: reference f32 Morton keys", flush=True)
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    L = 250.0
+    n = 100_000
+    pos = rng.uniform(-L / 2, L / 2, (n, 3)).astype(np.float32)
+    pos[:10] = np.float32(-L / 2)
+    pos[10:20] = np.float32(L / 2)  # clamps at 2^21 - 1
+    box_half = np.float32(L / 2)
+    inv_cell = np.float32(2097152.0 / L)
+    d_pos = jtp.DeviceArray.from_host(pos)
+    d_keys = jtp.DeviceArray((n,), np.uint64)
+    lib = jtp.load_library()
+    assert lib.jtp_morton_keys_f32(n, d_pos.ptr, C.c_float(box_half), C.c_float(inv_cell), d_keys.ptr, None) == 0
+    assert np.array_equal(d_keys.download(), orc.morton_f32_ref(pos, box_half, inv_cell))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tree_topology_and_monopoles(jtp, orc, precision):
+    print(f"case: tree topology precision={precision}", flush=True)
+    w = synth.lattice_zeldovich(n_side=24, n_cells=32, seed=5)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    # add exact duplicates to exercise the index tie-break of the key delta
+    pos[100:110] = pos[200]
+    ctx = _ctx(jtp, L=w.box_size, G=32, precision=precision, pm_enabled=0)
+    ctx.compute_forces(pos, w.species)
+    perm = ctx.stage(jtp.STAGE_PERM)
+    keys = ctx.stage(jtp.STAGE_KEYS)
+    nodes = ctx.stage(jtp.STAGE_NODES)
+    nodes_ref = orc.karras(keys)
+    assert np.array_equal(nodes, nodes_ref)
+    sums, boxes = orc.node_sums(pos[perm], w.species[perm], nodes_ref)
+    assert np.array_equal(ctx.stage(jtp.STAGE_SUMS), sums)
+    assert np.array_equal(ctx.stage(jtp.STAGE_BOXES), boxes)
+    mono = ctx.stage(jtp.STAGE_MONO)
+    n_p = sums[:, 0].astype(np.float64)
+    n_m = sums[:, 4].astype(np.float64)
+    has_p, has_m = n_p > 0, n_m > 0
+    com_p = sums[has_p, 1:4].astype(np.float64) / n_p[has_p, None]
+    com_m = sums[has_m, 5:8].astype(np.float64) / n_m[has_m, None]
+    tol = 0.51 if precision == 0 else 1e-6  # FP32 form stores the rounded fixed-point mean
+    assert np.max(np.abs(mono[has_p, 1:4] - com_p)) <= tol
+    assert np.max(np.abs(mono[has_m, 5:8] - com_m)) <= tol
+    assert np.allclose(mono[:, 0], 1.0 * n_p, rtol=1e-6)
+    assert np.allclose(mono[:, 4], -19.0 * n_m, rtol=1e-6)
+
+
+def test_tree_bruteforce_small(jtp, orc):
+    """Independent check of the topology: every internal node splits its range at the
+    highest differing key bit (index tie-break for equal keys)."""
+    print("case: brute-force split check", flush=True)
+    rng = np.random.default_rng(2)
+    n = 64
+    pos, sp = _random_fixed(rng, n, dup_frac=0.2)
+    ctx = _ctx(jtp, pm_enabled=0)
+    ctx.compute_forces(pos, sp)
+    keys = ctx.stage(jtp.STAGE_KEYS).astype(object)
+    nodes = ctx.stage(jtp.STAGE_NODES)
+
+    def ext(i):  # 96-bit key with index tie-break
+        return (int(keys[i]) << 32) | i
+
+    seen = set()
+
+    def check(node, f, l):
+        seen.add(node)
+        assert (nodes[node][0], nodes[node][1]) == (f, l)
+        hb = (ext(f) ^ ext(l)).bit_length() - 1
+        # split = last index whose highest differing bit is 0
+        split = max(i for i in range(f, l + 1) if ((ext(i) >> hb) & 1) == 0)
+        left, right = nodes[node][2], nodes[node][3]
+        for child, (cf, cl) in ((left, (f, split)), (right, (split + 1, l))):
+            if cf == cl:
+                assert child == ~cf
+            else:
+                assert child >= 0
+                check(child, cf, cl)
+
+    check(0, 0, n - 1)
+    assert len(seen) == n - 1
