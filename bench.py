@@ -92,6 +92,39 @@ def cpu_baseline(w_full):
     }
 
 
+WORKLOADS = {
+    "clustered_22M": dict(n_total=22_000_000, box_size=250.0, n_cells=256, n_halos=2000, seed=7),
+    "clustered_344k": dict(n_total=343_750, box_size=62.5, n_cells=64, n_halos=31, seed=99),
+}
+
+
+def accuracy(w, name, d_pos, d_sp, pos_fixed):
+    """Median / p99 relative acceleration error per species of both forms against the committed FP64 Ewald
+    truth for this workload (tests/golden/ewald_<name>.npz, made by tests/golden/make_ewald_truth.py)."""
+    path = os.path.join(ROOT, "tests", "golden", f"ewald_{name}.npz")
+    if not os.path.exists(path):
+        return {"status": f"no truth fixture for {name}"}
+    f = np.load(path)
+    if str(f["digest"]) != w.digest(pos_fixed):
+        return {"status": "inputs differ from the fixture's (digest mismatch)"}
+    tg, truth = f["targets"], f["acc"]
+    out = {"targets": int(len(tg)), "status": "ok"}
+    for prec, label in ((j.FP32, "fp32"), (j.FP64, "fp64")):
+        ctx = make_ctx(w, prec)
+        d_acc = j.DeviceArray((w.n, 3), np.float64 if prec == j.FP64 else np.float32)
+        ctx.compute_forces_device(w.n, d_pos, d_sp, d_acc)
+        acc = d_acc.download()[tg].astype(np.float64)
+        ctx.close()
+        err = np.linalg.norm(acc - truth, axis=1) / np.linalg.norm(truth, axis=1)
+        res = {}
+        for sp, nm in ((1, "m+"), (-1, "m-")):
+            e = err[w.species[tg] == sp]
+            res[nm] = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+        res["all"] = {"median": float(np.median(err)), "p99": float(np.percentile(err, 99))}
+        out[label] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +134,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", default="clustered_22M", choices=sorted(WORKLOADS))
+    ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the TreePM step is single-GPU; "
@@ -111,12 +146,16 @@ def main():
         sys.exit(1)
 
     t0 = time.perf_counter()
-    w = synth.clustered(n_total=args.n, box_size=250.0, n_cells=256, n_halos=2000, seed=7)
+    cfg = dict(WORKLOADS[args.workload])
+    if args.n != 22_000_000:
+        cfg["n_total"] = args.n
+    w = synth.clustered(**cfg)
+    if args.workload == "clustered_344k":
+        w.softening = synth.default_softening(250.0, 22_000_000)
     log(f"workload generated in {time.perf_counter() - t0:.1f} s (N={w.n})")
     pos = j.to_fixed(w.pos, w.box_size)
     d_pos = j.DeviceArray.from_host(pos)
     d_sp = j.DeviceArray.from_host(w.species)
-    del pos
 
     t32, st32, _ = run_form(w, d_pos, d_sp, j.FP32, args.warmup, args.steps)
     ms32 = t32 / args.steps * 1e3
@@ -128,10 +167,11 @@ def main():
         "metric": METRIC, "value": value, "unit": "particle-force-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"clustered two-species box N={w.n} (2000 NFW halos + uniform m+, void-filling m-), "
-                               f"L=250, PM 256^3, r_s=1.25 cells, r_cut=4.5 r_s, theta=0.5, leaf 8, "
-                               f"m+=1, |m-|=19; FP32 fixed-point form",
-                   "n_particles": w.n, "pm_grid": 256, "theta": w.theta, "r_s": w.r_s, "r_cut": w.r_cut,
+        "config": {"workload": f"clustered two-species box N={w.n} ({cfg['n_halos']} NFW halos + uniform m+, "
+                               f"void-filling m-), L={w.box_size:g}, PM {w.n_cells}^3, r_s=1.25 cells, "
+                               f"r_cut=4.5 r_s, theta={w.theta}, leaf {w.leaf_size}, m+=1, |m-|=19; "
+                               f"FP32 fixed-point form",
+                   "n_particles": w.n, "pm_grid": w.n_cells, "theta": w.theta, "r_s": w.r_s, "r_cut": w.r_cut,
                    "softening": w.softening, "parallelism": "single GPU"},
     }
     walk_ms = ph32["ms_walk"]
@@ -140,7 +180,7 @@ def main():
                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                        "traffic": None, "interactions": inter, "flops_per_interaction": FLOPS_PER_INTERACTION,
                        "walk_ms": walk_ms}
-    G3 = 256 ** 3
+    G3 = w.n_cells ** 3
     hbm = {
         "deposit": (w.n * 13 + 4 * G3) / (ph32["ms_deposit"] * 1e-3) / 1e9,
         "fft": (40 * G3) / (ph32["ms_fft"] * 1e-3) / 1e9,
@@ -154,6 +194,11 @@ def main():
         out["fp64"] = {"value": w.n * args.fp64_steps / t64, "ms_per_step": t64 / args.fp64_steps * 1e3,
                        "steps": args.fp64_steps, "phases_ms": phase_summary(st64)}
         out["fp32_over_fp64"] = out["fp64"]["ms_per_step"] / ms32
+
+    if args.accuracy:
+        log("accuracy vs Ewald truth ...")
+        out["accuracy_vs_ewald"] = accuracy(w, args.workload, d_pos, d_sp, pos)
+    del pos
 
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -93,7 +93,8 @@ void make_split_poly(Ctx& c) {
 
 static void free_particle_buffers(Ctx& c) {
     void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
-                    c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc};
+                    c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
+                    c.frontier0, c.frontier1, c.dense_corner};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c.keys0 = c.keys1 = nullptr;
@@ -103,6 +104,8 @@ static void free_particle_buffers(Ctx& c) {
     c.node_geo = nullptr;
     c.node_kids = nullptr;
     c.parent_int = c.parent_leaf = c.flags = nullptr;
+    c.frontier0 = c.frontier1 = nullptr;
+    c.dense_corner = nullptr;
     c.node_sum = c.node_mono = nullptr;
     c.groups = nullptr;
     c.pm_acc = nullptr;
@@ -132,6 +135,9 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.parent_int, cap));
     chk(dalloc(c.parent_leaf, cap));
     chk(dalloc(c.flags, cap));
+    chk(dalloc(c.frontier0, cap));
+    chk(dalloc(c.frontier1, cap));
+    chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * 8 * rs));
     chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));
     chk(dalloc_bytes(c.node_mono, (size_t)cap * (rs == 8 ? 64 : 32)));
     chk(dalloc(c.groups, cap + 4096 + 64));
@@ -229,7 +235,8 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)2 * G * rs);
     if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
-    if (e == hipSuccess) e = dalloc(c.dev_counters, 8);
+    if (e == hipSuccess) e = dalloc(c.dev_counters, 8 + 130 + 8);
+    if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_pinned, 4 * sizeof(unsigned long long), 0);
     if (e == hipSuccess) e = pm_tables(c);
@@ -253,7 +260,8 @@ int jtp_destroy(jtp_ctx* h) {
     Ctx& c = h->c;
     (void)hipDeviceSynchronize();
     free_particle_buffers(c);
-    void* ptrs[] = {c.rho, c.phi, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4, c.dev_counters, c.dev_inter};
+    void* ptrs[] = {c.rho, c.phi, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4, c.dev_counters, c.dev_inter,
+                    c.dense_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);
@@ -312,10 +320,15 @@ int jtp_compute_forces(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_
     // status: interaction count, group count, walk error flag
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned, c.dev_inter, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 1, c.dev_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (c.p.tree_enabled && n >= 2)
+        JTP_HIP(&c, hipMemcpyAsync((int32_t*)(c.host_pinned + 2), c.dev_counters + 8 + 129, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipStreamSynchronize(s));
     c.have_result = true;
     if (c.p.tree_enabled) {
         const int32_t* cnt = (const int32_t*)(c.host_pinned + 1);
+        if (n >= 2 && ((const int32_t*)(c.host_pinned + 2))[0] != 1)
+            return fail(&c, JTP_ERR_OVERFLOW, "tree deeper than the bottom-up round limit");
         if (cnt[1] == 1) return fail(&c, JTP_ERR_OVERFLOW, "walk stack overflow: a group's traversal exceeded the stack bound");
         if (cnt[1] != 0) return fail(&c, JTP_ERR_STATE, "walk found a malformed hierarchy (code " + std::to_string(cnt[1]) + ")");
     }

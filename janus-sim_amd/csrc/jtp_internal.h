@@ -20,6 +20,7 @@ constexpr int kSortTile = kBlock * kSortIPT;
 constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Morton cell (L/16)
 constexpr int kPolyDeg = 12;         // FP32 split-function polynomial degree
 constexpr int kFftPoints = 4096;     // complex points held in LDS per FFT workgroup
+constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by one wavefront
 
 // ---------------------------------------------------------------- small types
 template <typename R> struct Cplx { R x, y; };
@@ -76,14 +77,18 @@ struct Ctx {
     int32_t *parent_int = nullptr;   // parent of internal node
     int32_t *parent_leaf = nullptr;  // parent of leaf
     int32_t *flags = nullptr;        // bottom-up arrival counters
+    int32_t *frontier0 = nullptr, *frontier1 = nullptr;  // bottom-up frontiers
     void *node_sum = nullptr;        // NodeSum per internal node
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
     int2 *groups = nullptr;
     int32_t *cell4 = nullptr;        // level-4 cell offsets (4097)
-    int32_t *dev_counters = nullptr; // [0] n_groups, [1] error flag, [2] walk queue head
+    int32_t *dev_counters = nullptr; // [0] n_groups, [1] error flag, [2] walk queue head, [3] dense cells,
+                                     // [8 ..] bottom-up round sizes, [8 + 129] root-complete flag
     unsigned long long *dev_inter = nullptr;  // interaction counter
     // PM
     int32_t *cell_start = nullptr;   // G^3 + 1
+    int32_t *dense_slot = nullptr;   // G^3: slot of a dense cell's corner sums (written for dense cells only)
+    void *dense_corner = nullptr;    // real [n / kDenseCell + 1][8] corner sums of dense cells
     void *rho = nullptr;             // real G^3 deposited mass per cell
     void *phi = nullptr;             // real G^3 potential
     void *spec = nullptr;            // complex G^2 (G/2+1)

@@ -30,11 +30,77 @@ __device__ inline R frac_of(uint32_t u, int logG) {
 }
 
 // ------------------------------------------------------------------ deposit
+// Deterministic CIC deposit ("sort + segmented reduction").  The particles are
+// in Morton order and G is a power of two, so every PM cell is a contiguous run
+// (cell_start).  Cells holding more than kDenseCell particles (halo cores hold up
+// to ~1e5) are first reduced by one wavefront each into their 8 corner sums, in a
+// fixed lane/tree order; then every grid node gathers, in a fixed order, the
+// corner sums of dense source cells and the particles of the others.
+template <typename R>
+__device__ inline void cic_corners(const uint4 r, int logG, R q, R w[8]) {
+    const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
+    const R wx[2] = {(R)1 - fx, fx}, wy[2] = {(R)1 - fy, fy}, wz[2] = {(R)1 - fz, fz};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = q * (wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2]);
+}
+
+__global__ void k_dense_mark(const int32_t* __restrict__ cs, int64_t ncells, int32_t* __restrict__ slot,
+                             int32_t* __restrict__ list, int32_t* __restrict__ count) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    if (cs[c + 1] - cs[c] > kDenseCell) {
+        const int32_t k = atomicAdd(count, 1);
+        slot[c] = k;
+        list[k] = (int32_t)c;
+    }
+}
+
+// One wavefront per dense cell: lane l takes particles s + l, s + l + 64, ...; then a
+// fixed butterfly reduction across lanes.  corner index c = dx + 2 dy + 4 dz (offset of
+// the node from the cell's lower corner).
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_reduce(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                                                      const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ count, int logG, R qp, R qm,
+                                                      R* __restrict__ corner) {
+    const int lane = threadIdx.x & 63;
+    const int32_t nd = *count;
+    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd;
+         k += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const int32_t c = list[k];
+        const int32_t s = cs[c], e = cs[c + 1];
+        R acc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = 0;
+        for (int32_t p = s + lane; p < e; p += 64) {
+            const uint4 r = parts[p];
+            R w[8];
+            cic_corners<R>(r, logG, (int32_t)r.w > 0 ? qp : qm, w);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] += w[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            R v = acc[q];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            acc[q] = v;
+        }
+        if (lane < 8) {
+            R v = acc[0];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) v = lane == q ? acc[q] : v;
+            corner[8 * (int64_t)k + lane] = v;
+        }
+    }
+}
+
 // Thread per grid node, nodes enumerated in Morton order so that the eight
 // source cells of a wavefront's nodes are contiguous particle runs.
 template <typename R>
-__global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __restrict__ cs, int logG,
-                          R qp, R qm, R* __restrict__ rho) {
+__global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                          const int32_t* __restrict__ slot, const R* __restrict__ corner, int logG, R qp, R qm,
+                          R* __restrict__ rho) {
     const int64_t G = 1ll << logG;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= G * G * G) return;
@@ -53,6 +119,11 @@ __global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __rest
                                ck = (uint32_t)((k + dz) & msk);
                 const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
                 const int32_t s = cs[cm], e = cs[cm + 1];
+                const int corner_idx = (-dx) + 2 * (-dy) + 4 * (-dz);  // node = cell corner (-dx, -dy, -dz)
+                if (e - s > kDenseCell) {
+                    acc += corner[8 * (int64_t)slot[cm] + corner_idx];
+                    continue;
+                }
                 for (int32_t p = s; p < e; ++p) {
                     const uint4 r = parts[p];
                     const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
@@ -288,13 +359,29 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     const int64_t G3 = c.G3();
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * c.logG, G3, c.cell_start, s);
     if (e != hipSuccess) return e;
+    int32_t* count = c.dev_counters + 3;
+    int32_t* list = c.frontier0;  // free after the tree build; holds <= n / kDenseCell cells
+    e = launch_fill_i32(count, 1, 0, s);
+    if (e != hipSuccess) return e;
     const unsigned nb = (unsigned)((G3 + kBlock - 1) / kBlock);
-    if (c.p.precision == JTP_FP64)
-        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.logG, c.m_plus, -c.m_minus,
-                                                            (double*)c.rho);
-    else
-        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.logG, (float)c.m_plus,
+    k_dense_mark<<<dim3(nb), dim3(kBlock), 0, s>>>(c.cell_start, G3, c.dense_slot, list, count);
+    int64_t nd_max = n / kDenseCell + 1;
+    unsigned nbd = (unsigned)((nd_max * 64 + kBlock - 1) / kBlock);
+    if (nbd > 8192) nbd = 8192;
+    if (c.p.precision == JTP_FP64) {
+        k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG, c.m_plus,
+                                                               -c.m_minus, (double*)c.dense_corner);
+        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot,
+                                                            (const double*)c.dense_corner, c.logG, c.m_plus,
+                                                            -c.m_minus, (double*)c.rho);
+    } else {
+        k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                              (float)c.m_plus, (float)-c.m_minus,
+                                                              (float*)c.dense_corner);
+        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot,
+                                                           (const float*)c.dense_corner, c.logG, (float)c.m_plus,
                                                            (float)-c.m_minus, (float*)c.rho);
+    }
     return hipGetLastError();
 }
 

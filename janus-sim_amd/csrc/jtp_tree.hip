@@ -16,11 +16,13 @@
 
 namespace jtp {
 
-struct NodeSum {
+struct alignas(16) NodeSum {  // 64 bytes: the stage copy reads this layout
     uint64_t sp[3];
     uint64_t sm[3];
     uint32_t np, nm;
+    uint32_t pad[2];
 };
+static_assert(sizeof(NodeSum) == 64, "NodeSum layout");
 
 __device__ inline int clz64_dev(uint64_t x) { return x == 0 ? 64 : __clzll((long long)x); }
 
@@ -107,21 +109,35 @@ __device__ inline void load_child(int32_t c, const uint4* parts, uint4* geo, Nod
     }
 }
 
-// One thread per particle climbs towards the root; the second arrival at a
-// node combines its two children (left, right in that fixed order).
-__global__ void k_bottom_up(const uint4* parts, int64_t n, uint4* geo, const int2* kids,
-                            const int32_t* parent_int, const int32_t* parent_leaf, int32_t* flags,
-                            NodeSum* sums) {
+// Bottom-up pass as a sequence of frontiers, one kernel per tree level.
+// Round 0 holds the nodes whose two children are leaves; a node joins the next
+// frontier when its second child completes (relaxed arrival counter).  Kernel
+// boundaries publish each round's writes to the next, so no device-scope fence
+// is needed inside a kernel (the arrival-counter climb with a fence per level is
+// what the reference does, nbody_gpu_twopass.rs:485-560).  Each node combines its
+// children in the fixed order (left, right), so the result does not depend on
+// which round or thread produced it.
+constexpr int kMaxRounds = 128;
+
+__global__ void k_bu_init(const int2* __restrict__ kids, int64_t nI, int32_t* __restrict__ cnt,
+                          int32_t* __restrict__ f0, int32_t* __restrict__ ctrl) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t p = parent_leaf[i];
-    // every climb ends at the root; the bounds and step checks only guard against a malformed hierarchy
-    for (int64_t step = 0; step < n; ++step) {
-        if (p < 0 || p >= n - 1) return;
-        __threadfence();
-        const int old = atomicAdd(&flags[p], 1);
-        if (old == 0) return;
-        __threadfence();
+    if (i >= nI) return;
+    const int2 k = kids[i];
+    const int leaves = (k.x < 0 ? 1 : 0) + (k.y < 0 ? 1 : 0);
+    cnt[i] = leaves;
+    if (leaves == 2) f0[atomicAdd(&ctrl[0], 1)] = (int32_t)i;
+}
+
+__global__ void k_bu_round(int r, const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
+                           const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+                           int32_t* __restrict__ cnt, NodeSum* __restrict__ sums, const int32_t* __restrict__ fin,
+                           int32_t* __restrict__ fout, int32_t* __restrict__ ctrl) {
+    const int32_t size = ctrl[r];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < size;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = fin[idx];
+        if (p < 0 || p >= nI) continue;
         const int2 k = kids[p];
         ChildData a, b;
         load_child(k.x, parts, geo, sums, a);
@@ -134,12 +150,18 @@ __global__ void k_bottom_up(const uint4* parts, int64_t n, uint4* geo, const int
         }
         s.np = a.np + b.np;
         s.nm = a.nm + b.nm;
+        s.pad[0] = s.pad[1] = 0;
         sums[p] = s;
         const uint4 g0 = geo[2 * p], g1 = geo[2 * p + 1];
         geo[2 * p] = make_uint4(min(a.mn[0], b.mn[0]), min(a.mn[1], b.mn[1]), min(a.mn[2], b.mn[2]), g0.w);
         geo[2 * p + 1] = make_uint4(max(a.mx[0], b.mx[0]), max(a.mx[1], b.mx[1]), max(a.mx[2], b.mx[2]), g1.w);
-        if (p == 0) return;
-        p = parent_int[p];
+        if (p == 0) {
+            ctrl[kMaxRounds + 1] = 1;  // root complete
+            continue;
+        }
+        const int32_t par = parent_int[p];
+        if (par < 0 || par >= nI) continue;
+        if (atomicAdd(&cnt[par], 1) == 1) fout[atomicAdd(&ctrl[r + 1], 1)] = par;
     }
 }
 
@@ -294,8 +316,17 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
         launch_fill_i32(c.parent_leaf, n, -1, s);
         k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, c.node_geo, c.node_kids, c.parent_int,
                                                     c.parent_leaf, c.flags);
-        k_bottom_up<<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, c.node_geo, c.node_kids, c.parent_int,
-                                                      c.parent_leaf, c.flags, (NodeSum*)c.node_sum);
+        int32_t* ctrl = c.dev_counters + 8;
+        launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
+        k_bu_init<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.node_kids, nI, c.flags, c.frontier0, ctrl);
+        int blocks = (int)((nI / 4 + kBlock - 1) / kBlock);
+        blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+        for (int r = 0; r < kMaxRounds; ++r) {
+            int32_t* fin = (r & 1) ? c.frontier1 : c.frontier0;
+            int32_t* fout = (r & 1) ? c.frontier0 : c.frontier1;
+            k_bu_round<<<dim3(blocks), dim3(kBlock), 0, s>>>(r, c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+                                                             c.flags, (NodeSum*)c.node_sum, fin, fout, ctrl);
+        }
         const double qp = c.p.g_constant * c.m_plus, qm = c.p.g_constant * c.m_minus;
         if (c.p.precision == JTP_FP64)
             k_mono_f64<<<dim3(nbI), dim3(kBlock), 0, s>>>((const NodeSum*)c.node_sum, c.node_geo, nI, qp, qm,

@@ -2,7 +2,11 @@
 
 Each generator returns a ``Workload`` with host positions in [-L/2, L/2)
 (float64, shape [n, 3]), species (+1 / -1, int8) and the force parameters.
-The softening default follows config.rs:191-196 (L / N^{1/3} / 30 with N the
+Only IEEE-exact numpy operations (+ - * / sqrt, sorting, bincount, FFT) touch
+per-particle data; transcendental functions are evaluated with the scalar
+``math`` module on small tables, so a seed gives bit-identical particles on any
+host of this image (numpy's SIMD transcendental ufuncs may differ in the last bit
+between CPU models).  The softening default follows config.rs:191-196 (L / N^{1/3} / 30 with N the
 total particle count); the split scale is r_s = 1.25 cells with r_cut = 4.5 r_s.
 
 * ``lattice_zeldovich``   - lattice + Zel'dovich displacement from a Gaussian field with a
@@ -13,6 +17,8 @@ total particle count); the split scale is r_s = 1.25 cells with r_cut = 4.5 r_s.
 """
 from __future__ import annotations
 
+import hashlib
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -38,6 +44,13 @@ class Workload:
     def n(self):
         return len(self.species)
 
+    def digest(self, pos_fixed: np.ndarray) -> str:
+        """SHA-256 of the fixed-point positions and species (identifies the exact inputs)."""
+        h = hashlib.sha256()
+        h.update(np.ascontiguousarray(pos_fixed, np.uint32).tobytes())
+        h.update(np.ascontiguousarray(self.species, np.int8).tobytes())
+        return h.hexdigest()
+
 
 def default_softening(box_size: float, n_total: int) -> float:
     return box_size / n_total ** (1.0 / 3.0) / 30.0
@@ -58,7 +71,10 @@ def zeldovich_displacement(n: int, box_size: float, seed: int, index: float = -2
     KX, KY, KZ = np.meshgrid(kx, kx, kz, indexing="ij")
     k2 = KX ** 2 + KY ** 2 + KZ ** 2
     k2[0, 0, 0] = 1.0
-    amp = k2 ** (index / 4.0)              # sqrt(P_psi(k)) with P_psi ~ k^index
+    # sqrt(P_psi(k)) with P_psi ~ k^index, from a table over the integer values of k^2
+    k2i = np.rint(k2).astype(np.int64)
+    table = np.array([0.0] + [math.pow(float(v), index / 4.0) for v in range(1, int(k2i.max()) + 1)])
+    amp = table[k2i]
     amp[0, 0, 0] = 0.0
     kmag = np.sqrt(k2)
     psi = []
@@ -90,9 +106,10 @@ def lattice_zeldovich(n_side: int = 64, box_size: float = 250.0, n_cells: int = 
 
 
 def _nfw_inverse_cdf(c: float, u: np.ndarray) -> np.ndarray:
+    """Radius / r_200 with enclosed-mass fraction u for an NFW profile of concentration c."""
     x = np.linspace(0.0, 1.0, 2049)
-    mu = lambda t: np.log1p(c * t) - c * t / (1.0 + c * t)
-    cdf = mu(x) / mu(1.0)
+    mu = np.array([math.log1p(c * t) - c * t / (1.0 + c * t) for t in x])
+    cdf = mu / mu[-1]
     return np.interp(u, cdf, x)
 
 
@@ -123,14 +140,14 @@ def clustered(n_total: int = 22_000_000, box_size: float = 250.0, n_cells: int =
     a, lo, hi = -1.9, 1e-5, 1e-2
     u = rng.random(n_halos)
     e = a + 1.0
-    mh = (lo ** e + u * (hi ** e - lo ** e)) ** (1.0 / e)
+    mh = np.array([math.pow(math.pow(lo, e) + v * (math.pow(hi, e) - math.pow(lo, e)), 1.0 / e) for v in u])
     n_in_halos = int(round(halo_fraction * n_plus))
     counts = np.floor(n_in_halos * mh / mh.sum()).astype(np.int64)
     counts[: n_in_halos - counts.sum()] += 1
     conc = rng.uniform(5.0, 15.0, n_halos)
     centres = rng.uniform(-L / 2, L / 2, (n_halos, 3))
     rho_bar = n_plus / L ** 3
-    r200 = (3.0 * counts / (800.0 * np.pi * rho_bar)) ** (1.0 / 3.0)
+    r200 = np.array([math.pow(3.0 * float(c) / (800.0 * math.pi * rho_bar), 1.0 / 3.0) for c in counts])
     parts = []
     for h in range(n_halos):
         k = int(counts[h])
@@ -146,11 +163,13 @@ def clustered(n_total: int = 22_000_000, box_size: float = 250.0, n_cells: int =
     # m-: rejection sampling where the smoothed m+ density is below its median
     G = n_cells
     rho = _cic_density(p_plus, L, G)
+    sig = 2.0  # cells; separable Gaussian filter exp(-sig^2 k^2 / 2)
     k = np.fft.fftfreq(G) * 2 * np.pi
-    kz = np.fft.rfftfreq(G) * 2 * np.pi
-    KX, KY, KZ = np.meshgrid(k, k, kz, indexing="ij")
-    sig = 2.0  # cells
-    sm = np.fft.irfftn(np.fft.rfftn(rho) * np.exp(-0.5 * sig ** 2 * (KX ** 2 + KY ** 2 + KZ ** 2)), s=(G, G, G))
+    gk = np.array([math.exp(-0.5 * sig ** 2 * float(v) ** 2) for v in k])
+    gz = gk[: G // 2 + 1].copy()
+    gz[G // 2] = gk[G // 2]
+    filt = gk[:, None, None] * gk[None, :, None] * gz[None, None, :]
+    sm = np.fft.irfftn(np.fft.rfftn(rho) * filt, s=(G, G, G))
     med = np.median(sm)
     low = sm < med
     out = []

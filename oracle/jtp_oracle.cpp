@@ -326,12 +326,17 @@ static inline double split_factor(int mode, double r2, double r2s, double rcut2,
     return S;
 }
 
+}  // extern "C"
+
 struct CellList {
     int nc = 1;
     std::vector<int64_t> start, idx;
-    void build(int64_t n, const uint32_t* pos, double L, double rcut) {
-        nc = (int)std::floor(L / rcut);
-        if (nc < 3) nc = 1;
+    int reach = 1;  // neighbour cells searched on each side
+    void build(int64_t n, const uint32_t* pos, double L, double rcut, int sub = 1) {
+        // cells of edge >= rcut / sub, searched +-sub cells around the target's cell
+        reach = sub;
+        nc = (int)std::floor(sub * L / rcut);
+        if (nc < 2 * sub + 1) { nc = 1; reach = 0; }
         if (nc > 256) nc = 256;
         const int64_t nc3 = (int64_t)nc * nc * nc;
         start.assign(nc3 + 1, 0);
@@ -347,6 +352,17 @@ struct CellList {
         std::vector<int64_t> fill(start.begin(), start.end() - 1);
         for (int64_t p = 0; p < n; ++p) idx[fill[cell[p]]++] = p;
     }
+    template <class F>
+    void for_neighbours(const uint32_t* u, F&& f) const {
+        const int64_t c = cell_of(u);
+        const int64_t cx = c % nc, cy = (c / nc) % nc, cz = c / ((int64_t)nc * nc);
+        for (int dz = -reach; dz <= reach; ++dz)
+            for (int dy = -reach; dy <= reach; ++dy)
+                for (int dx = -reach; dx <= reach; ++dx) {
+                    const int64_t q = ((cx + dx + nc) % nc) + nc * (((cy + dy + nc) % nc) + nc * ((cz + dz + nc) % nc));
+                    for (int64_t s = start[q]; s < start[q + 1]; ++s) f(idx[s]);
+                }
+    }
     int64_t cell_of(const uint32_t* u) const {
         int64_t c[3];
         for (int a = 0; a < 3; ++a) c[a] = (int64_t)(((uint64_t)u[a] * (uint64_t)nc) >> 32);
@@ -354,10 +370,12 @@ struct CellList {
     }
 };
 
+extern "C" {
+
 void orc_short_direct(int64_t n, const uint32_t* pos, const int8_t* sp, double qp, double qm, double L, double eps,
                       double rcut, double rs, int mode, int64_t nt, const int64_t* tidx, double* acc) {
     CellList cl;
-    cl.build(n, pos, L, rcut);
+    cl.build(n, pos, L, rcut, 2);
     const int nc = cl.nc;
     const double rcut2 = rcut * rcut, eps2 = eps * eps;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -377,14 +395,7 @@ void orc_short_direct(int64_t n, const uint32_t* pos, const int8_t* sp, double q
         if (nc == 1) {
             for (int64_t j = 0; j < n; ++j) visit(j);
         } else {
-            int64_t c = cl.cell_of(ui);
-            const int64_t cx = c % nc, cy = (c / nc) % nc, cz = c / ((int64_t)nc * nc);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int64_t q = ((cx + dx + nc) % nc) + nc * (((cy + dy + nc) % nc) + nc * ((cz + dz + nc) % nc));
-                        for (int64_t s = cl.start[q]; s < cl.start[q + 1]; ++s) visit(cl.idx[s]);
-                    }
+            cl.for_neighbours(ui, visit);
         }
         const double sg = sp[i] > 0 ? 1.0 : -1.0;
         for (int ax = 0; ax < 3; ++ax) acc[3 * t + ax] = sg * a[ax];
@@ -454,7 +465,7 @@ void orc_ewald(int64_t n, const uint32_t* pos, const int8_t* sp, double qp, doub
     }
     // real space via cell list with cell >= rc
     CellList cl;
-    cl.build(n, pos, L, rc);
+    cl.build(n, pos, L, rc, 2);
     const int nc = cl.nc;
     // image shells beyond the minimum image (none needed when rc <= L/2)
     const int nimg = rc <= 0.5 * L ? 0 : (int)std::ceil(rc / L);
@@ -488,14 +499,7 @@ void orc_ewald(int64_t n, const uint32_t* pos, const int8_t* sp, double qp, doub
         if (nc == 1 || nimg > 0) {
             for (int64_t j = 0; j < n; ++j) visit(j);
         } else {
-            const int64_t c = cl.cell_of(ui);
-            const int64_t cx = c % nc, cy = (c / nc) % nc, cz = c / ((int64_t)nc * nc);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int64_t q = ((cx + dx + nc) % nc) + nc * (((cy + dy + nc) % nc) + nc * ((cz + dz + nc) % nc));
-                        for (int64_t s = cl.start[q]; s < cl.start[q + 1]; ++s) visit(cl.idx[s]);
-                    }
+            cl.for_neighbours(ui, visit);
         }
         // reciprocal: -(4pi/L^3) sum k/k^2 e^{..} [sin(k.xi) C(k) - cos(k.xi) S(k)]
         const double xi[3] = {ui[0] * sc, ui[1] * sc, ui[2] * sc};

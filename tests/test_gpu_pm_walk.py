@@ -136,3 +136,19 @@ def test_walk_deterministic(jtp):
     a1 = ctx.compute_forces(pos, w.species)
     a2 = ctx.compute_forces(pos, w.species)
     assert np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_deposit_dense_cells(jtp, orc, precision):
+    """Halo cores put > 32 particles in a cell: the wavefront pre-reduction path must agree with the oracle."""
+    print(f"case: deposit dense cells precision={precision}", flush=True)
+    w = synth.clustered(n_total=400_000, box_size=100.0, n_cells=32, n_halos=20, seed=17)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=precision, tree_enabled=0)
+    ctx.compute_forces(pos, w.species)
+    rho = ctx.stage(jtp.STAGE_RHO).astype(np.float64)
+    rho_ref = orc.deposit(pos, w.species, w.m_plus, w.m_minus_abs, w.n_cells)
+    counts = np.bincount(orc.morton(pos) >> np.uint64(63 - 15), minlength=32 ** 3)
+    assert counts.max() > 1000  # the dense path is exercised
+    tol = 1e-12 if precision == 1 else 3e-6
+    assert np.max(np.abs(rho - rho_ref)) <= tol * np.max(np.abs(rho_ref))
