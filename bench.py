@@ -31,10 +31,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+OVERRIDES = {}
+
+
 def make_ctx(w, precision):
     p = j.default_params(w.box_size, w.n_cells)
     p.r_s, p.r_cut, p.theta, p.softening, p.leaf_size = w.r_s, w.r_cut, w.theta, w.softening, w.leaf_size
     p.precision = precision
+    for k, v in OVERRIDES.items():
+        setattr(p, k, v)
     return j.ForceContext(p, w.m_plus, w.m_minus_abs)
 
 
@@ -136,7 +141,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", default="clustered_22M", choices=sorted(WORKLOADS))
     ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
+    ap.add_argument("--set", action="append", default=[], metavar="FIELD=VALUE",
+                    help="override a jtp_params field (e.g. interlace=0, theta_cut=0.5, theta=0.4)")
     args = ap.parse_args()
+    for kv in args.set:
+        k, v = kv.split("=")
+        OVERRIDES[k] = float(v) if k in ("theta", "theta_cut", "r_s", "r_cut", "softening") else int(v)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the TreePM step is single-GPU; "
               "multi-GPU sharding is design-only) -- run with --gpus 1", file=sys.stderr)
@@ -172,7 +182,7 @@ def main():
                                f"r_cut=4.5 r_s, theta={w.theta}, leaf {w.leaf_size}, m+=1, |m-|=19; "
                                f"FP32 fixed-point form",
                    "n_particles": w.n, "pm_grid": w.n_cells, "theta": w.theta, "r_s": w.r_s, "r_cut": w.r_cut,
-                   "softening": w.softening, "parallelism": "single GPU"},
+                   "softening": w.softening, "parallelism": "single GPU", "overrides": OVERRIDES},
     }
     walk_ms = ph32["ms_walk"]
     achieved = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12

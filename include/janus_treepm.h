@@ -61,10 +61,19 @@ typedef struct jtp_params {
     int32_t precision;   /* JTP_FP32 / JTP_FP64                                      */
     int32_t split_mode;  /* JTP_SPLIT_*                                              */
     int32_t gradient;    /* JTP_GRAD_*                                               */
-    int32_t deconvolve;  /* 1: divide the Green function by the CIC window W(k)^2    */
+    int32_t deconvolve;  /* 1: divide the Green function by the CIC window W(k)^2 and match it to
+                            the gradient stencil D: G *= (k . D) / max(|D|^2, |k|^2/4) (least
+                            squares; the floor only acts near the Nyquist planes)          */
     int32_t pm_enabled;  /* 0: skip the long-range part                              */
     int32_t tree_enabled;/* 0: skip the short-range part                             */
+    int32_t interlace;   /* 1: average the PM force of two CIC grids offset by half a cell
+                            (cancels the leading aliasing error; one extra deposit/solve/interp) */
+    double  theta_cut;   /* opening angle for nodes that straddle r_cut, >= 0 (0 = always open
+                            them); the walk uses min(theta, theta_cut)                  */
 } jtp_params;
+/* Walk acceptance: a node's two monopoles are used when extent < theta * gap, gap being the
+ * periodic group-box to node-box distance; if the node also reaches beyond r_cut from the
+ * group box, it needs extent < theta_cut * gap. */
 
 /* Per-phase timings of the last jtp_compute_forces (HIP events on the call's stream). */
 typedef struct jtp_stats {
@@ -92,12 +101,15 @@ typedef struct jtp_stats {
 #define JTP_STAGE_ACC_PM    7 /* real[n][3] long-range acceleration, ORIGINAL particle order    */
 #define JTP_STAGE_SUMS      8 /* uint64[n-1][8]: n+, sum x+ y+ z+, n-, sum x- y- z- (exact, fixed units) */
 #define JTP_STAGE_BOXES     9 /* uint32[n-1][6] node bounding boxes bmin xyz, bmax xyz (fixed)  */
+#define JTP_STAGE_RHO2     10 /* real[G^3]  interlaced grid: mass deposited at positions + h/2    */
+#define JTP_STAGE_PHI2     11 /* real[G^3]  potential of the interlaced grid                      */
 
 /* ---- lifecycle (reference: CuFFTPoisson::new / drop, cufft_ffi.rs:85-118,339-353;
  *      TreePMForce::new, treepm_force.rs:35-51) */
 typedef struct jtp_ctx jtp_ctx;
 /* Defaults: r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, softening = L/G/30,
- * G = 1, leaf 8, group 64, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on. */
+ * G = 1, leaf 8, group 64, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on,
+ * interlacing on, theta_cut = 0.7 theta. */
 void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells);
 int  jtp_create(const jtp_params* p, jtp_ctx** out);
 int  jtp_destroy(jtp_ctx* ctx);

@@ -137,7 +137,7 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.flags, cap));
     chk(dalloc(c.frontier0, cap));
     chk(dalloc(c.frontier1, cap));
-    chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * 8 * rs));
+    chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * (8 + 27) * rs));
     chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));
     chk(dalloc_bytes(c.node_mono, (size_t)cap * (rs == 8 ? 64 : 32)));
     chk(dalloc(c.groups, cap + 4096 + 64));
@@ -168,6 +168,8 @@ static int check_params(const jtp_params* p, std::string& msg) {
     if (p->split_mode != JTP_SPLIT_POLY4 && !(p->r_s > 0)) { msg = "erfc split modes need r_s > 0"; return JTP_ERR_ARG; }
     if (p->gradient != JTP_GRAD_4PT && p->gradient != JTP_GRAD_2PT) { msg = "bad gradient"; return JTP_ERR_ARG; }
     if (p->k_min < 0) { msg = "k_min must be >= 0"; return JTP_ERR_ARG; }
+    if (p->interlace != 0 && p->interlace != 1) { msg = "interlace must be 0 or 1"; return JTP_ERR_ARG; }
+    if (!(p->theta_cut >= 0) || !std::isfinite(p->theta_cut)) { msg = "theta_cut must be >= 0"; return JTP_ERR_ARG; }
     return JTP_OK;
 }
 
@@ -203,6 +205,8 @@ void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
     p->deconvolve = 1;
     p->pm_enabled = 1;
     p->tree_enabled = 1;
+    p->interlace = 1;
+    p->theta_cut = 0.7 * p->theta;
 }
 
 int jtp_create(const jtp_params* p, jtp_ctx** out) {
@@ -230,9 +234,11 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     const int64_t G3 = c.G3();
     e = dalloc_bytes(c.rho, G3 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.phi, G3 * rs);
+    if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.rho2, G3 * rs);
+    if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.phi2, G3 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.spec, (size_t)G * G * c.Hc() * 2 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.twiddle, (size_t)2 * G * rs);
-    if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)2 * G * rs);
+    if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)4 * G * rs);
     if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
     if (e == hipSuccess) e = dalloc(c.dev_counters, 8 + 130 + 8);
@@ -260,8 +266,8 @@ int jtp_destroy(jtp_ctx* h) {
     Ctx& c = h->c;
     (void)hipDeviceSynchronize();
     free_particle_buffers(c);
-    void* ptrs[] = {c.rho, c.phi, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4, c.dev_counters, c.dev_inter,
-                    c.dense_slot};
+    void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4,
+                    c.dev_counters, c.dev_inter, c.dense_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);
@@ -311,6 +317,7 @@ int jtp_compute_forces(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_
     if (c.p.pm_enabled) JTP_HIP(&c, pm_deposit(c, n, s));
     JTP_HIP(&c, hipEventRecord(c.ev[3], s));
     if (c.p.pm_enabled) JTP_HIP(&c, pm_fft_solve(c, c.rho, c.phi, s));
+    if (c.p.pm_enabled && c.p.interlace) JTP_HIP(&c, pm_fft_solve(c, c.rho2, c.phi2, s));
     JTP_HIP(&c, hipEventRecord(c.ev[4], s));
     if (c.p.pm_enabled) JTP_HIP(&c, pm_interp(c, n, s));
     JTP_HIP(&c, hipEventRecord(c.ev[5], s));
@@ -402,6 +409,8 @@ int jtp_stage_size(jtp_ctx* h, int32_t stage, size_t* bytes) {
         case JTP_STAGE_SUMS: *bytes = nI * 64; break;
         case JTP_STAGE_RHO:
         case JTP_STAGE_PHI: *bytes = c.G3() * rs; break;
+        case JTP_STAGE_RHO2:
+        case JTP_STAGE_PHI2: *bytes = c.p.interlace ? c.G3() * rs : 0; break;
         case JTP_STAGE_ACC_PM: *bytes = n * 3 * rs; break;
         case JTP_STAGE_BOXES: *bytes = nI * 24; break;
         default: return fail(&c, JTP_ERR_ARG, "unknown stage");
@@ -426,6 +435,8 @@ int jtp_stage_copy(jtp_ctx* h, int32_t stage, void* host_dst, size_t bytes) {
         case JTP_STAGE_PERM: JTP_HIP(&c, hipMemcpy(host_dst, c.vals0, need, hipMemcpyDeviceToHost)); break;
         case JTP_STAGE_RHO: JTP_HIP(&c, hipMemcpy(host_dst, c.rho, need, hipMemcpyDeviceToHost)); break;
         case JTP_STAGE_PHI: JTP_HIP(&c, hipMemcpy(host_dst, c.phi, need, hipMemcpyDeviceToHost)); break;
+        case JTP_STAGE_RHO2: JTP_HIP(&c, hipMemcpy(host_dst, c.rho2, need, hipMemcpyDeviceToHost)); break;
+        case JTP_STAGE_PHI2: JTP_HIP(&c, hipMemcpy(host_dst, c.phi2, need, hipMemcpyDeviceToHost)); break;
         case JTP_STAGE_NODES:
         case JTP_STAGE_BOXES: {
             if (!c.p.tree_enabled) return fail(&c, JTP_ERR_STATE, "tree disabled");

@@ -91,6 +91,7 @@ struct Ctx {
     void *dense_corner = nullptr;    // real [n / kDenseCell + 1][8] corner sums of dense cells
     void *rho = nullptr;             // real G^3 deposited mass per cell
     void *phi = nullptr;             // real G^3 potential
+    void *rho2 = nullptr, *phi2 = nullptr;  // interlaced grid (nodes at the cell centres of the first)
     void *spec = nullptr;            // complex G^2 (G/2+1)
     void *twiddle = nullptr;         // complex G  e^{-2 pi i m/G}
     void *gtab = nullptr;            // real 2G: per-axis window/Gaussian factor, k_i^2

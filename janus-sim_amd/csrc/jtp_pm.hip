@@ -56,43 +56,113 @@ __global__ void k_dense_mark(const int32_t* __restrict__ cs, int64_t ncells, int
 }
 
 // One wavefront per dense cell: lane l takes particles s + l, s + l + 64, ...; then a
-// fixed butterfly reduction across lanes.  corner index c = dx + 2 dy + 4 dz (offset of
-// the node from the cell's lower corner).
+// fixed butterfly reduction across lanes.  Unshifted corner index c = dx + 2 dy + 4 dz
+// (offset of the node from the cell's lower corner).  With interlacing, also the 27 sums
+// for the shifted grid, whose nodes sit at the centres of cells c-1, c, c+1 per axis:
+// index (ex+1) + 3 (ey+1) + 9 (ez+1) for the node at the centre of cell c + e.
 template <typename R>
+__device__ inline void shifted_weights(R f, R w[3]) {
+    const R h = (R)0.5;
+    w[0] = f < h ? h - f : (R)0;        // node at the centre of the cell below
+    w[1] = f < h ? h + f : (R)1.5 - f;  // 1 - |f - 1/2|
+    w[2] = f < h ? (R)0 : f - h;        // node at the centre of the cell above
+}
+
+template <typename R, bool SHIFTED>
 __global__ __launch_bounds__(256) void k_dense_reduce(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
                                                       const int32_t* __restrict__ list,
                                                       const int32_t* __restrict__ count, int logG, R qp, R qm,
                                                       R* __restrict__ corner) {
+    constexpr int NC = SHIFTED ? 27 : 8;
     const int lane = threadIdx.x & 63;
     const int32_t nd = *count;
     for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd;
          k += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int32_t c = list[k];
         const int32_t s = cs[c], e = cs[c + 1];
-        R acc[8];
+        R acc[NC];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] = 0;
+        for (int q = 0; q < NC; ++q) acc[q] = 0;
         for (int32_t p = s + lane; p < e; p += 64) {
             const uint4 r = parts[p];
-            R w[8];
-            cic_corners<R>(r, logG, (int32_t)r.w > 0 ? qp : qm, w);
+            const R q = (int32_t)r.w > 0 ? qp : qm;
+            if (SHIFTED) {
+                R wx[3], wy[3], wz[3];
+                shifted_weights<R>(frac_of<R>(r.x, logG), wx);
+                shifted_weights<R>(frac_of<R>(r.y, logG), wy);
+                shifted_weights<R>(frac_of<R>(r.z, logG), wz);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) acc[q] += w[q];
+                for (int cz = 0; cz < 3; ++cz)
+#pragma unroll
+                    for (int cy = 0; cy < 3; ++cy)
+#pragma unroll
+                        for (int cx = 0; cx < 3; ++cx) acc[cx + 3 * cy + 9 * cz] += q * (wx[cx] * wy[cy] * wz[cz]);
+            } else {
+                R w[8];
+                cic_corners<R>(r, logG, q, w);
+#pragma unroll
+                for (int q2 = 0; q2 < 8; ++q2) acc[q2] += w[q2];
+            }
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < NC; ++q) {
             R v = acc[q];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
             acc[q] = v;
         }
-        if (lane < 8) {
+        if (lane < NC) {
             R v = acc[0];
 #pragma unroll
-            for (int q = 1; q < 8; ++q) v = lane == q ? acc[q] : v;
-            corner[8 * (int64_t)k + lane] = v;
+            for (int q = 1; q < NC; ++q) v = lane == q ? acc[q] : v;
+            corner[NC * (int64_t)k + lane] = v;
         }
     }
+}
+
+// Shifted grid (interlacing): node n' sits at the centre of cell m = n' - 1 and gathers the
+// particles of cells m-1, m, m+1 per axis with weight 1 - |x - x_node| / h, i.e. per axis
+// (d = cell offset from m, f = fraction in that cell): d = -1: max(0, f - 1/2); d = 0: 1 - |f - 1/2|;
+// d = +1: max(0, 1/2 - f).
+template <typename R>
+__global__ void k_deposit_shifted(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                                  const int32_t* __restrict__ slot, const R* __restrict__ corner27, int logG, R qp,
+                                  R qm, R* __restrict__ rho2) {
+    const int64_t G = 1ll << logG;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= G * G * G) return;
+    const int32_t i = (int32_t)compact3_21((uint64_t)m);
+    const int32_t j = (int32_t)compact3_21((uint64_t)m >> 1);
+    const int32_t k = (int32_t)compact3_21((uint64_t)m >> 2);
+    const int32_t msk = (int32_t)(G - 1);
+    R acc = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const uint32_t ci = (uint32_t)((i + dx) & msk), cj = (uint32_t)((j + dy) & msk),
+                               ck = (uint32_t)((k + dz) & msk);
+                const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
+                const int32_t s = cs[cm], e = cs[cm + 1];
+                if (e - s > kDenseCell) {
+                    // this node is the centre of cell (cell - d): shifted index e = -d
+                    acc += corner27[27 * (int64_t)slot[cm] + (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz)];
+                    continue;
+                }
+                for (int32_t p = s; p < e; ++p) {
+                    const uint4 r = parts[p];
+                    R wx[3], wy[3], wz[3];
+                    shifted_weights<R>(frac_of<R>(r.x, logG), wx);
+                    shifted_weights<R>(frac_of<R>(r.y, logG), wy);
+                    shifted_weights<R>(frac_of<R>(r.z, logG), wz);
+                    const R q = (int32_t)r.w > 0 ? qp : qm;
+                    acc += q * (wx[1 - dx] * wy[1 - dy] * wz[1 - dz]);
+                }
+            }
+        }
+    }
+    const int64_t ii = (i + 1) & msk, jj = (j + 1) & msk, kk = (k + 1) & msk;
+    rho2[(kk * G + jj) * G + ii] = acc;
 }
 
 // Thread per grid node, nodes enumerated in Morton order so that the eight
@@ -292,7 +362,7 @@ __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, int l
 template <typename R>
 __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec, int logG,
                                                      const Cplx<R>* __restrict__ tw, const R* __restrict__ gtab,
-                                                     R norm, int k_min) {
+                                                     R norm, int k_min, int matched) {
     __shared__ Cplx<R> buf[kFftPoints];
     const int G = 1 << logG, H = G / 2 + 1;
     const int KB = kFftPoints / G;
@@ -309,6 +379,8 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
     lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, false);
     const R* T = gtab;
     const R* K2 = gtab + G;
+    const R* Ks = gtab + 2 * G;  // signed k_i
+    const R* D = gtab + 3 * G;   // transfer function of the gradient stencil along one axis
     const int ky = y;
     const int aky = ky <= G / 2 ? ky : G - ky;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
@@ -318,8 +390,16 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
         const int akx = kx;
         const int kmax = max(akx, max(aky, akz));
         R f = 0;
-        if (kx < H && !(akx == 0 && aky == 0 && akz == 0) && kmax >= k_min)
+        if (kx < H && !(akx == 0 && aky == 0 && akz == 0) && kmax >= k_min) {
             f = norm * T[kx] * T[ky] * T[kz] / (K2[kx] + K2[ky] + K2[kz]);
+            if (matched) {
+                // least-squares match of the stencil gradient D to i k: f *= (k . D) / |D|^2, with the
+                // denominator floored at |k|^2 / 4 where D vanishes (Nyquist planes)
+                const R dd = D[kx] * D[kx] + D[ky] * D[ky] + D[kz] * D[kz];
+                const R fl = (R)0.25 * (K2[kx] + K2[ky] + K2[kz]);
+                f *= (Ks[kx] * D[kx] + Ks[ky] * D[ky] + Ks[kz] * D[kz]) / (dd > fl ? dd : fl);
+            }
+        }
         Cplx<R> v = buf[kz * KB + jb];
         buf[kz * KB + jb] = {v.x * f, v.y * f};
     }
@@ -344,7 +424,7 @@ static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s) {
     k_fft_x_r2c<R><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, logG, tw);
     k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 0);
     k_fft_z_green<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, (const R*)c.gtab,
-                                                                     (R)c.green_norm, c.p.k_min);
+                                                                     (R)c.green_norm, c.p.k_min, c.p.deconvolve);
     k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 1);
     k_fft_x_c2r<R><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, logG, tw);
     return hipGetLastError();
@@ -368,19 +448,34 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     int64_t nd_max = n / kDenseCell + 1;
     unsigned nbd = (unsigned)((nd_max * 64 + kBlock - 1) / kBlock);
     if (nbd > 8192) nbd = 8192;
+    const int64_t nd_cap = n / kDenseCell + 1;
     if (c.p.precision == JTP_FP64) {
-        k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG, c.m_plus,
-                                                               -c.m_minus, (double*)c.dense_corner);
-        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot,
-                                                            (const double*)c.dense_corner, c.logG, c.m_plus,
+        double* c8 = (double*)c.dense_corner;
+        double* c27 = c8 + 8 * nd_cap;
+        k_dense_reduce<double, false><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                      c.m_plus, -c.m_minus, c8);
+        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, c.m_plus,
                                                             -c.m_minus, (double*)c.rho);
+        if (c.p.interlace) {
+            k_dense_reduce<double, true><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                         c.m_plus, -c.m_minus, c27);
+            k_deposit_shifted<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
+                                                                        c.logG, c.m_plus, -c.m_minus, (double*)c.rho2);
+        }
     } else {
-        k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                              (float)c.m_plus, (float)-c.m_minus,
-                                                              (float*)c.dense_corner);
-        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot,
-                                                           (const float*)c.dense_corner, c.logG, (float)c.m_plus,
-                                                           (float)-c.m_minus, (float*)c.rho);
+        float* c8 = (float*)c.dense_corner;
+        float* c27 = c8 + 8 * nd_cap;
+        const float qp = (float)c.m_plus, qm = (float)-c.m_minus;
+        k_dense_reduce<float, false><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG, qp,
+                                                                     qm, c8);
+        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, qp, qm,
+                                                           (float*)c.rho);
+        if (c.p.interlace) {
+            k_dense_reduce<float, true><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                        qp, qm, c27);
+            k_deposit_shifted<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
+                                                                       c.logG, qp, qm, (float*)c.rho2);
+        }
     }
     return hipGetLastError();
 }
@@ -400,16 +495,13 @@ __device__ inline R dstencil(const R* __restrict__ phi, int64_t b, int64_t st, i
 }
 
 template <typename R, bool FOUR>
-__global__ void k_interp(const uint4* __restrict__ parts, int64_t n, const R* __restrict__ phi, int logG, R inv,
-                         R* __restrict__ acc) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ inline void interp_grid(const uint4 r, const R* __restrict__ phi, int logG, R inv, uint32_t shift, R a[3]) {
     const int G = 1 << logG;
     const int64_t G2 = (int64_t)G * G;
-    const uint4 r = parts[i];
+    const uint32_t ux = r.x + shift, uy = r.y + shift, uz = r.z + shift;
     const int sh = 32 - logG;
-    const int ix = (int)(r.x >> sh), iy = (int)(r.y >> sh), iz = (int)(r.z >> sh);
-    const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
+    const int ix = (int)(ux >> sh), iy = (int)(uy >> sh), iz = (int)(uz >> sh);
+    const R fx = frac_of<R>(ux, logG), fy = frac_of<R>(uy, logG), fz = frac_of<R>(uz, logG);
     const R wx[2] = {(R)1 - fx, fx}, wy[2] = {(R)1 - fy, fy}, wz[2] = {(R)1 - fz, fz};
     R ax = 0, ay = 0, az = 0;
 #pragma unroll
@@ -422,19 +514,38 @@ __global__ void k_interp(const uint4* __restrict__ parts, int64_t n, const R* __
             for (int dx = 0; dx < 2; ++dx) {
                 const int cx = (ix + dx) & (G - 1);
                 const R w = wx[dx] * wy[dy] * wz[dz];
-                const R gx = dstencil<R, FOUR>(phi, (int64_t)cz * G2 + (int64_t)cy * G, 1, cx, G, inv);
-                const R gy = dstencil<R, FOUR>(phi, (int64_t)cz * G2 + cx, G, cy, G, inv);
-                const R gz = dstencil<R, FOUR>(phi, (int64_t)cy * G + cx, G2, cz, G, inv);
-                ax += w * gx;
-                ay += w * gy;
-                az += w * gz;
+                ax += w * dstencil<R, FOUR>(phi, (int64_t)cz * G2 + (int64_t)cy * G, 1, cx, G, inv);
+                ay += w * dstencil<R, FOUR>(phi, (int64_t)cz * G2 + cx, G, cy, G, inv);
+                az += w * dstencil<R, FOUR>(phi, (int64_t)cy * G + cx, G2, cz, G, inv);
             }
         }
     }
-    const R sg = (int32_t)r.w > 0 ? (R)-1 : (R)1;
-    acc[3 * i] = sg * ax;
-    acc[3 * i + 1] = sg * ay;
-    acc[3 * i + 2] = sg * az;
+    a[0] = ax;
+    a[1] = ay;
+    a[2] = az;
+}
+
+// With interlacing, the shifted grid is sampled at x + h/2 (its nodes are the cell centres of
+// the first grid) and the two accelerations are averaged.
+template <typename R, bool FOUR>
+__global__ void k_interp(const uint4* __restrict__ parts, int64_t n, const R* __restrict__ phi,
+                         const R* __restrict__ phi2, int logG, R inv, R* __restrict__ acc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 r = parts[i];
+    R a[3];
+    interp_grid<R, FOUR>(r, phi, logG, inv, 0u, a);
+    R sg = (int32_t)r.w > 0 ? (R)-1 : (R)1;
+    if (phi2) {
+        R b[3];
+        interp_grid<R, FOUR>(r, phi2, logG, inv, 1u << (31 - logG), b);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) a[ax] += b[ax];
+        sg *= (R)0.5;
+    }
+    acc[3 * i] = sg * a[0];
+    acc[3 * i + 1] = sg * a[1];
+    acc[3 * i + 2] = sg * a[2];
 }
 
 hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s) {
@@ -443,30 +554,35 @@ hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s) {
     const double h = c.p.box_size / (double)c.G();
     const bool four = c.p.gradient == JTP_GRAD_4PT;
     const double inv = four ? 1.0 / (12.0 * h) : 1.0 / (2.0 * h);
+    const void* phi2 = c.p.interlace ? c.phi2 : nullptr;
     if (c.p.precision == JTP_FP64) {
-        if (four) k_interp<double, true><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const double*)c.phi, c.logG, inv, (double*)c.pm_acc);
-        else k_interp<double, false><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const double*)c.phi, c.logG, inv, (double*)c.pm_acc);
+        auto k = four ? k_interp<double, true> : k_interp<double, false>;
+        k<<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const double*)c.phi, (const double*)phi2, c.logG, inv,
+                                            (double*)c.pm_acc);
     } else {
-        if (four) k_interp<float, true><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const float*)c.phi, c.logG, (float)inv, (float*)c.pm_acc);
-        else k_interp<float, false><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const float*)c.phi, c.logG, (float)inv, (float*)c.pm_acc);
+        auto k = four ? k_interp<float, true> : k_interp<float, false>;
+        k<<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const float*)c.phi, (const float*)phi2, c.logG, (float)inv,
+                                            (float*)c.pm_acc);
     }
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ tables (host, FP64)
-// Twiddles e^{-2 pi i m/G}; per-axis Green factors T[i] = exp(-k_i^2 r_s^2) / sinc(k_i h/2)^4
-// (the sinc^4 only with deconvolution) and K2[i] = k_i^2, k_i = 2 pi i'/L with i' the signed
-// frequency; norm = -4 pi G / (V_cell G^3) (density = mass/V_cell, unnormalised inverse FFT).
+// Twiddles e^{-2 pi i m/G}; per-axis tables T[i] = exp(-k_i^2 r_s^2) / sinc(k_i h/2)^4 (the sinc^4
+// only with deconvolution), K2[i] = k_i^2, Ks[i] = k_i and D[i] = the gradient stencil's transfer
+// function (4-point: (8 sin kh - sin 2kh) / 6h; 2-point: sin kh / h), k_i = 2 pi i'/L with i' the
+// signed frequency; norm = -4 pi G / (V_cell G^3) (density = mass/V_cell, unnormalised inverse FFT).
 hipError_t pm_tables(Ctx& c) {
     const int G = (int)c.G();
     const double L = c.p.box_size, h = L / G;
-    std::vector<double> tw(2 * G), tab(2 * G);
+    std::vector<double> tw(2 * G), tab(4 * G);
     for (int m = 0; m < G; ++m) {
         const double a = -2.0 * M_PI * (double)m / (double)G;
         tw[2 * m] = std::cos(a);
         tw[2 * m + 1] = std::sin(a);
     }
     const double dk = 2.0 * M_PI / L;
+    const bool four = c.p.gradient == JTP_GRAD_4PT;
     for (int i = 0; i < G; ++i) {
         const int is = i <= G / 2 ? i : i - G;
         const double k = dk * is;
@@ -478,20 +594,20 @@ hipError_t pm_tables(Ctx& c) {
         }
         tab[i] = t;
         tab[G + i] = k * k;
+        tab[2 * G + i] = k;
+        tab[3 * G + i] = four ? (8.0 * std::sin(k * h) - std::sin(2.0 * k * h)) / (6.0 * h) : std::sin(k * h) / h;
     }
     c.green_norm = -4.0 * M_PI * c.p.g_constant / (h * h * h * (double)G * (double)G * (double)G);
     hipError_t e;
     if (c.p.precision == JTP_FP64) {
         e = hipMemcpy(c.twiddle, tw.data(), 2 * G * sizeof(double), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(c.gtab, tab.data(), 2 * G * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c.gtab, tab.data(), 4 * G * sizeof(double), hipMemcpyHostToDevice);
     } else {
-        std::vector<float> twf(2 * G), tabf(2 * G);
-        for (int m = 0; m < 2 * G; ++m) {
-            twf[m] = (float)tw[m];
-            tabf[m] = (float)tab[m];
-        }
+        std::vector<float> twf(2 * G), tabf(4 * G);
+        for (int m = 0; m < 2 * G; ++m) twf[m] = (float)tw[m];
+        for (int m = 0; m < 4 * G; ++m) tabf[m] = (float)tab[m];
         e = hipMemcpy(c.twiddle, twf.data(), 2 * G * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(c.gtab, tabf.data(), 2 * G * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c.gtab, tabf.data(), 4 * G * sizeof(float), hipMemcpyHostToDevice);
     }
     return e;
 }

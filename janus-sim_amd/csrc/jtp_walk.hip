@@ -42,6 +42,7 @@ struct WalkArgs {
     int64_t n;
     int32_t root;
     int32_t leaf;
+    double theta_cut2;  // theta_cut^2 for nodes whose box reaches beyond r_cut
     int64_t max_steps;
     double scale;   // length per fixed-point unit, L / 2^32
     double eps2;
@@ -60,12 +61,14 @@ template <typename R> struct EntryT;
 template <> struct EntryT<float> { using T = float4; };
 template <> struct EntryT<double> { using T = double4; };
 
-// Pair evaluation: returns the force factor f such that a += f * d.
-template <typename R, int MODE>
+// Pair evaluation: returns the force factor f such that a += f * d.  CUT applies the hard
+// cutoff r < r_cut (particle entries, and monopoles under the x^4 law whose weight turns
+// negative beyond r_cut); monopoles under the erfc laws keep the smooth tail of S.
+template <typename R, int MODE, bool CUT>
 struct Pair;
 
-template <int MODE>
-struct Pair<float, MODE> {
+template <int MODE, bool CUT>
+struct Pair<float, MODE, CUT> {
     __device__ static inline float f(float q, float r2, const WalkArgs& a, float eps2, float rcut2) {
         const float r2s = r2 + eps2;
         const float inv = __builtin_amdgcn_rsqf(r2s);
@@ -83,12 +86,12 @@ struct Pair<float, MODE> {
             S = __builtin_amdgcn_exp2f(r2s * a.escale) * P;
         }
         const float fv = q * S * inv3;
-        return r2 < rcut2 ? fv : 0.0f;
+        return (!CUT || r2 < rcut2) ? fv : 0.0f;
     }
 };
 
-template <int MODE>
-struct Pair<double, MODE> {
+template <int MODE, bool CUT>
+struct Pair<double, MODE, CUT> {
     __device__ static inline double f(double q, double r2, const WalkArgs& a, double eps2, double rcut2) {
         const double r2s = r2 + eps2;
         const double inv = rsqrt(r2s);
@@ -103,7 +106,7 @@ struct Pair<double, MODE> {
             if (MODE == JTP_SPLIT_ERFC_GAUSS) S += 1.1283791670955126 * x * exp(-x * x);
         }
         const double fv = q * S * inv3;
-        return r2 < rcut2 ? fv : 0.0;
+        return (!CUT || r2 < rcut2) ? fv : 0.0;
     }
 };
 
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     int32_t* S = stack_s[w];
     const M* mono = (const M*)a.mono;
     const R scale = (R)a.scale;
-    const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2;
+    const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2, thetac2 = (R)a.theta_cut2;
     const R qpart[2] = {(R)a.qm, (R)a.qp};
     const int maxapp = a.leaf > 2 ? a.leaf : 2;
     const uint64_t lt = lanemask_lt();
@@ -177,18 +180,28 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 tz = fixed_rel<R>(rec.z, gc[2], scale);
         R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
 
-        int sp = 1, lc = 0;
+        int sp = 1, lp = 0, lm = 0;
         int64_t steps = 0;
         if (lane == 0) S[0] = a.root;
         wave_lds_fence();
 
-        auto flush = [&](int cnt) {
+        // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
+        auto flush = [&](int np_, int nm_) {
             R px = 0, py = 0, pz = 0;
-            for (int j = 0; j < cnt; ++j) {
+            for (int j = 0; j < np_; ++j) {
                 const E e = L[j];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
                 const R r2 = dx * dx + dy * dy + dz * dz;
-                const R f = Pair<R, MODE>::f(e.w, r2, a, eps2, rcut2);
+                const R f = Pair<R, MODE, true>::f(e.w, r2, a, eps2, rcut2);
+                px += f * dx;
+                py += f * dy;
+                pz += f * dz;
+            }
+            for (int j = kListCap - nm_; j < kListCap; ++j) {
+                const E e = L[j];
+                const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const R f = Pair<R, MODE, MODE == JTP_SPLIT_POLY4>::f(e.w, r2, a, eps2, rcut2);
                 px += f * dx;
                 py += f * dy;
                 pz += f * dz;
@@ -204,7 +217,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 sy += py;
                 sz += pz;
             }
-            my_inter += (unsigned long long)cnt * (unsigned long long)nvalid;
+            my_inter += (unsigned long long)(np_ + nm_) * (unsigned long long)nvalid;
         };
 
         while (sp > 0) {
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 break;
             }
             int k = sp < 64 ? sp : 64;
-            const int kl = (kListCap - lc) / maxapp;
+            const int kl = (kListCap - lp - lm) / maxapp;
             k = k < kl ? k : kl;
             const int ks = kStackCap - sp > 1 ? kStackCap - sp : 1;
             k = k < ks ? k : ks;
@@ -222,7 +235,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const int32_t node = has ? S[sp + lane] : 0;
             wave_lds_fence();
 
-            int act = 0, napp = 0;
+            int act = 0, napp = 0, napm = 0;
             int32_t first = 0;
             if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
                 if (lane == 0) atomicExch(&a.counters[1], 3);  // malformed child index: skip it
@@ -243,7 +256,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     nmn[2] = nmx[2] = pr.z;
                     cnt = 1;
                 }
-                R gap2 = 0, ext = 0;
+                R gap2 = 0, far2 = 0, ext = 0;
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) {
                     const uint32_t e = nmx[ax] - nmn[ax];
@@ -255,13 +268,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     gap = gap > 0 ? gap : 0;
                     const R gr_ = (R)gap * scale;
                     gap2 += gr_ * gr_;
+                    const R fr_ = (R)(adc + (int64_t)gh[ax] + (int64_t)nh) * scale;
+                    far2 += fr_ * fr_;
                     const R er = (R)e * scale;
                     ext = er > ext ? er : ext;
                 }
                 if (gap2 < rcut2) {
-                    if (node >= 0 && ext * ext < theta2 * gap2) {
+                    const R th2 = far2 < rcut2 ? theta2 : thetac2;
+                    if (node >= 0 && ext * ext < th2 * gap2) {
                         act = 1;
-                        napp = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
+                        napm = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
                     } else if (cnt <= a.leaf) {
                         act = 2;
                         napp = cnt;
@@ -270,13 +286,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     }
                 }
             }
-            // appends
-            const int incl = wave_incl_scan(napp);
-            const int off = lc + incl - napp;
-            const int total = __shfl(incl, 63, 64);
+            // appends: one scan over (particles | monopoles << 16)
+            const int packed = napp | (napm << 16);
+            const int incl = wave_incl_scan(packed);
+            const int offp = lp + (incl & 0xffff) - napp;
+            const int offm = lm + (incl >> 16) - napm;
+            const int totl = __shfl(incl, 63, 64);
             if (act == 1) {
-                int o = off;
-                if (mono_q(mono, node, 0) != (R)0) L[o++] = mono_entry(mono, node, 0, gc, scale);
+                int o = kListCap - 1 - offm;
+                if (mono_q(mono, node, 0) != (R)0) L[o--] = mono_entry(mono, node, 0, gc, scale);
                 if (mono_q(mono, node, 1) != (R)0) L[o] = mono_entry(mono, node, 1, gc, scale);
             } else if (act == 2) {
                 for (int t = 0; t < napp; ++t) {
@@ -286,7 +304,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     e.y = fixed_rel<R>(pr.y, gc[1], scale);
                     e.z = fixed_rel<R>(pr.z, gc[2], scale);
                     e.w = qpart[(int32_t)pr.w > 0 ? 1 : 0];
-                    L[off + t] = e;
+                    L[offp + t] = e;
                 }
             }
             // pushes
@@ -301,15 +319,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 S[sp + 2 * rank + 1] = kd.x;
             }
             if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
-            lc += total;
+            lp += totl & 0xffff;
+            lm += totl >> 16;
             wave_lds_fence();
-            if (lc >= kFlushAt) {
-                flush(lc);
-                lc = 0;
+            if (lp + lm >= kFlushAt) {
+                flush(lp, lm);
+                lp = 0;
+                lm = 0;
                 wave_lds_fence();
             }
         }
-        if (lc > 0) flush(lc);
+        if (lp + lm > 0) flush(lp, lm);
         wave_lds_fence();
 
         if (valid) {
@@ -387,7 +407,9 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     a.n = n;
     a.root = n >= 2 ? 0 : ~0;
     a.leaf = c.p.leaf_size;
-    a.max_steps = 4 * n + 1024;  // each step pops >= 1 of the <= 2n-1 nodes
+    a.max_steps = 4 * n + 1024;
+    const double tc = c.p.theta_cut < c.p.theta ? c.p.theta_cut : c.p.theta;
+    a.theta_cut2 = tc * tc;  // each step pops >= 1 of the <= 2n-1 nodes
     a.scale = c.p.box_size / 4294967296.0;
     a.eps2 = c.p.softening * c.p.softening;
     a.rcut2 = c.p.r_cut * c.p.r_cut;

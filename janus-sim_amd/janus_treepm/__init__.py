@@ -30,6 +30,7 @@ SPLIT_ERFC_GAUSS, SPLIT_ERFC, SPLIT_POLY4 = 0, 1, 2
 GRAD_4PT, GRAD_2PT = 0, 1
 STAGE_KEYS, STAGE_PERM, STAGE_NODES, STAGE_MONO, STAGE_RHO, STAGE_PHI, STAGE_ACC_PM, STAGE_SUMS, STAGE_BOXES = (
     1, 2, 3, 4, 5, 6, 7, 8, 9)
+STAGE_RHO2, STAGE_PHI2 = 10, 11
 
 # every symbol declared in include/janus_treepm.h
 EXPORTED = [
@@ -53,6 +54,7 @@ class Params(C.Structure):
         ("g_constant", C.c_double), ("leaf_size", C.c_int32), ("group_size", C.c_int32),
         ("precision", C.c_int32), ("split_mode", C.c_int32), ("gradient", C.c_int32),
         ("deconvolve", C.c_int32), ("pm_enabled", C.c_int32), ("tree_enabled", C.c_int32),
+        ("interlace", C.c_int32), ("theta_cut", C.c_double),
     ]
 
 
@@ -225,6 +227,7 @@ class ForceContext:
             STAGE_KEYS: (np.uint64, (-1,)), STAGE_PERM: (np.int32, (-1,)), STAGE_NODES: (np.int32, (-1, 4)),
             STAGE_MONO: (np.float64, (-1, 8)), STAGE_RHO: (rt, (-1,)), STAGE_PHI: (rt, (-1,)),
             STAGE_ACC_PM: (rt, (-1, 3)), STAGE_SUMS: (np.uint64, (-1, 8)), STAGE_BOXES: (np.uint32, (-1, 6)),
+            STAGE_RHO2: (rt, (-1,)), STAGE_PHI2: (rt, (-1,)),
         }[stage]
         return buf.view(view[0]).reshape(view[1])
 
@@ -282,7 +285,8 @@ class TreePMForce:
     """
 
     def __init__(self, r_cut, grid_size, box_size, theta, softening, r_s=None, precision=FP64,
-                 split_mode=SPLIT_ERFC_GAUSS, gradient=GRAD_4PT, deconvolve=1, g_constant=1.0, pm_only=False):
+                 split_mode=SPLIT_ERFC_GAUSS, gradient=GRAD_4PT, deconvolve=1, g_constant=1.0, pm_only=False,
+                 interlace=1):
         p = default_params(box_size, grid_size)
         p.r_cut = float(r_cut)
         p.r_s = float(r_cut / 3.0 if r_s is None else r_s)
@@ -293,6 +297,7 @@ class TreePMForce:
         p.gradient = gradient
         p.deconvolve = deconvolve
         p.g_constant = g_constant
+        p.interlace = interlace
         p.tree_enabled = 0 if pm_only else 1
         self.params = p
         self.ctx = ForceContext(p)

@@ -237,8 +237,11 @@ void orc_fft3d(int G, double* re, double* im, int inverse) {
 }
 
 // phi = IFFT[ -4 pi g / k^2 * exp(-k^2 r_s^2) / W(k)^2 * FFT[rho / V_cell] ], W = prod sinc^2(k_i h/2)
-// (W only when deconv != 0); DC -> 0; max|k_i| < k_min -> 0.
-void orc_poisson(int G, double L, double g, double r_s, int k_min, int deconv, const double* rho, double* phi) {
+// (W only when deconv != 0); DC -> 0; max|k_i| < k_min -> 0.  match >= 0 additionally multiplies by
+// (k . D) / max(|D|^2, |k|^2 / 4) with D the transfer function of the gradient stencil (0: 4-point,
+// 1: 2-point); the floor keeps the factor bounded where D vanishes (the Nyquist planes).
+void orc_poisson(int G, double L, double g, double r_s, int k_min, int deconv, int match, const double* rho,
+                 double* phi) {
     const int64_t G3 = (int64_t)G * G * G;
     const double h = L / G, vcell = h * h * h;
     std::vector<cd> d(G3);
@@ -267,6 +270,18 @@ void orc_poisson(int G, double L, double g, double r_s, int k_min, int deconv, c
                         }
                     }
                     gr /= w * w;
+                }
+                if (match >= 0) {
+                    double kd = 0, dd = 0, kk2 = 0;
+                    for (double kc : {ki, kj, kk}) {
+                        const double kv = kc * dk;
+                        const double D = match == 0 ? (8.0 * std::sin(kv * h) - std::sin(2.0 * kv * h)) / (6.0 * h)
+                                                    : std::sin(kv * h) / h;
+                        kd += kv * D;
+                        dd += D * D;
+                        kk2 += kv * kv;
+                    }
+                    gr *= kd / std::max(dd, 0.25 * kk2);
                 }
                 d[idx] *= gr;
             }

@@ -30,7 +30,7 @@ class Oracle:
             "orc_node_sums": (None, [i64, vp, vp, vp, vp, vp]),
             "orc_deposit": (None, [i64, vp, vp, dbl, dbl, i32, vp]),
             "orc_fft3d": (None, [i32, vp, vp, i32]),
-            "orc_poisson": (None, [i32, dbl, dbl, dbl, i32, i32, vp, vp]),
+            "orc_poisson": (None, [i32, dbl, dbl, dbl, i32, i32, i32, vp, vp]),
             "orc_interp": (None, [i64, vp, vp, i32, dbl, vp, i32, vp]),
             "orc_short_direct": (None, [i64, vp, vp, dbl, dbl, dbl, dbl, dbl, dbl, i32, i64, vp, vp]),
             "orc_ewald": (None, [i64, vp, vp, dbl, dbl, dbl, dbl, dbl, dbl, i32, i64, vp, vp]),
@@ -110,10 +110,11 @@ class Oracle:
         self.lib.orc_fft3d(G, re.ctypes.data, im.ctypes.data, 1 if inverse else 0)
         return re + 1j * im
 
-    def poisson(self, rho, G, L, g, r_s, k_min=0, deconv=0):
+    def poisson(self, rho, G, L, g, r_s, k_min=0, deconv=0, match=-1):
+        """match: -1 none, 0 / 1 = Green function matched to the 4- / 2-point gradient stencil."""
         r = np.ascontiguousarray(rho, np.float64).reshape(-1)
         phi = np.empty_like(r)
-        self.lib.orc_poisson(G, L, g, r_s, k_min, deconv, r.ctypes.data, phi.ctypes.data)
+        self.lib.orc_poisson(G, L, g, r_s, k_min, deconv, match, r.ctypes.data, phi.ctypes.data)
         return phi
 
     def interp(self, pos_fixed, sp, G, L, phi, four=True):

@@ -86,7 +86,7 @@ def test_compat_vs_reference_cpu(jtp, orc):
     r_cut, eps = 8.0, 0.3
     p = jtp.default_params(L, G)
     p.r_cut, p.r_s, p.theta, p.softening = r_cut, r_cut / 3, 0.0, eps
-    p.precision, p.split_mode, p.gradient, p.deconvolve = 1, 2, 1, 0
+    p.precision, p.split_mode, p.gradient, p.deconvolve, p.interlace = 1, 2, 1, 0, 0
     ctx = jtp.ForceContext(p, 1.0, 1.0)
     acc = ctx.compute_forces(pos, sp)
     h = L / G

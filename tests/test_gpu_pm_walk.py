@@ -26,23 +26,33 @@ def _rel(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(den, 1e-300)
 
 
-@pytest.mark.parametrize("precision", [1, 0])
-def test_deposit_solve_interp(jtp, orc, precision):
-    print(f"case: PM chain precision={precision}", flush=True)
+def _half_cell(pos, G):
+    return (pos.astype(np.uint64) + np.uint64(1 << (31 - int(np.log2(G))))).astype(np.uint32)
+
+
+@pytest.mark.parametrize("precision,interlace", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_deposit_solve_interp(jtp, orc, precision, interlace):
+    """Both grids of the interlaced PM: the shifted grid is the oracle's deposit of positions + h/2."""
+    print(f"case: PM chain precision={precision} interlace={interlace}", flush=True)
     w = synth.lattice_zeldovich(n_side=20, n_cells=32, seed=9)
     pos = jtp.to_fixed(w.pos, w.box_size)
-    ctx = _ctx(jtp, w, precision=precision, tree_enabled=0)
+    ctx = _ctx(jtp, w, precision=precision, tree_enabled=0, interlace=interlace)
     acc = ctx.compute_forces(pos, w.species)
-    rho = ctx.stage(jtp.STAGE_RHO).astype(np.float64)
-    rho_ref = orc.deposit(pos, w.species, w.m_plus, w.m_minus_abs, w.n_cells)
-    tol = 1e-12 if precision == 1 else 2e-6
-    scale = np.max(np.abs(rho_ref))
-    assert np.max(np.abs(rho - rho_ref)) <= tol * scale * 20
-    phi = ctx.stage(jtp.STAGE_PHI).astype(np.float64)
-    phi_ref = orc.poisson(rho_ref, w.n_cells, w.box_size, 1.0, w.r_s, 0, 1)
-    tol = 1e-10 if precision == 1 else 2e-5
-    assert np.max(np.abs(phi - phi_ref)) <= tol * np.max(np.abs(phi_ref))
-    acc_ref = orc.interp(pos, w.species, w.n_cells, w.box_size, phi_ref, four=True)
+    G = w.n_cells
+    grids = [(pos, jtp.STAGE_RHO, jtp.STAGE_PHI)]
+    if interlace:
+        grids.append((_half_cell(pos, G), jtp.STAGE_RHO2, jtp.STAGE_PHI2))
+    acc_ref = np.zeros((w.n, 3))
+    for p_in, st_rho, st_phi in grids:
+        rho = ctx.stage(st_rho).astype(np.float64)
+        rho_ref = orc.deposit(p_in, w.species, w.m_plus, w.m_minus_abs, G)
+        tol = 1e-12 if precision == 1 else 2e-6
+        assert np.max(np.abs(rho - rho_ref)) <= tol * np.max(np.abs(rho_ref)) * 20
+        phi = ctx.stage(st_phi).astype(np.float64)
+        phi_ref = orc.poisson(rho_ref, G, w.box_size, 1.0, w.r_s, 0, 1, match=0)
+        tol = 1e-10 if precision == 1 else 2e-5
+        assert np.max(np.abs(phi - phi_ref)) <= tol * np.max(np.abs(phi_ref))
+        acc_ref += orc.interp(p_in, w.species, G, w.box_size, phi_ref, four=True) / len(grids)
     tol = 1e-9 if precision == 1 else 1e-4
     assert np.max(np.abs(acc - acc_ref)) <= tol * np.max(np.abs(acc_ref))
     assert np.array_equal(ctx.stage(jtp.STAGE_ACC_PM), acc)
@@ -92,8 +102,10 @@ def test_pm_solve_numpy(jtp, precision):
     h = L / G
     sinc = lambda x: np.sinc(x / np.pi)
     W = sinc(KX * h / 2) ** 2 * sinc(KY * h / 2) ** 2 * sinc(KZ * h / 2) ** 2
+    D = [(8 * np.sin(K * h) - np.sin(2 * K * h)) / (6 * h) for K in (KX, KY, KZ)]
     with np.errstate(divide="ignore", invalid="ignore"):
         green = -4 * np.pi / k2 * np.exp(-k2 * rs ** 2) / W ** 2 / h ** 3
+        green *= (KX * D[0] + KY * D[1] + KZ * D[2]) / np.maximum(D[0] ** 2 + D[1] ** 2 + D[2] ** 2, 0.25 * k2)
     green[0, 0, 0] = 0
     ref = np.real(np.fft.ifftn(np.fft.fftn(rho) * green))
     tol = 1e-11 if precision == 1 else 3e-5
@@ -152,3 +164,6 @@ def test_deposit_dense_cells(jtp, orc, precision):
     assert counts.max() > 1000  # the dense path is exercised
     tol = 1e-12 if precision == 1 else 3e-6
     assert np.max(np.abs(rho - rho_ref)) <= tol * np.max(np.abs(rho_ref))
+    rho2 = ctx.stage(jtp.STAGE_RHO2).astype(np.float64)
+    rho2_ref = orc.deposit(_half_cell(pos, w.n_cells), w.species, w.m_plus, w.m_minus_abs, w.n_cells)
+    assert np.max(np.abs(rho2 - rho2_ref)) <= tol * np.max(np.abs(rho2_ref))

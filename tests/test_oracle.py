@@ -120,6 +120,22 @@ def test_poisson_golden(orc):
     assert np.max(np.abs(phi - ref)) < 1e-12 * np.max(np.abs(ref))
 
 
+def test_poisson_matched_stencil(orc):
+    # a single mode along x: the matched factor is k / D(k), so the stencil gradient of phi is exact
+    G, L = 16, 10.0
+    h = L / G
+    x = np.arange(G) * h
+    m = 3
+    rho = np.broadcast_to(np.cos(2 * np.pi * m * x / L), (G, G, G)).copy()
+    k = 2 * np.pi * m / L
+    D4 = (8 * np.sin(k * h) - np.sin(2 * k * h)) / (6 * h)
+    phi = orc.poisson(rho, G, L, 1.0, 0.0, 0, 0, match=0).reshape(G, G, G)
+    assert np.allclose(phi, -4 * np.pi / k ** 2 * (k / D4) * rho / h ** 3, atol=1e-11)
+    D2 = np.sin(k * h) / h
+    phi = orc.poisson(rho, G, L, 1.0, 0.0, 0, 0, match=1).reshape(G, G, G)
+    assert np.allclose(phi, -4 * np.pi / k ** 2 * (k / D2) * rho / h ** 3, atol=1e-11)
+
+
 def test_poisson_single_mode_kmin(orc):
     # rho = cos(2 pi m x / L): phi = green(k) rho exactly; k_min above m removes it
     G, L = 16, 10.0
