@@ -85,6 +85,7 @@ typedef struct jtp_stats {
     double  ms_interp;    /* gradient + CIC interpolation                      */
     double  ms_walk;      /* group walk (one kernel launch)                    */
     uint64_t interactions;/* pair + monopole interactions evaluated by the walk */
+    uint64_t interactions_mono; /* of which monopole interactions                */
     int64_t n_groups;
     int64_t n_particles;
 } jtp_stats;

@@ -51,44 +51,53 @@ static hipError_t dalloc_bytes(void*& p, size_t bytes) {
     return hipMalloc(&p, bytes);
 }
 
-// Chebyshev interpolant of P on [0, X] (degree kPolyDeg), converted to a
-// polynomial in t = 2x/X - 1.  P(x) = erfcx(x) [+ 2x/sqrt(pi)], so that
-// S(x) = exp(-x^2) P(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)].
-void make_split_poly(Ctx& c) {
-    const int N = kPolyDeg + 1;
-    const double rho_max = std::sqrt(c.p.r_cut * c.p.r_cut + c.p.softening * c.p.softening);
-    const double X = (c.p.r_s > 0 ? rho_max / (2.0 * c.p.r_s) : 1.0) * 1.0001 + 1e-12;
-    const bool gauss = c.p.split_mode == JTP_SPLIT_ERFC_GAUSS;
-    std::vector<double> f(N), cheb(N, 0.0);
+// Chebyshev interpolants (degree kPolyDeg) of the split factor for the FP32 walk, converted to
+// polynomials in t = 2x/X - 1 for Horner evaluation:
+//   particles: S(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)] on [0, Xp], Xp covering rho <= sqrt(r_cut^2 + eps^2)
+//              (absolute error < 1e-5 for Xp up to ~2.5, i.e. r_cut = 4.5 r_s);
+//   monopoles: P(x) = erfcx(x) [+ 2x/sqrt(pi)] on [0, Xm], Xm = Xp (1 + sqrt(3) theta), used as
+//              exp(-x^2) P(x) (absolute error ~2e-5 of the pair force at theta = 0.5).
+static void cheb_fit(double X, int deg, float* out, double (*f)(double, bool), bool gauss) {
+    const int N = deg + 1;
+    std::vector<double> fv(N), cheb(N, 0.0);
     for (int k = 0; k < N; ++k) {
         const double t = std::cos(M_PI * (k + 0.5) / N);
-        const double x = 0.5 * X * (t + 1.0);
-        double v = std::erfc(x) * std::exp(x * x);
-        if (gauss) v += 2.0 * x / std::sqrt(M_PI);
-        f[k] = v;
+        fv[k] = f(0.5 * X * (t + 1.0), gauss);
     }
     for (int j = 0; j < N; ++j) {
         double s = 0;
-        for (int k = 0; k < N; ++k) s += f[k] * std::cos(M_PI * j * (k + 0.5) / N);
+        for (int k = 0; k < N; ++k) s += fv[k] * std::cos(M_PI * j * (k + 0.5) / N);
         cheb[j] = 2.0 * s / N;
     }
     cheb[0] *= 0.5;
-    // T_j(t) as monomials: T_0 = 1, T_1 = t, T_{j+1} = 2t T_j - T_{j-1}
-    std::vector<std::vector<double>> T(N, std::vector<double>(N, 0.0));
+    std::vector<std::vector<double>> T(N, std::vector<double>(N, 0.0));  // T_j(t) as monomials
     T[0][0] = 1.0;
     if (N > 1) T[1][1] = 1.0;
     for (int j = 1; j + 1 < N; ++j)
-        for (int d = 0; d < N; ++d) {
-            double v = -T[j - 1][d];
-            if (d > 0) v += 2.0 * T[j][d - 1];
-            T[j + 1][d] = v;
-        }
+        for (int d = 0; d < N; ++d) T[j + 1][d] = -T[j - 1][d] + (d > 0 ? 2.0 * T[j][d - 1] : 0.0);
     for (int d = 0; d < N; ++d) {
         double s = 0;
         for (int j = 0; j < N; ++j) s += cheb[j] * T[j][d];
-        c.poly.c[d] = (float)s;
+        out[d] = (float)s;
     }
-    c.poly_tscale = (float)(c.p.r_s > 0 ? 1.0 / (c.p.r_s * X) : 0.0);
+}
+static double split_direct(double x, bool gauss) {
+    return std::erfc(x) + (gauss ? 2.0 * x / std::sqrt(M_PI) * std::exp(-x * x) : 0.0);
+}
+static double split_scaled(double x, bool gauss) {
+    return std::erfc(x) * std::exp(x * x) + (gauss ? 2.0 * x / std::sqrt(M_PI) : 0.0);
+}
+
+void make_split_poly(Ctx& c) {
+    if (!(c.p.r_s > 0)) return;
+    const double rho_max = std::sqrt(c.p.r_cut * c.p.r_cut + c.p.softening * c.p.softening);
+    const double Xp = rho_max / (2.0 * c.p.r_s) * 1.0001 + 1e-12;
+    const double Xm = Xp * (1.0 + std::sqrt(3.0) * c.p.theta);
+    const bool gauss = c.p.split_mode == JTP_SPLIT_ERFC_GAUSS;
+    cheb_fit(Xp, kPolyDeg, c.poly.cp, split_direct, gauss);
+    cheb_fit(Xm, kPolyDeg, c.poly.cm, split_scaled, gauss);
+    c.poly_tscale = (float)(1.0 / (c.p.r_s * Xp));
+    c.poly_tscale_m = (float)(1.0 / (c.p.r_s * Xm));
 }
 
 static void free_particle_buffers(Ctx& c) {
@@ -241,7 +250,7 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)4 * G * rs);
     if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
-    if (e == hipSuccess) e = dalloc(c.dev_counters, 8 + 130 + 8);
+    if (e == hipSuccess) e = dalloc(c.dev_counters, kCtrTotal);
     if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_pinned, 4 * sizeof(unsigned long long), 0);
@@ -326,9 +335,10 @@ int jtp_compute_forces(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_
     JTP_HIP(&c, hipEventRecord(c.ev[6], s));
     // status: interaction count, group count, walk error flag
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned, c.dev_inter, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 3, c.dev_inter + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 1, c.dev_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     if (c.p.tree_enabled && n >= 2)
-        JTP_HIP(&c, hipMemcpyAsync((int32_t*)(c.host_pinned + 2), c.dev_counters + 8 + 129, sizeof(int32_t),
+        JTP_HIP(&c, hipMemcpyAsync((int32_t*)(c.host_pinned + 2), c.dev_counters + kCtrRoot, sizeof(int32_t),
                                    hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipStreamSynchronize(s));
     c.have_result = true;
@@ -391,6 +401,7 @@ int jtp_get_stats(jtp_ctx* h, jtp_stats* out) {
     JTP_HIP(&c, hipEventElapsedTime(&tot, c.ev[0], c.ev[6]));
     out->ms_total = tot;
     out->interactions = c.host_pinned[0];
+    out->interactions_mono = c.host_pinned[3];
     out->n_groups = ((const int32_t*)(c.host_pinned + 1))[0];
     return JTP_OK;
 }
@@ -533,7 +544,11 @@ void jtp_free(void* p) {
 }
 int jtp_copy_h2d(void* d, const void* h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice) == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
 int jtp_copy_d2h(void* h, const void* d, size_t b) { return hipMemcpy(h, d, b, hipMemcpyDeviceToHost) == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
-int jtp_copy_d2d(void* d, const void* s, size_t b) { return hipMemcpy(d, s, b, hipMemcpyDeviceToDevice) == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
+int jtp_copy_d2d(void* d, const void* s, size_t b) {
+    if (b && (!d || !s)) return JTP_ERR_ARG;
+    if (device_copy(d, s, b, nullptr) != hipSuccess) return JTP_ERR_HIP;
+    return hipStreamSynchronize(nullptr) == hipSuccess ? JTP_OK : JTP_ERR_HIP;
+}
 int jtp_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
 int jtp_device_count(int* out) {
     if (!out) return JTP_ERR_ARG;

@@ -41,6 +41,38 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
+// Appends the lanes with `pred` to a global list using one atomic per wavefront; returns the
+// lane's slot (or -1).  Every lane of the wavefront must call it.
+__device__ inline int wave_append(bool pred, int32_t* counter) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, (int)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return pred ? base + (int)__popcll(m & lanemask_lt()) : -1;
+}
+
+// Appends the threads with `pred` to a global list using one atomic per 256-thread block (the
+// throughput of same-address atomics, not their count per wavefront, limits frontier appends).
+// Every thread of the block must call it; `lds` holds 6 int32 of scratch.
+__device__ inline int block_append(bool pred, int32_t* counter, int32_t* lds) {
+    const int w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(pred);
+    if (lane_id() == 0) lds[w] = (int32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int32_t tot = lds[0] + lds[1] + lds[2] + lds[3];
+        lds[4] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    int off = lds[4];
+    for (int i = 0; i < w; ++i) off += lds[i];
+    const int r = pred ? off + (int)__popcll(m & lanemask_lt()) : -1;
+    __syncthreads();
+    return r;
+}
+
 // Exclusive prefix sum over a 256-thread block, one value per thread.
 // `lds` must hold 4 uint32 scratch entries; `total` receives the block sum.
 __device__ inline uint32_t block256_excl_scan(uint32_t v, uint32_t* lds, uint32_t* total) {

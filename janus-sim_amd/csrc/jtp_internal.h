@@ -18,9 +18,17 @@ constexpr int kBlock = 256;          // threads per workgroup for streaming kern
 constexpr int kSortIPT = 8;          // keys per thread per radix-sort tile
 constexpr int kSortTile = kBlock * kSortIPT;
 constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Morton cell (L/16)
-constexpr int kPolyDeg = 12;         // FP32 split-function polynomial degree
+constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
 constexpr int kFftPoints = 4096;     // complex points held in LDS per FFT workgroup
 constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by one wavefront
+constexpr int kMaxRounds = 128;      // bottom-up levels (tree height bound, checked)
+constexpr int kXcds = 8;             // MI355X accelerator complex dies; workgroup i runs on XCD i % 8
+// device counter block (int32): offsets
+constexpr int kCtrNGroups = 0, kCtrErr = 1, kCtrDense = 3;
+constexpr int kCtrRounds = 8;                       // kMaxRounds + 2 entries (round sizes, root flag)
+constexpr int kCtrRoot = kCtrRounds + kMaxRounds + 1;
+constexpr int kCtrXcd = kCtrRounds + kMaxRounds + 4;  // kXcds walk-queue heads
+constexpr int kCtrTotal = kCtrXcd + kXcds + 4;
 
 // ---------------------------------------------------------------- small types
 template <typename R> struct Cplx { R x, y; };
@@ -48,11 +56,15 @@ __host__ __device__ inline uint64_t morton_fixed(uint32_t ux, uint32_t uy, uint3
     return spread3_21(ux >> 11) | (spread3_21(uy >> 11) << 1) | (spread3_21(uz >> 11) << 2);
 }
 
-// Split-function coefficients for the FP32 walk.  The split factor is
-// S(x) = exp(-x^2) * P(t), t = x * tscale - 1, x = rho / (2 r_s), with P a
-// degree-kPolyDeg polynomial (Horner in t) interpolating erfcx(x) [+ 2x/sqrt(pi)].
+// Split-function coefficients for the FP32 walk, x = rho / (2 r_s):
+//  * particle pairs (always cut at r_cut, so x <= Xp): S(x) ~= Pp(t), t = 2x/Xp - 1, Pp the
+//    degree-kPolyDeg Chebyshev interpolant of S(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)];
+//  * monopoles (no cut under the erfc laws, x up to Xm ~ r_cut (1 + sqrt(3) theta) / 2 r_s):
+//    S(x) ~= exp(-x^2) Pm(t), t = min(2x/Xm - 1, 1), Pm interpolating erfcx(x) [+ 2x/sqrt(pi)],
+//    which stays bounded where x runs past the fitted range.
 struct SplitPoly {
-    float c[kPolyDeg + 1];
+    float cp[kPolyDeg + 1];
+    float cm[kPolyDeg + 1];
 };
 
 // ---------------------------------------------------------------- context
@@ -82,8 +94,7 @@ struct Ctx {
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
     int2 *groups = nullptr;
     int32_t *cell4 = nullptr;        // level-4 cell offsets (4097)
-    int32_t *dev_counters = nullptr; // [0] n_groups, [1] error flag, [2] walk queue head, [3] dense cells,
-                                     // [8 ..] bottom-up round sizes, [8 + 129] root-complete flag
+    int32_t *dev_counters = nullptr; // kCtr* layout
     unsigned long long *dev_inter = nullptr;  // interaction counter
     // PM
     int32_t *cell_start = nullptr;   // G^3 + 1
@@ -97,7 +108,8 @@ struct Ctx {
     void *gtab = nullptr;            // real 2G: per-axis window/Gaussian factor, k_i^2
     void *pm_acc = nullptr;          // real 3n (sorted order)
     SplitPoly poly{};
-    float poly_tscale = 0.0f;
+    float poly_tscale = 0.0f;    // particles: t = rho * poly_tscale - 1
+    float poly_tscale_m = 0.0f;  // monopoles
     double green_norm = 0.0;
     unsigned long long* host_pinned = nullptr;  // [0] interactions, [1] (n_groups | err << 32)
 
@@ -122,6 +134,7 @@ hipError_t launch_morton_keys(const uint32_t* pos, uint64_t* keys, int32_t* vals
 hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, float box_half,
                                   float inv_cell, hipStream_t s);
 hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s);   // keys0/vals0 -> keys0/vals0 (sorted)
+hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_gather_parts(const uint32_t* pos, const int8_t* species, const int32_t* perm,
                                uint4* parts, int64_t n, hipStream_t s);
 

@@ -47,9 +47,9 @@ __device__ inline void cic_corners(const uint4 r, int logG, R q, R w[8]) {
 __global__ void k_dense_mark(const int32_t* __restrict__ cs, int64_t ncells, int32_t* __restrict__ slot,
                              int32_t* __restrict__ list, int32_t* __restrict__ count) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    if (cs[c + 1] - cs[c] > kDenseCell) {
-        const int32_t k = atomicAdd(count, 1);
+    const bool dense = c < ncells && cs[c + 1] - cs[c] > kDenseCell;
+    const int k = wave_append(dense, count);
+    if (k >= 0) {
         slot[c] = k;
         list[k] = (int32_t)c;
     }
@@ -439,7 +439,7 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     const int64_t G3 = c.G3();
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * c.logG, G3, c.cell_start, s);
     if (e != hipSuccess) return e;
-    int32_t* count = c.dev_counters + 3;
+    int32_t* count = c.dev_counters + kCtrDense;
     int32_t* list = c.frontier0;  // free after the tree build; holds <= n / kDenseCell cells
     e = launch_fill_i32(count, 1, 0, s);
     if (e != hipSuccess) return e;

@@ -262,6 +262,25 @@ hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ device copy
+// Streaming copy (16-byte words when both pointers and the size allow it).  Also the
+// reference copy kernel for calibrating HBM byte counters.
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const bool aligned = (((uintptr_t)dst | (uintptr_t)src | bytes) & 15) == 0;
+    if (!aligned) return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+    const int64_t n16 = (int64_t)(bytes / 16);
+    int64_t blocks = (n16 + kBlock - 1) / kBlock;
+    if (blocks > 65536) blocks = 65536;
+    k_copy16<<<dim3((unsigned)blocks), dim3(kBlock), 0, s>>>((const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ particle gather
 // parts[i] = (x, y, z, species) of the i-th particle in key order (reorder_by_idx_*,
 // nbody_gpu_twopass.rs:1490-1514, fused into one 16-byte record).

@@ -117,51 +117,61 @@ __device__ inline void load_child(int32_t c, const uint4* parts, uint4* geo, Nod
 // what the reference does, nbody_gpu_twopass.rs:485-560).  Each node combines its
 // children in the fixed order (left, right), so the result does not depend on
 // which round or thread produced it.
-constexpr int kMaxRounds = 128;
-
-__global__ void k_bu_init(const int2* __restrict__ kids, int64_t nI, int32_t* __restrict__ cnt,
-                          int32_t* __restrict__ f0, int32_t* __restrict__ ctrl) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nI) return;
-    const int2 k = kids[i];
-    const int leaves = (k.x < 0 ? 1 : 0) + (k.y < 0 ? 1 : 0);
-    cnt[i] = leaves;
-    if (leaves == 2) f0[atomicAdd(&ctrl[0], 1)] = (int32_t)i;
+__global__ __launch_bounds__(256) void k_bu_init(const int2* __restrict__ kids, int64_t nI, int32_t* __restrict__ cnt,
+                                                 int32_t* __restrict__ f0, int32_t* __restrict__ ctrl) {
+    __shared__ int32_t lds[6];
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nI; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        int leaves = 0;
+        if (i < nI) {
+            const int2 k = kids[i];
+            leaves = (k.x < 0 ? 1 : 0) + (k.y < 0 ? 1 : 0);
+            cnt[i] = leaves;
+        }
+        const int slot = block_append(leaves == 2, &ctrl[0], lds);
+        if (slot >= 0) f0[slot] = (int32_t)i;
+    }
 }
 
-__global__ void k_bu_round(int r, const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
+__global__ __launch_bounds__(256) void k_bu_round(int r, const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
                            const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
                            int32_t* __restrict__ cnt, NodeSum* __restrict__ sums, const int32_t* __restrict__ fin,
                            int32_t* __restrict__ fout, int32_t* __restrict__ ctrl) {
+    __shared__ int32_t lds[6];
     const int32_t size = ctrl[r];
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < size;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t p = fin[idx];
-        if (p < 0 || p >= nI) continue;
-        const int2 k = kids[p];
-        ChildData a, b;
-        load_child(k.x, parts, geo, sums, a);
-        load_child(k.y, parts, geo, sums, b);
-        NodeSum s;
+    // uniform trip count per block so that the frontier append can use block_append
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < size; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t idx = base + threadIdx.x;
+        int32_t par = -1;
+        bool ready = false;
+        const int32_t p = idx < size ? fin[idx] : -1;
+        if (p >= 0 && p < nI) {
+            const int2 k = kids[p];
+            ChildData a, b;
+            load_child(k.x, parts, geo, sums, a);
+            load_child(k.y, parts, geo, sums, b);
+            NodeSum s;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            s.sp[ax] = a.sp[ax] + b.sp[ax];
-            s.sm[ax] = a.sm[ax] + b.sm[ax];
+            for (int ax = 0; ax < 3; ++ax) {
+                s.sp[ax] = a.sp[ax] + b.sp[ax];
+                s.sm[ax] = a.sm[ax] + b.sm[ax];
+            }
+            s.np = a.np + b.np;
+            s.nm = a.nm + b.nm;
+            s.pad[0] = s.pad[1] = 0;
+            sums[p] = s;
+            const uint4 g0 = geo[2 * p], g1 = geo[2 * p + 1];
+            geo[2 * p] = make_uint4(min(a.mn[0], b.mn[0]), min(a.mn[1], b.mn[1]), min(a.mn[2], b.mn[2]), g0.w);
+            geo[2 * p + 1] = make_uint4(max(a.mx[0], b.mx[0]), max(a.mx[1], b.mx[1]), max(a.mx[2], b.mx[2]), g1.w);
+            if (p == 0) {
+                ctrl[kMaxRounds + 1] = 1;  // root complete
+            } else {
+                par = parent_int[p];
+                if (par >= 0 && par < nI) ready = atomicAdd(&cnt[par], 1) == 1;
+            }
         }
-        s.np = a.np + b.np;
-        s.nm = a.nm + b.nm;
-        s.pad[0] = s.pad[1] = 0;
-        sums[p] = s;
-        const uint4 g0 = geo[2 * p], g1 = geo[2 * p + 1];
-        geo[2 * p] = make_uint4(min(a.mn[0], b.mn[0]), min(a.mn[1], b.mn[1]), min(a.mn[2], b.mn[2]), g0.w);
-        geo[2 * p + 1] = make_uint4(max(a.mx[0], b.mx[0]), max(a.mx[1], b.mx[1]), max(a.mx[2], b.mx[2]), g1.w);
-        if (p == 0) {
-            ctrl[kMaxRounds + 1] = 1;  // root complete
-            continue;
-        }
-        const int32_t par = parent_int[p];
-        if (par < 0 || par >= nI) continue;
-        if (atomicAdd(&cnt[par], 1) == 1) fout[atomicAdd(&ctrl[r + 1], 1)] = par;
+        const int slot = block_append(ready, &ctrl[r + 1], lds);
+        if (slot >= 0) fout[slot] = par;
     }
 }
 
@@ -284,10 +294,10 @@ __global__ __launch_bounds__(256) void k_group_offsets(const int32_t* __restrict
     }
     if (tid == 0) {
         goff[kNCell4] = (int32_t)tot;
-        counters[0] = (int32_t)tot;
-        counters[1] = 0;   // walk error flag
-        counters[2] = 0;   // walk queue head
+        counters[kCtrNGroups] = (int32_t)tot;
+        counters[kCtrErr] = 0;
     }
+    if (tid < kXcds) counters[kCtrXcd + tid] = 0;  // walk queue heads
 }
 
 __global__ void k_groups(const int32_t* __restrict__ cell4, const int32_t* __restrict__ goff, int gs,
@@ -316,11 +326,12 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
         launch_fill_i32(c.parent_leaf, n, -1, s);
         k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, c.node_geo, c.node_kids, c.parent_int,
                                                     c.parent_leaf, c.flags);
-        int32_t* ctrl = c.dev_counters + 8;
+        int32_t* ctrl = c.dev_counters + kCtrRounds;
         launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
-        k_bu_init<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.node_kids, nI, c.flags, c.frontier0, ctrl);
+        const unsigned nbInit = nbI < 4096 ? nbI : 4096;
+        k_bu_init<<<dim3(nbInit), dim3(kBlock), 0, s>>>(c.node_kids, nI, c.flags, c.frontier0, ctrl);
         int blocks = (int)((nI / 4 + kBlock - 1) / kBlock);
-        blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+        blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
         for (int r = 0; r < kMaxRounds; ++r) {
             int32_t* fin = (r & 1) ? c.frontier1 : c.frontier0;
             int32_t* fout = (r & 1) ? c.frontier0 : c.frontier1;

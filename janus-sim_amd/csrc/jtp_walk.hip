@@ -34,7 +34,7 @@ struct WalkArgs {
     const int2* kids;
     const void* mono;
     const int2* groups;
-    int32_t* counters;  // [0] n_groups, [1] error flag, [2] queue head
+    int32_t* counters;  // kCtr* layout: group count, error flag, per-XCD queue heads
     unsigned long long* inter;
     const void* pm_acc;  // sorted order, nullptr when the PM part is disabled
     const int32_t* perm;
@@ -51,8 +51,9 @@ struct WalkArgs {
     double inv2rs;  // 1 / (2 r_s)
     double qp, qm;  // signed particle charges sigma G |m| for species +1 / -1
     SplitPoly poly;
-    float tscale;   // t = rho * tscale - 1
-    float escale;   // exp(-x^2) = exp2(r2s * escale)
+    float tscale;    // particles: t = rho * tscale - 1
+    float tscale_m;  // monopoles
+    float escale;    // exp(-x^2) = exp2(r2s * escale)
 };
 
 __device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
@@ -77,12 +78,19 @@ struct Pair<float, MODE, CUT> {
         if (MODE == JTP_SPLIT_POLY4) {
             const float u = r2 * (1.0f / rcut2);
             S = 1.0f - u * u;
-        } else {
-            const float rp = r2s * inv;
-            const float t = rp * a.tscale - 1.0f;
-            float P = a.poly.c[kPolyDeg];
+        } else if (CUT) {
+            // particle pair: rho <= sqrt(r_cut^2 + eps^2), direct interpolant of S
+            const float t = r2s * inv * a.tscale - 1.0f;
+            float P = a.poly.cp[kPolyDeg];
 #pragma unroll
-            for (int d = kPolyDeg - 1; d >= 0; --d) P = P * t + a.poly.c[d];
+            for (int d = kPolyDeg - 1; d >= 0; --d) P = P * t + a.poly.cp[d];
+            S = P;
+        } else {
+            // monopole: exp(-x^2) times the interpolant of the scaled factor (bounded beyond its range)
+            const float t = fminf(r2s * inv * a.tscale_m - 1.0f, 1.0f);
+            float P = a.poly.cm[kPolyDeg];
+#pragma unroll
+            for (int d = kPolyDeg - 1; d >= 0; --d) P = P * t + a.poly.cm[d];
             S = __builtin_amdgcn_exp2f(r2s * a.escale) * P;
         }
         const float fv = q * S * inv3;
@@ -150,14 +158,31 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     const R qpart[2] = {(R)a.qm, (R)a.qp};
     const int maxapp = a.leaf > 2 ? a.leaf : 2;
     const uint64_t lt = lanemask_lt();
-    unsigned long long my_inter = 0;
-    const int32_t n_groups = a.counters[0];
+    unsigned long long my_inter = 0, my_mono = 0;
+    const int32_t n_groups = a.counters[kCtrNGroups];
+    // XCD-local queues: the groups (in Morton order) are cut into kXcds contiguous ranges and a
+    // workgroup first drains the range of the XCD it runs on (workgroup i -> XCD i % 8), so
+    // concurrent groups on one XCD share tree nodes in that XCD's L2; then it steals from the
+    // following ranges.  Each group's result does not depend on who computes it.
+    const int xcd = blockIdx.x % kXcds;
+    int steal = 0;
 
     while (true) {
-        int g = 0;
-        if (lane == 0) g = atomicAdd(&a.counters[2], 1);
-        g = __shfl(g, 0, 64);
-        if (g >= n_groups) break;
+        int g = -1;
+        while (steal < kXcds) {
+            const int q = (xcd + steal) % kXcds;
+            const int32_t start = (int32_t)((int64_t)n_groups * q / kXcds);
+            const int32_t end = (int32_t)((int64_t)n_groups * (q + 1) / kXcds);
+            int t = 0;
+            if (lane == 0) t = atomicAdd(&a.counters[kCtrXcd + q], 1);
+            t = __shfl(t, 0, 64);
+            if (start + t < end) {
+                g = start + t;
+                break;
+            }
+            ++steal;
+        }
+        if (g < 0) break;
         const int2 gr = a.groups[g];
         const bool valid = lane < gr.y;
         const int64_t ti = (int64_t)gr.x + (valid ? lane : 0);
@@ -186,8 +211,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         wave_lds_fence();
 
         // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
-        auto flush = [&](int np_, int nm_) {
+        auto flush = [&](int np_in, int nm_in) {
+            // the counts are wave-uniform: keep the loop control in scalar registers
+            const int np_ = __builtin_amdgcn_readfirstlane(np_in);
+            const int nm_ = __builtin_amdgcn_readfirstlane(nm_in);
             R px = 0, py = 0, pz = 0;
+#pragma unroll 4
             for (int j = 0; j < np_; ++j) {
                 const E e = L[j];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
@@ -197,6 +226,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 py += f * dy;
                 pz += f * dz;
             }
+#pragma unroll 4
             for (int j = kListCap - nm_; j < kListCap; ++j) {
                 const E e = L[j];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
@@ -218,11 +248,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 sz += pz;
             }
             my_inter += (unsigned long long)(np_ + nm_) * (unsigned long long)nvalid;
+            my_mono += (unsigned long long)nm_ * (unsigned long long)nvalid;
         };
 
         while (sp > 0) {
             if (++steps > a.max_steps) {  // a finite hierarchy never gets here; report instead of spinning
-                if (lane == 0) atomicExch(&a.counters[1], 2);
+                if (lane == 0) atomicExch(&a.counters[kCtrErr], 2);
                 break;
             }
             int k = sp < 64 ? sp : 64;
@@ -238,7 +269,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             int act = 0, napp = 0, napm = 0;
             int32_t first = 0;
             if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
-                if (lane == 0) atomicExch(&a.counters[1], 3);  // malformed child index: skip it
+                if (lane == 0) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skip it
             } else if (has) {
                 uint32_t nmn[3], nmx[3];
                 int32_t cnt;
@@ -311,7 +342,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const uint64_t om = __ballot(act == 3);
             const int nopen = __popcll(om);
             if (sp + 2 * nopen > kStackPhys) {
-                if (lane == 0) atomicExch(&a.counters[1], 1);
+                if (lane == 0) atomicExch(&a.counters[kCtrErr], 1);
             } else if (act == 3) {
                 const int rank = __popcll(om & lt);
                 const int2 kd = a.kids[node];
@@ -348,7 +379,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             out[3 * o + 2] = oz;
         }
     }
-    if (lane == 0 && my_inter) atomicAdd(a.inter, my_inter);
+    if (lane == 0 && my_inter) {
+        atomicAdd(a.inter, my_inter);
+        atomicAdd(a.inter + 1, my_mono);
+    }
 }
 
 template <typename R>
@@ -420,6 +454,7 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     a.poly = c.poly;
     // t = x * 2/X - 1 with x = rho/(2 r_s) and X the fit range (make_split_poly)
     a.tscale = c.poly_tscale;
+    a.tscale_m = c.poly_tscale_m;
     a.escale = (float)(c.p.r_s > 0 ? -1.4426950408889634 / (4.0 * c.p.r_s * c.p.r_s) : 0.0);
     const int64_t gmax = c.n_groups_max;
     const int mode = c.p.split_mode;

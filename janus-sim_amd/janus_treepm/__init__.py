@@ -62,7 +62,8 @@ class Stats(C.Structure):
     _fields_ = [
         ("ms_total", C.c_double), ("ms_sort", C.c_double), ("ms_build", C.c_double), ("ms_deposit", C.c_double),
         ("ms_fft", C.c_double), ("ms_interp", C.c_double), ("ms_walk", C.c_double),
-        ("interactions", C.c_uint64), ("n_groups", C.c_int64), ("n_particles", C.c_int64),
+        ("interactions", C.c_uint64), ("interactions_mono", C.c_uint64), ("n_groups", C.c_int64),
+        ("n_particles", C.c_int64),
     ]
 
     def as_dict(self):
