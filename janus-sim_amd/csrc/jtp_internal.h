@@ -67,6 +67,10 @@ struct SplitPoly {
     float cm[kPolyDeg + 1];
 };
 
+// Lanes of a wavefront that share one walk target: groups of <= 16 targets use 4 lanes per
+// target, <= 32 use 2, larger groups 1.
+inline int lanes_per_target(int group_size) { return group_size <= 16 ? 4 : (group_size <= 32 ? 2 : 1); }
+
 // ---------------------------------------------------------------- context
 struct Ctx {
     jtp_params p{};

@@ -27,6 +27,7 @@ constexpr int kStackCap = 640;          // logical stack bound per wave
 constexpr int kStackPhys = kStackCap + 192;
 constexpr int kListCap = 512;
 constexpr int kFlushAt = 256;
+constexpr int kPad = 4;  // list slots kept free for the lane-sharing padding
 
 struct WalkArgs {
     const uint4* parts;
@@ -141,8 +142,11 @@ __device__ inline double4 mono_entry(const double4* mono, int32_t node, int s, c
 __device__ inline float mono_q(const uint4* mono, int32_t node, int s) { return __uint_as_float(mono[2 * node + s].w); }
 __device__ inline double mono_q(const double4* mono, int32_t node, int s) { return mono[2 * node + s].w; }
 
-template <typename R, int MODE, int WAVES>
+// LPT lanes share one target: a group holds up to 64 / LPT targets and each target's
+// interaction list is split over its LPT lanes (entries j = sub, sub + LPT, ...).
+template <typename R, int MODE, int WAVES, int LPT>
 __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
+    constexpr int GL = 64 / LPT;  // targets per wavefront
     using E = typename EntryT<R>::T;
     using M = typename std::conditional<sizeof(R) == 4, uint4, double4>::type;
     __shared__ E list_s[WAVES][kListCap];
@@ -184,11 +188,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         }
         if (g < 0) break;
         const int2 gr = a.groups[g];
-        const bool valid = lane < gr.y;
-        const int64_t ti = (int64_t)gr.x + (valid ? lane : 0);
+        const int tgt = lane % GL, sub = lane / GL;
+        const bool valid = tgt < gr.y;
+        const int64_t ti = (int64_t)gr.x + (valid ? tgt : 0);
         const uint4 rec = a.parts[ti];
-        const uint64_t vmask = __ballot(valid);
-        const int nvalid = __popcll(vmask);
+        const int nvalid = gr.y < GL ? gr.y : GL;
 
         uint32_t gmn[3], gmx[3], gc[3], gh[3];
         {
@@ -215,10 +219,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             // the counts are wave-uniform: keep the loop control in scalar registers
             const int np_ = __builtin_amdgcn_readfirstlane(np_in);
             const int nm_ = __builtin_amdgcn_readfirstlane(nm_in);
+            const int itp = (np_ + LPT - 1) / LPT, itm = (nm_ + LPT - 1) / LPT;
+            if (LPT > 1) {  // zero-charge padding so every lane reads a finite entry
+                if (lane < LPT - 1) {
+                    E z;
+                    z.x = z.y = z.z = (R)0;
+                    z.w = (R)0;
+                    L[np_ + lane] = z;
+                    L[kListCap - nm_ - 1 - lane] = z;
+                }
+                wave_lds_fence();
+            }
             R px = 0, py = 0, pz = 0;
 #pragma unroll 4
-            for (int j = 0; j < np_; ++j) {
-                const E e = L[j];
+            for (int jj = 0; jj < itp; ++jj) {
+                const E e = L[jj * LPT + sub];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
                 const R r2 = dx * dx + dy * dy + dz * dz;
                 const R f = Pair<R, MODE, true>::f(e.w, r2, a, eps2, rcut2);
@@ -226,15 +241,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 py += f * dy;
                 pz += f * dz;
             }
+            const int m0 = kListCap - itm * LPT + sub;
 #pragma unroll 4
-            for (int j = kListCap - nm_; j < kListCap; ++j) {
-                const E e = L[j];
+            for (int jj = 0; jj < itm; ++jj) {
+                const E e = L[m0 + jj * LPT];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
                 const R r2 = dx * dx + dy * dy + dz * dz;
                 const R f = Pair<R, MODE, MODE == JTP_SPLIT_POLY4>::f(e.w, r2, a, eps2, rcut2);
                 px += f * dx;
                 py += f * dy;
                 pz += f * dz;
+            }
+#pragma unroll
+            for (int off = GL; off < 64; off <<= 1) {  // combine the LPT partial sums of a target
+                px += __shfl_xor(px, off, 64);
+                py += __shfl_xor(py, off, 64);
+                pz += __shfl_xor(pz, off, 64);
             }
             if (sizeof(R) == 4) {  // compensated accumulation of the chunk partials
                 R y = px - cx, t = sx + y;
@@ -257,7 +279,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 break;
             }
             int k = sp < 64 ? sp : 64;
-            const int kl = (kListCap - lp - lm) / maxapp;
+            const int kl = (kListCap - 2 * kPad - lp - lm) / maxapp;
             k = k < kl ? k : kl;
             const int ks = kStackCap - sp > 1 ? kStackCap - sp : 1;
             k = k < ks ? k : ks;
@@ -363,7 +385,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         if (lp + lm > 0) flush(lp, lm);
         wave_lds_fence();
 
-        if (valid) {
+        if (valid && sub == 0) {
             const R sg = (int32_t)rec.w > 0 ? (R)1 : (R)-1;
             R ox = sg * sx, oy = sg * sy, oz = sg * sz;
             if (a.pm_acc) {
@@ -413,7 +435,7 @@ hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int WAVES>
+template <typename R, int MODE, int WAVES, int LPT>
 static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
@@ -422,8 +444,15 @@ static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
     const int64_t cap = (int64_t)cus * 16;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    k_walk<R, MODE, WAVES><<<dim3((unsigned)blocks), dim3(64 * WAVES), 0, s>>>(a);
+    k_walk<R, MODE, WAVES, LPT><<<dim3((unsigned)blocks), dim3(64 * WAVES), 0, s>>>(a);
     return hipGetLastError();
+}
+
+template <typename R, int WAVES, int LPT>
+static hipError_t launch_walk_mode(int mode, const WalkArgs& a, int64_t gmax, hipStream_t s) {
+    if (mode == JTP_SPLIT_ERFC_GAUSS) return launch_walk<R, JTP_SPLIT_ERFC_GAUSS, WAVES, LPT>(a, gmax, s);
+    if (mode == JTP_SPLIT_ERFC) return launch_walk<R, JTP_SPLIT_ERFC, WAVES, LPT>(a, gmax, s);
+    return launch_walk<R, JTP_SPLIT_POLY4, WAVES, LPT>(a, gmax, s);
 }
 
 hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
@@ -458,14 +487,15 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     a.escale = (float)(c.p.r_s > 0 ? -1.4426950408889634 / (4.0 * c.p.r_s * c.p.r_s) : 0.0);
     const int64_t gmax = c.n_groups_max;
     const int mode = c.p.split_mode;
+    const int lpt = lanes_per_target(c.p.group_size);
     if (c.p.precision == JTP_FP64) {
-        if (mode == JTP_SPLIT_ERFC_GAUSS) return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2>(a, gmax, s);
-        if (mode == JTP_SPLIT_ERFC) return launch_walk<double, JTP_SPLIT_ERFC, 2>(a, gmax, s);
-        return launch_walk<double, JTP_SPLIT_POLY4, 2>(a, gmax, s);
+        if (lpt == 4) return launch_walk_mode<double, 2, 4>(mode, a, gmax, s);
+        if (lpt == 2) return launch_walk_mode<double, 2, 2>(mode, a, gmax, s);
+        return launch_walk_mode<double, 2, 1>(mode, a, gmax, s);
     }
-    if (mode == JTP_SPLIT_ERFC_GAUSS) return launch_walk<float, JTP_SPLIT_ERFC_GAUSS, 4>(a, gmax, s);
-    if (mode == JTP_SPLIT_ERFC) return launch_walk<float, JTP_SPLIT_ERFC, 4>(a, gmax, s);
-    return launch_walk<float, JTP_SPLIT_POLY4, 4>(a, gmax, s);
+    if (lpt == 4) return launch_walk_mode<float, 4, 4>(mode, a, gmax, s);
+    if (lpt == 2) return launch_walk_mode<float, 4, 2>(mode, a, gmax, s);
+    return launch_walk_mode<float, 4, 1>(mode, a, gmax, s);
 }
 
 }  // namespace jtp

@@ -112,12 +112,14 @@ def test_pm_solve_numpy(jtp, precision):
     assert np.max(np.abs(phi - ref)) <= tol * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("precision,mode", [(1, 0), (0, 0), (1, 1), (0, 1), (1, 2), (0, 2)])
-def test_walk_theta0_direct(jtp, orc, precision, mode):
-    print(f"case: walk theta=0 precision={precision} mode={mode}", flush=True)
+@pytest.mark.parametrize("precision,mode,gs", [(1, 0, 64), (0, 0, 64), (1, 1, 64), (0, 1, 64), (1, 2, 64),
+                                               (0, 2, 64), (0, 0, 32), (1, 0, 16), (0, 0, 16), (0, 2, 32)])
+def test_walk_theta0_direct(jtp, orc, precision, mode, gs):
+    """theta = 0: the walk is the direct sum; group sizes 32 / 16 split each target over 2 / 4 lanes."""
+    print(f"case: walk theta=0 precision={precision} mode={mode} group={gs}", flush=True)
     w = synth.clustered(n_total=40_000, box_size=100.0, n_cells=32, n_halos=30, seed=12)
     pos = jtp.to_fixed(w.pos, w.box_size)
-    ctx = _ctx(jtp, w, precision=precision, pm_enabled=0, theta=0.0, split_mode=mode)
+    ctx = _ctx(jtp, w, precision=precision, pm_enabled=0, theta=0.0, split_mode=mode, group_size=gs)
     acc = ctx.compute_forces(pos, w.species).astype(np.float64)
     ref = orc.short_direct(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, w.r_cut, w.r_s, mode)
     e = _rel(acc, ref)
@@ -142,11 +144,12 @@ def test_walk_theta_half(jtp, orc, precision):
     assert np.median(e) < 2e-3 and np.percentile(e, 99) < 5e-2
 
 
-def test_walk_deterministic(jtp):
-    print("case: walk determinism", flush=True)
+@pytest.mark.parametrize("gs", [64, 16])
+def test_walk_deterministic(jtp, gs):
+    print(f"case: walk determinism group={gs}", flush=True)
     w = synth.clustered(n_total=100_000, box_size=100.0, n_cells=32, n_halos=40, seed=14)
     pos = jtp.to_fixed(w.pos, w.box_size)
-    ctx = _ctx(jtp, w, precision=0)
+    ctx = _ctx(jtp, w, precision=0, group_size=gs)
     a1 = ctx.compute_forces(pos, w.species)
     a2 = ctx.compute_forces(pos, w.species)
     assert np.array_equal(a1, a2)
