@@ -2,7 +2,7 @@
 
 Workload: the clustered two-species box (N = 2.2e7, 1.1e7 m+ / 1.1e7 m-, 60 % of m+ in 2000 NFW
 halos, m- rejection-sampled in under-dense regions), L = 250, PM 256^3, r_s = 1.25 cells,
-r_cut = 4.5 r_s, theta = 0.5, leaf 8, m+ = 1, |m-| = 19, eps = L / N^(1/3) / 30.
+r_cut = 4.5 r_s, theta = 0.5, leaf 4, m+ = 1, |m-| = 19, eps = L / N^(1/3) / 30.
 A step is one complete force evaluation (sort, tree, deposit, FFT, interpolation, walk) on inputs
 already resident in HBM.  Prints one JSON line.
 """
@@ -98,6 +98,18 @@ def cpu_baseline(w_full):
     }
 
 
+def measured_traffic(workload):
+    """HBM bytes per walk launch from the committed FETCH_SIZE / WRITE_SIZE summary of this workload
+    (profiles/walk_fp32_<workload>_hbm_traffic.txt, calibrated against a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", f"walk_fp32_{workload}_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("MACHINE: "):
+            return float(json.loads(line[len("MACHINE: "):])["traffic_bytes"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 WORKLOADS = {
     "clustered_22M": dict(n_total=22_000_000, box_size=250.0, n_cells=256, n_halos=2000, seed=7),
     "clustered_344k": dict(n_total=343_750, box_size=62.5, n_cells=64, n_halos=31, seed=99),
@@ -131,6 +143,57 @@ def accuracy(w, name, d_pos, d_sp, pos_fixed):
     return out
 
 
+def run_ensemble(args):
+    """Uneven ensemble of 48 boxes (synth.ensemble_c3): total particle-force-updates/s of the FP32 form
+    with the boxes run sequentially on one stream, concurrently over 4 streams, and as one replayed graph."""
+    t0 = time.perf_counter()
+    ws = synth.ensemble_c3()
+    log(f"ensemble generated in {time.perf_counter() - t0:.1f} s ({len(ws)} boxes, N={sum(w.n for w in ws)})")
+    params = []
+    for w in ws:
+        p = j.default_params(w.box_size, w.n_cells)
+        p.r_s, p.r_cut, p.theta, p.softening, p.leaf_size = w.r_s, w.r_cut, w.theta, w.softening, w.leaf_size
+        p.theta_cut = 0.7 * w.theta
+        for k, v in OVERRIDES.items():
+            setattr(p, k, v)
+        params.append(p)
+    ens = j.Ensemble(params, [w.m_plus for w in ws], [w.m_minus_abs for w in ws], n_streams=4)
+    dpos = [j.DeviceArray.from_host(j.to_fixed(w.pos, w.box_size)) for w in ws]
+    dsp = [j.DeviceArray.from_host(w.species) for w in ws]
+    dacc = [j.DeviceArray((w.n, 3), np.float32) for w in ws]
+    n = [w.n for w in ws]
+    ntot = sum(n)
+    modes = {}
+    for mode in ("sequential", "concurrent", "graph"):
+        kw = dict(sequential=(mode == "sequential"), graph=(mode == "graph"))
+        for _ in range(args.warmup):
+            ens.compute_forces_device(n, dpos, dsp, dacc, **kw)
+        j.load_library().jtp_device_synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            ens.compute_forces_device(n, dpos, dsp, dacc, **kw)
+        j.load_library().jtp_device_synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        modes[mode] = {"ms_per_eval": dt * 1e3, "updates_per_s": ntot / dt}
+        log(f"  ensemble {mode}: {dt * 1e3:.2f} ms per evaluation of all boxes")
+    st = ens.stats()
+    best = max(modes, key=lambda m: modes[m]["updates_per_s"])
+    out = {
+        "metric": "TreePM particle-force-updates/s over an uneven ensemble of 48 boxes on 1 GPU",
+        "value": modes[best]["updates_per_s"], "unit": "particle-force-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": modes[best]["ms_per_eval"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"48 lattice+Zel'dovich boxes, N_b = 2 n^3 log2-uniform in [2*32^3, 2*128^3] "
+                               f"(total {ntot}), G in {{32,64,128}}, L ~ U[100,500], |m-|/m+ ~ U[5,40]",
+                   "best_mode": best, "parallelism": "single GPU", "overrides": OVERRIDES},
+        "modes": modes, "interactions": st["interactions"],
+        "concurrent_over_sequential": modes["sequential"]["ms_per_eval"] / modes["concurrent"]["ms_per_eval"],
+        "graph_over_sequential": modes["sequential"]["ms_per_eval"] / modes["graph"]["ms_per_eval"],
+    }
+    ens.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +203,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="clustered_22M", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="clustered_22M", choices=sorted(WORKLOADS) + ["ensemble_c3"])
     ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's streaming copy kernel on MIB MiB, 5 times (HBM counter calibration)")
@@ -163,6 +226,10 @@ def main():
         for _ in range(5):
             assert j.load_library().jtp_copy_d2d(b.ptr, a.ptr, nb) == 0
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 5}))
+        return
+
+    if args.workload == "ensemble_c3":
+        run_ensemble(args)
         return
 
     t0 = time.perf_counter()
@@ -197,10 +264,13 @@ def main():
     }
     walk_ms = ph32["ms_walk"]
     achieved = FLOPS_PER_INTERACTION * inter / (walk_ms * 1e-3) / 1e12
+    traffic, traffic_src = (None, None)
+    if not OVERRIDES and args.n == 22_000_000:
+        traffic, traffic_src = measured_traffic(args.workload)
     out["roofline"] = {"bound": "valu", "kernel": "k_walk (group walk, FP32)", "achieved": achieved,
                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                       "traffic": None, "interactions": inter, "interactions_mono": inter_mono,
-                       "flops_per_interaction": FLOPS_PER_INTERACTION,
+                       "traffic": traffic, "traffic_source": traffic_src, "interactions": inter,
+                       "interactions_mono": inter_mono, "flops_per_interaction": FLOPS_PER_INTERACTION,
                        "walk_ms": walk_ms}
     G3 = w.n_cells ** 3
     grids = 2 if OVERRIDES.get("interlace", 1) else 1  # interlaced PM runs every grid phase twice

@@ -77,7 +77,7 @@ typedef struct jtp_params {
 
 /* Per-phase timings of the last jtp_compute_forces (HIP events on the call's stream). */
 typedef struct jtp_stats {
-    double  ms_total;
+    double  ms_total;     /* force call, or the whole step (kicks, drifts) after jtp_step */
     double  ms_sort;      /* Morton keys + radix sort + particle gather        */
     double  ms_build;     /* Karras hierarchy + bottom-up monopoles + groups   */
     double  ms_deposit;   /* cell offsets + deterministic CIC gather deposit   */
@@ -109,7 +109,7 @@ typedef struct jtp_stats {
  *      TreePMForce::new, treepm_force.rs:35-51) */
 typedef struct jtp_ctx jtp_ctx;
 /* Defaults: r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, softening = L/G/30,
- * G = 1, leaf 8, group 64, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on,
+ * G = 1, leaf 4, group 64, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on,
  * interlacing on, theta_cut = 0.7 theta. */
 void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells);
 int  jtp_create(const jtp_params* p, jtp_ctx** out);
@@ -129,6 +129,50 @@ int  jtp_set_species_mass(jtp_ctx* ctx, double m_plus, double m_minus_abs);
  * a_i = sigma_i * sum_j sigma_j G |m_j| (x_j - x_i)/|.|^3, periodic, mean density removed. */
 int  jtp_compute_forces(jtp_ctx* ctx, int64_t n, const uint32_t* d_pos_fixed,
                         const int8_t* d_species, void* d_acc, void* stream);
+
+/* ---- one integrator step around the force.
+ * Replaces GpuNBodyTwoPass::step_treepm_gpu (nbody_gpu_twopass.rs:4486-4649) with its kernels
+ * drift_f32 / kick_f32 / add_pm_forces (:33-78).
+ *   JTP_STEP_DKD: drift dt/2, force, kick dt, drift dt/2 (the reference's order; d_acc is output);
+ *   JTP_STEP_KDK: kick dt/2 with d_acc = a(t) (input), drift dt, force -> d_acc, kick dt/2.
+ * Kick, per species s: v += (a - H_s v dtau_dt) dt_kick (kick_f32's explicit Hubble friction;
+ * the reference passes one H for both species, JanusExpansion::at_time h_plus).
+ * Drift: u += round(v dt_drift 2^32 / L) in 64-bit integer arithmetic from an FP64 displacement,
+ * so the periodic wrap is exact (drift_f32 wraps f32 positions once per axis).
+ * d_vel and d_acc: float[n*3] (JTP_FP32) or double[n*3] (JTP_FP64), caller order.
+ * JTP_STEP_GRAPH: the step's launches are captured once into a hipGraph and replayed; the
+ * graph is re-captured when n, a pointer, the scheme, the stream, the masses or the buffer
+ * capacity change (dt and H live in device memory and never force a re-capture).
+ * Synchronises the stream before returning (status check), like jtp_compute_forces. */
+#define JTP_STEP_DKD   0
+#define JTP_STEP_KDK   1
+#define JTP_STEP_GRAPH 0x100
+int  jtp_step(jtp_ctx* ctx, int64_t n, uint32_t* d_pos_fixed, void* d_vel, const int8_t* d_species,
+              void* d_acc, int32_t flags, double dt, double hubble_plus, double hubble_minus,
+              double dtau_dt, void* stream);
+
+/* ---- ensembles of independent boxes (no counterpart in the reference, which runs one box per
+ * process; the batch replaces a loop of step_treepm_gpu / TreePMForce calls over boxes).
+ * Each box has its own parameters (L, G, r_s, ...) and species masses and its own buffers.
+ * jtp_ensemble_compute_forces evaluates the force of every box: boxes are spread over n_streams
+ * internal streams that fork from and join back into the caller's stream, so small boxes run
+ * concurrently; JTP_ENS_SEQUENTIAL runs them one after another on the caller's stream;
+ * JTP_ENS_GRAPH captures the whole batch (all streams) into one hipGraph and replays it
+ * (re-captured when a count, a pointer or the stream changes).  Results per box are identical in
+ * every mode and to jtp_compute_forces on that box.  Arrays are indexed by box. */
+typedef struct jtp_ensemble jtp_ensemble;
+#define JTP_ENS_SEQUENTIAL 0x1
+#define JTP_ENS_GRAPH      0x100
+int  jtp_ensemble_create(int32_t n_boxes, const jtp_params* params, const double* m_plus,
+                         const double* m_minus_abs, int32_t n_streams, jtp_ensemble** out);
+int  jtp_ensemble_destroy(jtp_ensemble* ens);
+const char* jtp_ensemble_last_error(const jtp_ensemble* ens);
+int  jtp_ensemble_compute_forces(jtp_ensemble* ens, const int64_t* n, const uint32_t* const* d_pos_fixed,
+                                 const int8_t* const* d_species, void* const* d_acc, int32_t flags,
+                                 void* stream);
+/* box >= 0: that box's phases (-1 when graph-replayed); box = -1: ms_total of the whole batch,
+ * summed interactions, groups and particles. */
+int  jtp_ensemble_get_stats(jtp_ensemble* ens, int32_t box, jtp_stats* out);
 
 /* Position conversion to fixed point on the device (x in any period; wrapped). */
 int  jtp_pos_to_fixed_f64(jtp_ctx* ctx, int64_t n, const double* d_pos, uint32_t* d_fixed, void* stream);

@@ -51,18 +51,22 @@ static hipError_t dalloc_bytes(void*& p, size_t bytes) {
     return hipMalloc(&p, bytes);
 }
 
-// Chebyshev interpolants (degree kPolyDeg) of the split factor for the FP32 walk, converted to
-// polynomials in t = 2x/X - 1 for Horner evaluation:
-//   particles: S(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)] on [0, Xp], Xp covering rho <= sqrt(r_cut^2 + eps^2)
-//              (absolute error < 1e-5 for Xp up to ~2.5, i.e. r_cut = 4.5 r_s);
-//   monopoles: P(x) = erfcx(x) [+ 2x/sqrt(pi)] on [0, Xm], Xm = Xp (1 + sqrt(3) theta), used as
-//              exp(-x^2) P(x) (absolute error ~2e-5 of the pair force at theta = 0.5).
-static void cheb_fit(double X, int deg, float* out, double (*f)(double, bool), bool gauss) {
+// Chebyshev interpolants of the split factor for the FP32 walk (forms and degrees in
+// jtp_internal.h), with maximum errors at r_cut = 4.5 r_s, theta = 0.5 (FP32 Horner included,
+// relative to the Newtonian pair force):
+//   particles, erfc + Gaussian: 1 / rho^3 + H(rho^2), degree 8: 6e-6;
+//   particles, erfc only: S(x), degree 10: 6e-6;
+//   monopoles: exp(-x^2) P(x), P = erfcx(x) [+ 2x/sqrt(pi)] on [0, Xm], Xm = Xp (1 + sqrt(3) theta),
+//              degree 8: 1.5e-4 (well below the monopole truncation error at theta = 0.5).
+// Degree-deg Chebyshev interpolant of f on [0, X], returned as monomial coefficients of
+// t = 2 y / X - 1 (zeros above deg, up to kPolyDeg).
+template <typename F>
+static void cheb_fit(double X, int deg, float* out, F f) {
     const int N = deg + 1;
     std::vector<double> fv(N), cheb(N, 0.0);
     for (int k = 0; k < N; ++k) {
         const double t = std::cos(M_PI * (k + 0.5) / N);
-        fv[k] = f(0.5 * X * (t + 1.0), gauss);
+        fv[k] = f(0.5 * X * (t + 1.0));
     }
     for (int j = 0; j < N; ++j) {
         double s = 0;
@@ -75,17 +79,35 @@ static void cheb_fit(double X, int deg, float* out, double (*f)(double, bool), b
     if (N > 1) T[1][1] = 1.0;
     for (int j = 1; j + 1 < N; ++j)
         for (int d = 0; d < N; ++d) T[j + 1][d] = -T[j - 1][d] + (d > 0 ? 2.0 * T[j][d - 1] : 0.0);
-    for (int d = 0; d < N; ++d) {
+    for (int d = 0; d <= kPolyDeg; ++d) {
         double s = 0;
-        for (int j = 0; j < N; ++j) s += cheb[j] * T[j][d];
+        if (d < N)
+            for (int j = 0; j < N; ++j) s += cheb[j] * T[j][d];
         out[d] = (float)s;
     }
 }
+// S(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)]: the short-range share of the pair force.
 static double split_direct(double x, bool gauss) {
     return std::erfc(x) + (gauss ? 2.0 * x / std::sqrt(M_PI) * std::exp(-x * x) : 0.0);
 }
+// exp(x^2) S(x) = erfcx(x) [+ 2x/sqrt(pi)].
 static double split_scaled(double x, bool gauss) {
     return std::erfc(x) * std::exp(x * x) + (gauss ? 2.0 * x / std::sqrt(M_PI) : 0.0);
+}
+// (S(x) - 1) / x^3 for the erfc + Gaussian law.  Its Taylor series follows from those of erfc and
+// x exp(-x^2): S(x) - 1 = (2/sqrt(pi)) sum_{n>=1} (-1)^n x^{2n+1} / n! * 2n / (2n+1); the series is
+// used below x = 0.5, where the direct form cancels.
+static double split_excess_x3(double x) {
+    if (x < 0.5) {
+        const double x2 = x * x;
+        double term = 1.0, sum = 0.0;  // term = x^{2n-2} / n!
+        for (int nn = 1; nn < 30; ++nn) {
+            term = (nn == 1) ? 1.0 : term * x2 / nn;
+            sum += ((nn & 1) ? -1.0 : 1.0) * term * (2.0 * nn) / (2.0 * nn + 1.0);
+        }
+        return 2.0 / std::sqrt(M_PI) * sum;
+    }
+    return (split_direct(x, true) - 1.0) / (x * x * x);
 }
 
 void make_split_poly(Ctx& c) {
@@ -94,9 +116,19 @@ void make_split_poly(Ctx& c) {
     const double Xp = rho_max / (2.0 * c.p.r_s) * 1.0001 + 1e-12;
     const double Xm = Xp * (1.0 + std::sqrt(3.0) * c.p.theta);
     const bool gauss = c.p.split_mode == JTP_SPLIT_ERFC_GAUSS;
-    cheb_fit(Xp, kPolyDeg, c.poly.cp, split_direct, gauss);
-    cheb_fit(Xm, kPolyDeg, c.poly.cm, split_scaled, gauss);
-    c.poly_tscale = (float)(1.0 / (c.p.r_s * Xp));
+    if (gauss) {
+        // H(v) = (S(x) - 1) / rho^3 with rho = sqrt(v), x = rho / (2 r_s)
+        const double Vp = rho_max * rho_max * 1.0002 + 1e-30;
+        const double cinv = 1.0 / (2.0 * c.p.r_s);
+        cheb_fit(Vp, kPolyDegPair, c.poly.cp,
+                 [&](double v) { return cinv * cinv * cinv * split_excess_x3(std::sqrt(v) * cinv); });
+        c.poly_tscale = (float)(2.0 / Vp);
+        cheb_fit(Xm, kPolyDegMono, c.poly.cm, [&](double x) { return split_scaled(x, true); });
+    } else {
+        cheb_fit(Xp, kPolyDeg, c.poly.cp, [&](double x) { return split_direct(x, false); });
+        c.poly_tscale = (float)(1.0 / (c.p.r_s * Xp));
+        cheb_fit(Xm, kPolyDeg, c.poly.cm, [&](double x) { return split_scaled(x, false); });
+    }
     c.poly_tscale_m = (float)(1.0 / (c.p.r_s * Xm));
 }
 
@@ -206,7 +238,7 @@ void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
     p->theta = 0.5;
     p->softening = h / 30.0;
     p->g_constant = 1.0;
-    p->leaf_size = 8;
+    p->leaf_size = 4;
     p->group_size = 64;
     p->precision = JTP_FP32;
     p->split_mode = JTP_SPLIT_ERFC_GAUSS;
@@ -254,9 +286,11 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_pinned, 4 * sizeof(unsigned long long), 0);
+    if (e == hipSuccess) e = dalloc(c.dev_step, 1);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_step, sizeof(StepParams), 0);
     if (e == hipSuccess) e = pm_tables(c);
     if (e == hipSuccess) {
-        for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c.ev[i]);
+        for (int i = 0; i < 10 && e == hipSuccess; ++i) e = hipEventCreate(&c.ev[i]);
         c.events_ok = (e == hipSuccess);
     }
     if (e != hipSuccess) {
@@ -275,12 +309,15 @@ int jtp_destroy(jtp_ctx* h) {
     Ctx& c = h->c;
     (void)hipDeviceSynchronize();
     free_particle_buffers(c);
+    if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
+    if (c.graph.capture_stream) (void)hipStreamDestroy(c.graph.capture_stream);
     void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4,
-                    c.dev_counters, c.dev_inter, c.dense_slot};
+                    c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);
-    for (int i = 0; i < 8; ++i)
+    if (c.host_step) (void)hipHostFree(c.host_step);
+    for (int i = 0; i < 10; ++i)
         if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
     delete h;
     return JTP_OK;
@@ -294,6 +331,61 @@ int jtp_set_species_mass(jtp_ctx* h, double m_plus, double m_minus_abs) {
         return fail(&h->c, JTP_ERR_ARG, "species masses must be finite and >= 0");
     h->c.m_plus = m_plus;
     h->c.m_minus = m_minus_abs;
+    h->c.graph_dirty = true;  // the masses are kernel arguments of a captured step
+    return JTP_OK;
+}
+
+// Enqueues one force evaluation and the copies of its status words to pinned host memory; no
+// synchronisation, so it can be captured into a graph.  `timed` brackets the phases with events.
+static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t* d_species, void* d_acc,
+                          hipStream_t s, bool timed) {
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[0], s));
+    JTP_HIP(&c, launch_fill_i32((int32_t*)c.dev_inter, 4, 0, s));
+    JTP_HIP(&c, launch_morton_keys(d_pos, c.keys0, c.vals0, n, s));
+    JTP_HIP(&c, radix_sort_pairs(c, n, s));
+    JTP_HIP(&c, launch_gather_parts(d_pos, d_species, c.vals0, c.parts, n, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[1], s));
+    if (c.p.tree_enabled) JTP_HIP(&c, build_tree(c, n, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[2], s));
+    if (c.p.pm_enabled) JTP_HIP(&c, pm_deposit(c, n, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[3], s));
+    if (c.p.pm_enabled) JTP_HIP(&c, pm_fft_solve(c, c.rho, c.phi, s));
+    if (c.p.pm_enabled && c.p.interlace) JTP_HIP(&c, pm_fft_solve(c, c.rho2, c.phi2, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[4], s));
+    if (c.p.pm_enabled) JTP_HIP(&c, pm_interp(c, n, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[5], s));
+    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s));
+    else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[6], s));
+    // status: interaction counts, group count, walk error flag, root flag
+    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned, c.dev_inter, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 3, c.dev_inter + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 1, c.dev_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (c.p.tree_enabled && n >= 2)
+        JTP_HIP(&c, hipMemcpyAsync((int32_t*)(c.host_pinned + 2), c.dev_counters + kCtrRoot, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
+    return JTP_OK;
+}
+
+// Interprets the status words of the last force evaluation (after the stream synchronised).
+static int check_status(Ctx& c, int64_t n) {
+    c.have_result = true;
+    if (c.p.tree_enabled) {
+        const int32_t* cnt = (const int32_t*)(c.host_pinned + 1);
+        if (n >= 2 && ((const int32_t*)(c.host_pinned + 2))[0] != 1)
+            return fail(&c, JTP_ERR_STATE, "bottom-up pass did not reach the root (malformed hierarchy)");
+        if (cnt[1] == 1) return fail(&c, JTP_ERR_OVERFLOW, "walk stack overflow: a group's traversal exceeded the stack bound");
+        if (cnt[1] != 0) return fail(&c, JTP_ERR_STATE, "walk found a malformed hierarchy (code " + std::to_string(cnt[1]) + ")");
+    }
+    return JTP_OK;
+}
+
+static int prepare(Ctx& c, int64_t n) {
+    const int64_t old_cap = c.cap_n;
+    int rc = ensure_capacity(c, n);
+    if (rc != JTP_OK) return rc;
+    if (c.cap_n != old_cap) c.graph_dirty = true;  // buffers moved
+    c.n_groups_max = (n + c.p.group_size - 1) / c.p.group_size + 4096;
     return JTP_OK;
 }
 
@@ -307,49 +399,127 @@ int jtp_compute_forces(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_
     c.have_result = false;
     c.last_n = n;
     c.last_stream = s;
+    c.last_kind = 0;
     if (n == 0) {
         c.have_result = true;
         return JTP_OK;
     }
-    int rc = ensure_capacity(c, n);
+    int rc = prepare(c, n);
     if (rc != JTP_OK) return rc;
-    c.n_groups_max = (n + c.p.group_size - 1) / c.p.group_size + 4096;
-
-    JTP_HIP(&c, hipEventRecord(c.ev[0], s));
-    JTP_HIP(&c, launch_fill_i32((int32_t*)c.dev_inter, 4, 0, s));
-    JTP_HIP(&c, launch_morton_keys(d_pos, c.keys0, c.vals0, n, s));
-    JTP_HIP(&c, radix_sort_pairs(c, n, s));
-    JTP_HIP(&c, launch_gather_parts(d_pos, d_species, c.vals0, c.parts, n, s));
-    JTP_HIP(&c, hipEventRecord(c.ev[1], s));
-    if (c.p.tree_enabled) JTP_HIP(&c, build_tree(c, n, s));
-    JTP_HIP(&c, hipEventRecord(c.ev[2], s));
-    if (c.p.pm_enabled) JTP_HIP(&c, pm_deposit(c, n, s));
-    JTP_HIP(&c, hipEventRecord(c.ev[3], s));
-    if (c.p.pm_enabled) JTP_HIP(&c, pm_fft_solve(c, c.rho, c.phi, s));
-    if (c.p.pm_enabled && c.p.interlace) JTP_HIP(&c, pm_fft_solve(c, c.rho2, c.phi2, s));
-    JTP_HIP(&c, hipEventRecord(c.ev[4], s));
-    if (c.p.pm_enabled) JTP_HIP(&c, pm_interp(c, n, s));
-    JTP_HIP(&c, hipEventRecord(c.ev[5], s));
-    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s));
-    else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
-    JTP_HIP(&c, hipEventRecord(c.ev[6], s));
-    // status: interaction count, group count, walk error flag
-    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned, c.dev_inter, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 3, c.dev_inter + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 1, c.dev_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    if (c.p.tree_enabled && n >= 2)
-        JTP_HIP(&c, hipMemcpyAsync((int32_t*)(c.host_pinned + 2), c.dev_counters + kCtrRoot, sizeof(int32_t),
-                                   hipMemcpyDeviceToHost, s));
+    rc = enqueue_forces(c, n, d_pos, d_species, d_acc, s, true);
+    if (rc != JTP_OK) return rc;
     JTP_HIP(&c, hipStreamSynchronize(s));
-    c.have_result = true;
-    if (c.p.tree_enabled) {
-        const int32_t* cnt = (const int32_t*)(c.host_pinned + 1);
-        if (n >= 2 && ((const int32_t*)(c.host_pinned + 2))[0] != 1)
-            return fail(&c, JTP_ERR_OVERFLOW, "tree deeper than the bottom-up round limit");
-        if (cnt[1] == 1) return fail(&c, JTP_ERR_OVERFLOW, "walk stack overflow: a group's traversal exceeded the stack bound");
-        if (cnt[1] != 0) return fail(&c, JTP_ERR_STATE, "walk found a malformed hierarchy (code " + std::to_string(cnt[1]) + ")");
+    return check_status(c, n);
+}
+
+// One step: kicks and drifts around an enqueued force evaluation (see jtp_step in the header).
+static int enqueue_step(Ctx& c, int64_t n, uint32_t* d_pos, void* d_vel, const int8_t* d_species, void* d_acc,
+                        int scheme, hipStream_t s, bool timed) {
+    if (scheme == JTP_STEP_DKD) {
+        JTP_HIP(&c, launch_drift(c, d_pos, d_vel, n, s));
+        int rc = enqueue_forces(c, n, d_pos, d_species, d_acc, s, timed);
+        if (rc != JTP_OK) return rc;
+        JTP_HIP(&c, launch_kick(c, d_vel, d_acc, d_species, n, s));
+        JTP_HIP(&c, launch_drift(c, d_pos, d_vel, n, s));
+    } else {
+        JTP_HIP(&c, launch_kick(c, d_vel, d_acc, d_species, n, s));
+        JTP_HIP(&c, launch_drift(c, d_pos, d_vel, n, s));
+        int rc = enqueue_forces(c, n, d_pos, d_species, d_acc, s, timed);
+        if (rc != JTP_OK) return rc;
+        JTP_HIP(&c, launch_kick(c, d_vel, d_acc, d_species, n, s));
     }
     return JTP_OK;
+}
+
+static void drop_graph(Ctx& c) {
+    if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
+    c.graph.exec = nullptr;
+    c.graph.n = -1;
+}
+
+// Captures the step into c.graph (on the context's own capture stream: the null stream cannot be
+// captured) unless the cached graph was captured for the same arguments.
+static int ensure_graph(Ctx& c, int64_t n, uint32_t* d_pos, void* d_vel, const int8_t* d_species, void* d_acc,
+                        int scheme, hipStream_t s) {
+    StepGraph& g = c.graph;
+    if (g.exec && !c.graph_dirty && g.n == n && g.pos == d_pos && g.vel == d_vel && g.species == d_species &&
+        g.acc == d_acc && g.scheme == scheme && g.stream == s)
+        return JTP_OK;
+    drop_graph(c);
+    if (!g.capture_stream) JTP_HIP(&c, hipStreamCreateWithFlags(&g.capture_stream, hipStreamNonBlocking));
+    JTP_HIP(&c, hipStreamBeginCapture(g.capture_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c, n, d_pos, d_vel, d_species, d_acc, scheme, g.capture_stream, false);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g.capture_stream, &graph);
+    if (rc != JTP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(&c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g.exec = nullptr;
+        return hip_fail(&c, e, "hipGraphInstantiate");
+    }
+    g.n = n;
+    g.pos = d_pos;
+    g.vel = d_vel;
+    g.species = d_species;
+    g.acc = d_acc;
+    g.scheme = scheme;
+    g.stream = s;
+    c.graph_dirty = false;
+    return JTP_OK;
+}
+
+int jtp_step(jtp_ctx* h, int64_t n, uint32_t* d_pos, void* d_vel, const int8_t* d_species, void* d_acc,
+             int32_t flags, double dt, double hubble_plus, double hubble_minus, double dtau_dt, void* stream) {
+    if (!h) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    const int scheme = flags & 0xff;
+    if ((flags & ~(0xff | JTP_STEP_GRAPH)) || (scheme != JTP_STEP_DKD && scheme != JTP_STEP_KDK))
+        return fail(&c, JTP_ERR_ARG, "unknown step flags");
+    if (n < 0 || n > (int64_t)0x7fffffff - 8192) return fail(&c, JTP_ERR_ARG, "n out of range");
+    if (n > 0 && (!d_pos || !d_vel || !d_species || !d_acc)) return fail(&c, JTP_ERR_ARG, "null device pointer");
+    if (!std::isfinite(dt) || !std::isfinite(hubble_plus) || !std::isfinite(hubble_minus) || !std::isfinite(dtau_dt))
+        return fail(&c, JTP_ERR_ARG, "step scalars must be finite");
+    hipStream_t s = (hipStream_t)stream;
+    const bool use_graph = (flags & JTP_STEP_GRAPH) != 0;
+    c.have_result = false;
+    c.last_n = n;
+    c.last_stream = s;
+    c.last_kind = use_graph ? 2 : 1;
+    if (n == 0) {
+        c.have_result = true;
+        return JTP_OK;
+    }
+    int rc = prepare(c, n);
+    if (rc != JTP_OK) return rc;
+
+    StepParams& sp = *c.host_step;  // the previous step synchronised, so the staging buffer is free
+    std::memset(&sp, 0, sizeof(sp));
+    const double units = 4294967296.0 / c.p.box_size;
+    sp.kick_dt = scheme == JTP_STEP_DKD ? dt : 0.5 * dt;
+    sp.drift_units = (scheme == JTP_STEP_DKD ? 0.5 * dt : dt) * units;
+    sp.hubble_plus = hubble_plus;
+    sp.hubble_minus = hubble_minus;
+    sp.dtau_dt = dtau_dt;
+    if (use_graph) {
+        rc = ensure_graph(c, n, d_pos, d_vel, d_species, d_acc, scheme, s);
+        if (rc != JTP_OK) return rc;
+    }
+    JTP_HIP(&c, hipEventRecord(c.ev[8], s));
+    JTP_HIP(&c, hipMemcpyAsync(c.dev_step, c.host_step, sizeof(StepParams), hipMemcpyHostToDevice, s));
+    if (use_graph) {
+        JTP_HIP(&c, hipGraphLaunch(c.graph.exec, s));
+    } else {
+        rc = enqueue_step(c, n, d_pos, d_vel, d_species, d_acc, scheme, s, true);
+        if (rc != JTP_OK) return rc;
+    }
+    JTP_HIP(&c, hipEventRecord(c.ev[9], s));
+    JTP_HIP(&c, hipStreamSynchronize(s));
+    return check_status(c, n);
 }
 
 int jtp_pos_to_fixed_f64(jtp_ctx* h, int64_t n, const double* d_pos, uint32_t* d_fixed, void* stream) {
@@ -388,17 +558,26 @@ int jtp_get_stats(jtp_ctx* h, jtp_stats* out) {
     if (!c.have_result) return fail(&c, JTP_ERR_STATE, "no completed force evaluation");
     out->n_particles = c.last_n;
     if (c.last_n <= 0) return JTP_OK;
-    JTP_HIP(&c, hipEventSynchronize(c.ev[6]));
-    float ms[6];
-    for (int i = 0; i < 6; ++i) JTP_HIP(&c, hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]));
+    // force phases from ev[0..6] (not recorded inside a captured step: reported as -1);
+    // the total covers the force call, or the whole step with its kicks and drifts
+    float ms[6] = {-1, -1, -1, -1, -1, -1};
+    if (c.last_kind < 2) {
+        JTP_HIP(&c, hipEventSynchronize(c.ev[6]));
+        for (int i = 0; i < 6; ++i) JTP_HIP(&c, hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]));
+    }
     out->ms_sort = ms[0];
     out->ms_build = ms[1];
     out->ms_deposit = ms[2];
     out->ms_fft = ms[3];
     out->ms_interp = ms[4];
     out->ms_walk = ms[5];
-    float tot;
-    JTP_HIP(&c, hipEventElapsedTime(&tot, c.ev[0], c.ev[6]));
+    float tot = -1;
+    if (c.last_kind == 0) {
+        JTP_HIP(&c, hipEventElapsedTime(&tot, c.ev[0], c.ev[6]));
+    } else if (c.last_kind != 3) {
+        JTP_HIP(&c, hipEventSynchronize(c.ev[9]));
+        JTP_HIP(&c, hipEventElapsedTime(&tot, c.ev[8], c.ev[9]));
+    }
     out->ms_total = tot;
     out->interactions = c.host_pinned[0];
     out->interactions_mono = c.host_pinned[3];
@@ -559,6 +738,225 @@ int jtp_device_count(int* out) {
         return JTP_ERR_HIP;
     }
     *out = n;
+    return JTP_OK;
+}
+
+// ---------------------------------------------------------------- ensembles of boxes
+constexpr int kMaxEnsStreams = 8;
+
+struct jtp_ensemble {
+    std::vector<jtp_ctx*> boxes;
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    hipEvent_t join[kMaxEnsStreams]{};
+    std::string err;
+    // captured batch
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    std::vector<int64_t> g_n;
+    std::vector<const void*> g_ptrs;
+    hipStream_t g_stream = nullptr;
+    int last_mode = 0;  // 0 concurrent, 1 sequential, 2 graph
+    bool have_result = false;
+};
+
+static int ens_fail(jtp_ensemble* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    return code;
+}
+#define ENS_HIP(e, call)                                                                    \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return ens_fail((e), JTP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int jtp_ensemble_destroy(jtp_ensemble* e) {
+    if (!e) return JTP_OK;
+    (void)hipDeviceSynchronize();
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->capture_stream) (void)hipStreamDestroy(e->capture_stream);
+    for (hipStream_t s : e->streams)
+        if (s) (void)hipStreamDestroy(s);
+    hipEvent_t evs[] = {e->fork, e->t0, e->t1};
+    for (hipEvent_t v : evs)
+        if (v) (void)hipEventDestroy(v);
+    for (int k = 0; k < kMaxEnsStreams; ++k)
+        if (e->join[k]) (void)hipEventDestroy(e->join[k]);
+    for (jtp_ctx* b : e->boxes) jtp_destroy(b);
+    delete e;
+    return JTP_OK;
+}
+
+int jtp_ensemble_create(int32_t n_boxes, const jtp_params* params, const double* m_plus, const double* m_minus_abs,
+                        int32_t n_streams, jtp_ensemble** out) {
+    if (!out) return JTP_ERR_ARG;
+    *out = nullptr;
+    if (n_boxes < 1 || !params || !m_plus || !m_minus_abs) return JTP_ERR_ARG;
+    if (n_streams == 0) n_streams = 4;
+    if (n_streams < 1 || n_streams > kMaxEnsStreams) return JTP_ERR_ARG;
+    jtp_ensemble* e = new (std::nothrow) jtp_ensemble();
+    if (!e) return JTP_ERR_ALLOC;
+    for (int b = 0; b < n_boxes; ++b) {
+        jtp_ctx* h = nullptr;
+        int rc = jtp_create(&params[b], &h);
+        if (rc == JTP_OK) rc = jtp_set_species_mass(h, m_plus[b], m_minus_abs[b]);
+        if (rc != JTP_OK) {
+            if (h) jtp_destroy(h);
+            jtp_ensemble_destroy(e);
+            return rc;
+        }
+        e->boxes.push_back(h);
+    }
+    hipError_t er = hipSuccess;
+    for (int k = 0; k < n_streams && er == hipSuccess; ++k) {
+        hipStream_t s = nullptr;
+        er = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);  // default priority
+        if (er == hipSuccess) e->streams.push_back(s);
+        if (er == hipSuccess) er = hipEventCreateWithFlags(&e->join[k], hipEventDisableTiming);
+    }
+    if (er == hipSuccess) er = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+    if (er == hipSuccess) er = hipEventCreate(&e->t0);
+    if (er == hipSuccess) er = hipEventCreate(&e->t1);
+    if (er != hipSuccess) {
+        (void)hipGetLastError();
+        jtp_ensemble_destroy(e);
+        return JTP_ERR_HIP;
+    }
+    *out = e;
+    return JTP_OK;
+}
+
+const char* jtp_ensemble_last_error(const jtp_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+// Enqueues every box.  concurrent: fork from `s`, box b on streams[b % K], join back into `s`.
+static int ens_enqueue(jtp_ensemble* e, const int64_t* n, const uint32_t* const* pos, const int8_t* const* sp,
+                       void* const* acc, bool concurrent, hipStream_t s, bool timed) {
+    const int K = (int)e->streams.size();
+    if (concurrent) {
+        ENS_HIP(e, hipEventRecord(e->fork, s));
+        for (int k = 0; k < K; ++k) ENS_HIP(e, hipStreamWaitEvent(e->streams[k], e->fork, 0));
+    }
+    for (size_t b = 0; b < e->boxes.size(); ++b) {
+        if (n[b] == 0) continue;
+        Ctx& c = e->boxes[b]->c;
+        hipStream_t bs = concurrent ? e->streams[b % K] : s;
+        int rc = enqueue_forces(c, n[b], pos[b], sp[b], acc[b], bs, timed);
+        if (rc != JTP_OK) return ens_fail(e, rc, "box " + std::to_string(b) + ": " + c.err);
+    }
+    if (concurrent) {
+        for (int k = 0; k < K; ++k) {
+            ENS_HIP(e, hipEventRecord(e->join[k], e->streams[k]));
+            ENS_HIP(e, hipStreamWaitEvent(s, e->join[k], 0));
+        }
+    }
+    return JTP_OK;
+}
+
+int jtp_ensemble_compute_forces(jtp_ensemble* e, const int64_t* n, const uint32_t* const* pos,
+                                const int8_t* const* sp, void* const* acc, int32_t flags, void* stream) {
+    if (!e) return JTP_ERR_ARG;
+    if (!n || !pos || !sp || !acc) return ens_fail(e, JTP_ERR_ARG, "null array");
+    if (flags & ~(JTP_ENS_SEQUENTIAL | JTP_ENS_GRAPH)) return ens_fail(e, JTP_ERR_ARG, "unknown flags");
+    const bool graph = (flags & JTP_ENS_GRAPH) != 0;
+    const bool concurrent = !(flags & JTP_ENS_SEQUENTIAL);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t B = e->boxes.size();
+    e->have_result = false;
+    bool dirty = false;
+    for (size_t b = 0; b < B; ++b) {
+        if (n[b] < 0 || n[b] > (int64_t)0x7fffffff - 8192) return ens_fail(e, JTP_ERR_ARG, "n out of range");
+        if (n[b] > 0 && (!pos[b] || !sp[b] || !acc[b])) return ens_fail(e, JTP_ERR_ARG, "null device pointer");
+        Ctx& c = e->boxes[b]->c;
+        c.have_result = false;
+        c.last_n = n[b];
+        c.last_kind = graph ? 3 : 0;
+        c.last_stream = s;
+        if (n[b] > 0) {
+            int rc = prepare(c, n[b]);  // allocations happen here, outside any capture
+            if (rc != JTP_OK) return ens_fail(e, rc, "box " + std::to_string(b) + ": " + c.err);
+            dirty = dirty || c.graph_dirty;
+        }
+    }
+    e->last_mode = graph ? 2 : (concurrent ? 0 : 1);
+    if (graph) {
+        std::vector<const void*> ptrs;
+        for (size_t b = 0; b < B; ++b) {
+            ptrs.push_back(pos[b]);
+            ptrs.push_back(sp[b]);
+            ptrs.push_back(acc[b]);
+        }
+        std::vector<int64_t> nv(n, n + B);
+        ptrs.push_back((const void*)(intptr_t)(concurrent ? 1 : 0));
+        if (!e->exec || dirty || nv != e->g_n || ptrs != e->g_ptrs || s != e->g_stream) {
+            if (e->exec) (void)hipGraphExecDestroy(e->exec);
+            e->exec = nullptr;
+            if (!e->capture_stream) ENS_HIP(e, hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
+            ENS_HIP(e, hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal));
+            int rc = ens_enqueue(e, n, pos, sp, acc, concurrent, e->capture_stream, false);
+            hipGraph_t g = nullptr;
+            hipError_t er = hipStreamEndCapture(e->capture_stream, &g);
+            if (rc != JTP_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (er != hipSuccess) return ens_fail(e, JTP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(er));
+            er = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (er != hipSuccess) {
+                e->exec = nullptr;
+                return ens_fail(e, JTP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(er));
+            }
+            e->g_n = nv;
+            e->g_ptrs = ptrs;
+            e->g_stream = s;
+            // the captured kernels bake in every box's current masses and buffers
+            for (size_t b = 0; b < B; ++b) e->boxes[b]->c.graph_dirty = false;
+        }
+        ENS_HIP(e, hipEventRecord(e->t0, s));
+        ENS_HIP(e, hipGraphLaunch(e->exec, s));
+    } else {
+        ENS_HIP(e, hipEventRecord(e->t0, s));
+        int rc = ens_enqueue(e, n, pos, sp, acc, concurrent, s, true);
+        if (rc != JTP_OK) return rc;
+    }
+    ENS_HIP(e, hipEventRecord(e->t1, s));
+    ENS_HIP(e, hipStreamSynchronize(s));
+    e->have_result = true;
+    for (size_t b = 0; b < B; ++b) {
+        Ctx& c = e->boxes[b]->c;
+        if (n[b] == 0) {
+            c.have_result = true;
+            continue;
+        }
+        int rc = check_status(c, n[b]);
+        if (rc != JTP_OK) return ens_fail(e, rc, "box " + std::to_string(b) + ": " + c.err);
+    }
+    return JTP_OK;
+}
+
+int jtp_ensemble_get_stats(jtp_ensemble* e, int32_t box, jtp_stats* out) {
+    if (!e || !out) return JTP_ERR_ARG;
+    if (box >= (int32_t)e->boxes.size() || box < -1) return ens_fail(e, JTP_ERR_ARG, "box out of range");
+    if (!e->have_result) return ens_fail(e, JTP_ERR_STATE, "no completed ensemble evaluation");
+    if (box >= 0) {
+        int rc = jtp_get_stats(e->boxes[box], out);
+        if (rc != JTP_OK) return ens_fail(e, rc, e->boxes[box]->c.err);
+        return JTP_OK;  // inside the batch graph nothing is timed per box (all -1)
+    }
+    std::memset(out, 0, sizeof(*out));
+    ENS_HIP(e, hipEventSynchronize(e->t1));
+    float ms = 0;
+    ENS_HIP(e, hipEventElapsedTime(&ms, e->t0, e->t1));
+    out->ms_total = ms;
+    out->ms_sort = out->ms_build = out->ms_deposit = out->ms_fft = out->ms_interp = out->ms_walk = -1;
+    for (jtp_ctx* h : e->boxes) {
+        const Ctx& c = h->c;
+        if (c.last_n <= 0) continue;
+        out->interactions += c.host_pinned[0];
+        out->interactions_mono += c.host_pinned[3];
+        out->n_groups += ((const int32_t*)(c.host_pinned + 1))[0];
+        out->n_particles += c.last_n;
+    }
     return JTP_OK;
 }
 

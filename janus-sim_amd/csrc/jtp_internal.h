@@ -21,11 +21,11 @@ constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Mor
 constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
 constexpr int kFftPoints = 4096;     // complex points held in LDS per FFT workgroup
 constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by one wavefront
-constexpr int kMaxRounds = 128;      // bottom-up levels (tree height bound, checked)
+constexpr int kMaxRounds = 128;      // size of the bottom-up control block (frontier count ... root flag)
 constexpr int kXcds = 8;             // MI355X accelerator complex dies; workgroup i runs on XCD i % 8
 // device counter block (int32): offsets
 constexpr int kCtrNGroups = 0, kCtrErr = 1, kCtrDense = 3;
-constexpr int kCtrRounds = 8;                       // kMaxRounds + 2 entries (round sizes, root flag)
+constexpr int kCtrRounds = 8;                       // [0] global frontier size, [kMaxRounds+1] root flag
 constexpr int kCtrRoot = kCtrRounds + kMaxRounds + 1;
 constexpr int kCtrXcd = kCtrRounds + kMaxRounds + 4;  // kXcds walk-queue heads
 constexpr int kCtrTotal = kCtrXcd + kXcds + 4;
@@ -56,12 +56,17 @@ __host__ __device__ inline uint64_t morton_fixed(uint32_t ux, uint32_t uy, uint3
     return spread3_21(ux >> 11) | (spread3_21(uy >> 11) << 1) | (spread3_21(uz >> 11) << 2);
 }
 
-// Split-function coefficients for the FP32 walk, x = rho / (2 r_s):
-//  * particle pairs (always cut at r_cut, so x <= Xp): S(x) ~= Pp(t), t = 2x/Xp - 1, Pp the
-//    degree-kPolyDeg Chebyshev interpolant of S(x) = erfc(x) [+ 2x/sqrt(pi) exp(-x^2)];
+// Split-function coefficients for the FP32 walk, rho^2 = r^2 + eps^2, x = rho / (2 r_s):
+//  * particle pairs under erfc + Gaussian (always cut at r_cut): S(x) - 1 = x^3 h(x^2) with h entire,
+//    so S / rho^3 = 1 / rho^3 + H(rho^2), H(v) = (S(x) - 1) / rho^3 interpolated by the degree
+//    kPolyDegPair Chebyshev interpolant Pp(t), t = 2 rho^2 / Vp - 1 on [0, Vp = rho_max^2]:
+//    no rho, no separate S, and a faster-converging fit than S(x) itself;
+//  * particle pairs under erfc only: S(x) ~= Pp(t), t = 2x/Xp - 1, degree kPolyDeg;
 //  * monopoles (no cut under the erfc laws, x up to Xm ~ r_cut (1 + sqrt(3) theta) / 2 r_s):
-//    S(x) ~= exp(-x^2) Pm(t), t = min(2x/Xm - 1, 1), Pm interpolating erfcx(x) [+ 2x/sqrt(pi)],
-//    which stays bounded where x runs past the fitted range.
+//    S(x) ~= exp(-x^2) Pm(t), t = min(2x/Xm - 1, 1), Pm interpolating erfcx(x) [+ 2x/sqrt(pi)]
+//    (degree kPolyDegMono, kPolyDeg under erfc only), which stays bounded past the fitted range.
+constexpr int kPolyDegPair = 8;
+constexpr int kPolyDegMono = 8;
 struct SplitPoly {
     float cp[kPolyDeg + 1];
     float cm[kPolyDeg + 1];
@@ -70,6 +75,26 @@ struct SplitPoly {
 // Lanes of a wavefront that share one walk target: groups of <= 16 targets use 4 lanes per
 // target, <= 32 use 2, larger groups 1.
 inline int lanes_per_target(int group_size) { return group_size <= 16 ? 4 : (group_size <= 32 ? 2 : 1); }
+
+// Scalars of one integrator step, read by the kick / drift kernels from device memory.
+struct StepParams {
+    double kick_dt;      // time step of each kick
+    double hubble_plus;  // H for species +1
+    double hubble_minus; // H for species -1
+    double dtau_dt;
+    double drift_units;  // drift time step * 2^32 / L (velocity -> fixed-point displacement)
+    double pad[3];
+};
+
+// Captured step graph and what it was captured for.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    int64_t n = -1;
+    const void *pos = nullptr, *vel = nullptr, *species = nullptr, *acc = nullptr;
+    int scheme = -1;
+    hipStream_t stream = nullptr;
+};
 
 // ---------------------------------------------------------------- context
 struct Ctx {
@@ -112,14 +137,21 @@ struct Ctx {
     void *gtab = nullptr;            // real 2G: per-axis window/Gaussian factor, k_i^2
     void *pm_acc = nullptr;          // real 3n (sorted order)
     SplitPoly poly{};
-    float poly_tscale = 0.0f;    // particles: t = rho * poly_tscale - 1
+    float poly_tscale = 0.0f;    // particles: t = rho^2 * poly_tscale - 1 (erfc + Gaussian), rho * poly_tscale - 1
     float poly_tscale_m = 0.0f;  // monopoles
     double green_norm = 0.0;
     unsigned long long* host_pinned = nullptr;  // [0] interactions, [1] (n_groups | err << 32)
 
+    StepParams* dev_step = nullptr;   // device copy of the step scalars
+    StepParams* host_step = nullptr;  // pinned staging of the step scalars
+    StepGraph graph;
+    bool graph_dirty = true;          // masses or buffers changed since the capture
+
     int64_t last_n = -1;
     bool have_result = false;
-    hipEvent_t ev[8]{};
+    int last_kind = 0;                // 0 force call, 1 eager step, 2 graph step, 3 ensemble graph
+    // ev[0..6]: force phase boundaries; ev[8], ev[9]: step begin / end
+    hipEvent_t ev[10]{};
     bool events_ok = false;
     hipStream_t last_stream = nullptr;
     int64_t n_groups_max = 0;
@@ -153,6 +185,8 @@ hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s);            // pm_acc <- 
 hipError_t pm_tables(Ctx& c);                                      // twiddles + Green tables (sync)
 
 hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
+hipError_t launch_kick(Ctx& c, void* vel, const void* acc, const int8_t* species, int64_t n, hipStream_t s);
+hipError_t launch_drift(Ctx& c, uint32_t* pos, const void* vel, int64_t n, hipStream_t s);
 hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
 
 // host-side: FP32 split polynomial for the context's split mode and x range

@@ -71,7 +71,7 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, uint4* __
     geo[2 * i + 1].w = (uint32_t)last;
     if (lc >= 0) parent_int[lc] = (int32_t)i; else parent_leaf[~lc] = (int32_t)i;
     if (rc >= 0) parent_int[rc] = (int32_t)i; else parent_leaf[~rc] = (int32_t)i;
-    flags[i] = 0;
+    flags[i] = (lc < 0 ? 1 : 0) + (rc < 0 ? 1 : 0);  // arrivals still missing: 2 - this
 }
 
 struct ChildData {
@@ -109,128 +109,196 @@ __device__ inline void load_child(int32_t c, const uint4* parts, uint4* geo, Nod
     }
 }
 
-// Bottom-up pass as a sequence of frontiers, one kernel per tree level.
-// Round 0 holds the nodes whose two children are leaves; a node joins the next
-// frontier when its second child completes (relaxed arrival counter).  Kernel
-// boundaries publish each round's writes to the next, so no device-scope fence
-// is needed inside a kernel (the arrival-counter climb with a fence per level is
-// what the reference does, nbody_gpu_twopass.rs:485-560).  Each node combines its
-// children in the fixed order (left, right), so the result does not depend on
-// which round or thread produced it.
-__global__ __launch_bounds__(256) void k_bu_init(const int2* __restrict__ kids, int64_t nI, int32_t* __restrict__ cnt,
-                                                 int32_t* __restrict__ f0, int32_t* __restrict__ ctrl) {
-    __shared__ int32_t lds[6];
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nI; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        int leaves = 0;
-        if (i < nI) {
-            const int2 k = kids[i];
-            leaves = (k.x < 0 ? 1 : 0) + (k.y < 0 ? 1 : 0);
-            cnt[i] = leaves;
-        }
-        const int slot = block_append(leaves == 2, &ctrl[0], lds);
-        if (slot >= 0) f0[slot] = (int32_t)i;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_bu_round(int r, const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
-                           const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
-                           int32_t* __restrict__ cnt, NodeSum* __restrict__ sums, const int32_t* __restrict__ fin,
-                           int32_t* __restrict__ fout, int32_t* __restrict__ ctrl) {
-    __shared__ int32_t lds[6];
-    const int32_t size = ctrl[r];
-    // uniform trip count per block so that the frontier append can use block_append
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < size; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t idx = base + threadIdx.x;
-        int32_t par = -1;
-        bool ready = false;
-        const int32_t p = idx < size ? fin[idx] : -1;
-        if (p >= 0 && p < nI) {
-            const int2 k = kids[p];
-            ChildData a, b;
-            load_child(k.x, parts, geo, sums, a);
-            load_child(k.y, parts, geo, sums, b);
-            NodeSum s;
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                s.sp[ax] = a.sp[ax] + b.sp[ax];
-                s.sm[ax] = a.sm[ax] + b.sm[ax];
-            }
-            s.np = a.np + b.np;
-            s.nm = a.nm + b.nm;
-            s.pad[0] = s.pad[1] = 0;
-            sums[p] = s;
-            const uint4 g0 = geo[2 * p], g1 = geo[2 * p + 1];
-            geo[2 * p] = make_uint4(min(a.mn[0], b.mn[0]), min(a.mn[1], b.mn[1]), min(a.mn[2], b.mn[2]), g0.w);
-            geo[2 * p + 1] = make_uint4(max(a.mx[0], b.mx[0]), max(a.mx[1], b.mx[1]), max(a.mx[2], b.mx[2]), g1.w);
-            if (p == 0) {
-                ctrl[kMaxRounds + 1] = 1;  // root complete
-            } else {
-                par = parent_int[p];
-                if (par >= 0 && par < nI) ready = atomicAdd(&cnt[par], 1) == 1;
-            }
-        }
-        const int slot = block_append(ready, &ctrl[r + 1], lds);
-        if (slot >= 0) fout[slot] = par;
-    }
-}
-
 // Rounded integer mean of a fixed-point coordinate sum.
 __device__ inline uint32_t mean_fixed(uint64_t s, uint32_t cnt) {
     return (uint32_t)((s + (uint64_t)(cnt >> 1)) / (uint64_t)cnt);
 }
-
-// FP32 walk record: {com+ xyz (fixed), q+ (float bits)}, {com- xyz, q-}; q = sigma G |m| n.
-__global__ void k_mono_f32(const NodeSum* __restrict__ sums, const uint4* __restrict__ geo, int64_t nI,
-                           float qp_unit, float qm_unit, uint4* __restrict__ mono) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nI) return;
-    const NodeSum s = sums[i];
-    const uint4 g0 = geo[2 * i], g1 = geo[2 * i + 1];
-    uint32_t c[3] = {g0.x + (g1.x - g0.x) / 2, g0.y + (g1.y - g0.y) / 2, g0.z + (g1.z - g0.z) / 2};
-    uint4 a, b;
-    if (s.np) {
-        a = make_uint4(mean_fixed(s.sp[0], s.np), mean_fixed(s.sp[1], s.np), mean_fixed(s.sp[2], s.np),
-                       __float_as_uint(qp_unit * (float)s.np));
-    } else {
-        a = make_uint4(c[0], c[1], c[2], __float_as_uint(0.0f));
-    }
-    if (s.nm) {
-        b = make_uint4(mean_fixed(s.sm[0], s.nm), mean_fixed(s.sm[1], s.nm), mean_fixed(s.sm[2], s.nm),
-                       __float_as_uint(-qm_unit * (float)s.nm));
-    } else {
-        b = make_uint4(c[0], c[1], c[2], __float_as_uint(0.0f));
-    }
-    mono[2 * i] = a;
-    mono[2 * i + 1] = b;
-}
-
-// FP64 walk record: {com+ xyz (fixed units, as double), q+}, {com- xyz, q-}.
 __device__ inline double mean_fixed_f64(uint64_t s, uint32_t cnt) {
     const uint64_t q = s / cnt, r = s % cnt;
     return (double)q + (double)r / (double)cnt;
 }
-__global__ void k_mono_f64(const NodeSum* __restrict__ sums, const uint4* __restrict__ geo, int64_t nI,
-                           double qp_unit, double qm_unit, double4* __restrict__ mono) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nI) return;
-    const NodeSum s = sums[i];
-    const uint4 g0 = geo[2 * i], g1 = geo[2 * i + 1];
-    const double c[3] = {(double)g0.x + 0.5 * (double)(g1.x - g0.x), (double)g0.y + 0.5 * (double)(g1.y - g0.y),
-                         (double)g0.z + 0.5 * (double)(g1.z - g0.z)};
-    double4 a, b;
-    if (s.np)
-        a = make_double4(mean_fixed_f64(s.sp[0], s.np), mean_fixed_f64(s.sp[1], s.np),
-                         mean_fixed_f64(s.sp[2], s.np), qp_unit * (double)s.np);
-    else
-        a = make_double4(c[0], c[1], c[2], 0.0);
-    if (s.nm)
-        b = make_double4(mean_fixed_f64(s.sm[0], s.nm), mean_fixed_f64(s.sm[1], s.nm),
-                         mean_fixed_f64(s.sm[2], s.nm), -qm_unit * (double)s.nm);
-    else
-        b = make_double4(c[0], c[1], c[2], 0.0);
-    mono[2 * i] = a;
-    mono[2 * i + 1] = b;
+
+// Walk records of one node, written as soon as its sums and box are final.
+//   FP32: {com+ xyz (fixed), q+ (float bits)}, {com- xyz, q-}
+//   FP64: {com+ xyz (fixed units, as double), q+}, {com- xyz, q-}
+// q = sigma G |m| n; an empty species gets the box centre and q = 0.
+template <int PREC>
+__device__ inline void write_mono(int32_t p, const NodeSum& s, const uint32_t mn[3], const uint32_t mx[3],
+                                  void* mono, double qp, double qm) {
+    if (PREC == JTP_FP64) {
+        const double c[3] = {(double)mn[0] + 0.5 * (double)(mx[0] - mn[0]),
+                             (double)mn[1] + 0.5 * (double)(mx[1] - mn[1]),
+                             (double)mn[2] + 0.5 * (double)(mx[2] - mn[2])};
+        double4 a, b;
+        if (s.np)
+            a = make_double4(mean_fixed_f64(s.sp[0], s.np), mean_fixed_f64(s.sp[1], s.np),
+                             mean_fixed_f64(s.sp[2], s.np), qp * (double)s.np);
+        else
+            a = make_double4(c[0], c[1], c[2], 0.0);
+        if (s.nm)
+            b = make_double4(mean_fixed_f64(s.sm[0], s.nm), mean_fixed_f64(s.sm[1], s.nm),
+                             mean_fixed_f64(s.sm[2], s.nm), -qm * (double)s.nm);
+        else
+            b = make_double4(c[0], c[1], c[2], 0.0);
+        ((double4*)mono)[2 * p] = a;
+        ((double4*)mono)[2 * p + 1] = b;
+    } else {
+        const uint32_t c[3] = {mn[0] + (mx[0] - mn[0]) / 2, mn[1] + (mx[1] - mn[1]) / 2, mn[2] + (mx[2] - mn[2]) / 2};
+        const float qpf = (float)qp, qmf = (float)qm;
+        uint4 a, b;
+        if (s.np)
+            a = make_uint4(mean_fixed(s.sp[0], s.np), mean_fixed(s.sp[1], s.np), mean_fixed(s.sp[2], s.np),
+                           __float_as_uint(qpf * (float)s.np));
+        else
+            a = make_uint4(c[0], c[1], c[2], __float_as_uint(0.0f));
+        if (s.nm)
+            b = make_uint4(mean_fixed(s.sm[0], s.nm), mean_fixed(s.sm[1], s.nm), mean_fixed(s.sm[2], s.nm),
+                           __float_as_uint(-qmf * (float)s.nm));
+        else
+            b = make_uint4(c[0], c[1], c[2], __float_as_uint(0.0f));
+        ((uint4*)mono)[2 * p] = a;
+        ((uint4*)mono)[2 * p + 1] = b;
+    }
+}
+
+// Combines the two children of internal node p in the fixed order (left, right): exact integer sums,
+// bounding box (the .w words keep the node's range) and walk records.  The result does not depend on
+// which thread or pass completes the node.
+template <int PREC>
+__device__ inline void finish_node(int32_t p, const uint4* __restrict__ parts, uint4* __restrict__ geo,
+                                   const int2* __restrict__ kids, NodeSum* __restrict__ sums, void* mono,
+                                   double qp, double qm) {
+    const int2 k = kids[p];
+    ChildData a, b;
+    load_child(k.x, parts, geo, sums, a);
+    load_child(k.y, parts, geo, sums, b);
+    NodeSum s;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        s.sp[ax] = a.sp[ax] + b.sp[ax];
+        s.sm[ax] = a.sm[ax] + b.sm[ax];
+    }
+    s.np = a.np + b.np;
+    s.nm = a.nm + b.nm;
+    s.pad[0] = s.pad[1] = 0;
+    sums[p] = s;
+    const uint32_t mn[3] = {min(a.mn[0], b.mn[0]), min(a.mn[1], b.mn[1]), min(a.mn[2], b.mn[2])};
+    const uint32_t mx[3] = {max(a.mx[0], b.mx[0]), max(a.mx[1], b.mx[1]), max(a.mx[2], b.mx[2])};
+    const uint32_t first = geo[2 * p].w, last = geo[2 * p + 1].w;
+    geo[2 * p] = make_uint4(mn[0], mn[1], mn[2], first);
+    geo[2 * p + 1] = make_uint4(mx[0], mx[1], mx[2], last);
+    write_mono<PREC>(p, s, mn, mx, mono, qp, qm);
+}
+
+// Bottom-up pass in two launches, with no device-scope fence (the arrival-counter climb with a fence
+// per level is what the reference does, nbody_gpu_twopass.rs:485-560).
+//
+// 1. k_bu_local: one workgroup per chunk of kBuChunk consecutive sorted leaves.  A Karras node whose
+//    leaf range lies inside the chunk ("local") has only local descendants and its index lies in the
+//    chunk, so the workgroup completes all of its local nodes level by level with LDS frontiers and
+//    LDS arrival counters, separated by workgroup barriers.  A local node whose parent is not local
+//    bumps the parent's global arrival counter (flags[], preset by k_karras to the number of leaf
+//    children); the arrival that completes the parent appends it to the global frontier.
+// 2. k_bu_top: one workgroup finishes the remaining nodes (those whose range crosses a chunk
+//    boundary: about the chunk count times the height of the crossing paths) level by level.
+constexpr int kBuChunk = 2048;
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
+                                                  const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+                                                  int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
+                                                  double qp, double qm, int32_t* __restrict__ gfront,
+                                                  int32_t* __restrict__ ctrl) {
+    __shared__ int32_t cnt[kBuChunk];  // arrival count of a local node, -1 if the node is not local
+    __shared__ int32_t fr[2][kBuChunk];
+    __shared__ int32_t fsz[2];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kBuChunk;
+    const int64_t lastLeaf = base + kBuChunk - 1;
+    if (tid < 2) fsz[tid] = 0;
+    __syncthreads();
+    for (int k = tid; k < kBuChunk; k += blockDim.x) {
+        const int64_t i = base + k;
+        int32_t c = -1;
+        if (i < nI) {
+            // leaf children from the topology, not from flags[] (other chunks may already be arriving there)
+            const int2 kc = kids[i];
+            const int32_t leaves = (kc.x < 0 ? 1 : 0) + (kc.y < 0 ? 1 : 0);
+            const int64_t first = geo[2 * i].w, last = geo[2 * i + 1].w;
+            if (first >= base && last <= lastLeaf) {
+                c = leaves;
+                if (leaves == 2) fr[0][atomicAdd(&fsz[0], 1)] = (int32_t)i;
+            } else if (leaves == 2) {
+                gfront[atomicAdd(&ctrl[0], 1)] = (int32_t)i;
+            }
+        }
+        cnt[k] = c;
+    }
+    __syncthreads();
+    int cur = 0;
+    while (true) {
+        const int size = fsz[cur];
+        if (size == 0) break;  // block-uniform: every thread read fsz after the same barrier
+        __syncthreads();       // everyone has read fsz[cur] before it can be reused
+        if (tid == 0) fsz[cur ^ 1] = 0;
+        __syncthreads();
+        for (int k = tid; k < size; k += blockDim.x) {
+            const int32_t p = fr[cur][k];
+            finish_node<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
+            if (p == 0) {
+                ctrl[kMaxRounds + 1] = 1;  // root complete
+                continue;
+            }
+            const int32_t par = parent_int[p];
+            if (par < 0 || par >= nI) continue;
+            const int64_t lp = (int64_t)par - base;
+            if (lp >= 0 && lp < kBuChunk && cnt[lp] >= 0) {
+                if (atomicAdd(&cnt[lp], 1) == 1) fr[cur ^ 1][atomicAdd(&fsz[cur ^ 1], 1)] = par;
+            } else if (atomicAdd(&flags[par], 1) == 1) {
+                gfront[atomicAdd(&ctrl[0], 1)] = par;
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(1024) void k_bu_top(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
+                                                 const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+                                                 int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
+                                                 double qp, double qm, int32_t* __restrict__ f0,
+                                                 int32_t* __restrict__ f1, int32_t* __restrict__ ctrl) {
+    __shared__ int32_t fsz[2];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        fsz[0] = ctrl[0];
+        fsz[1] = 0;
+    }
+    __syncthreads();
+    int cur = 0;
+    // Every node enters a frontier at most once, so the loop ends after at most nI passes.
+    while (true) {
+        const int size = fsz[cur];
+        if (size == 0) break;
+        int32_t* fin = cur ? f1 : f0;
+        int32_t* fout = cur ? f0 : f1;
+        __syncthreads();
+        if (tid == 0) fsz[cur ^ 1] = 0;
+        __syncthreads();
+        for (int k = tid; k < size; k += blockDim.x) {
+            const int32_t p = fin[k];
+            finish_node<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
+            if (p == 0) {
+                ctrl[kMaxRounds + 1] = 1;
+                continue;
+            }
+            const int32_t par = parent_int[p];
+            if (par >= 0 && par < nI && atomicAdd(&flags[par], 1) == 1) fout[atomicAdd(&fsz[cur ^ 1], 1)] = par;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
 }
 
 // ------------------------------------------------------------------ cell offsets
@@ -318,33 +386,31 @@ __global__ void k_groups(const int32_t* __restrict__ cell4, const int32_t* __res
 }
 
 hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
-    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
     if (n >= 2) {
         const int64_t nI = n - 1;
         const unsigned nbI = (unsigned)((nI + kBlock - 1) / kBlock);
-        launch_fill_i32(c.parent_int, nI, -1, s);
-        launch_fill_i32(c.parent_leaf, n, -1, s);
+        hipError_t e = launch_fill_i32(c.parent_int, nI, -1, s);
+        if (e == hipSuccess) e = launch_fill_i32(c.parent_leaf, n, -1, s);
+        if (e != hipSuccess) return e;
         k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, c.node_geo, c.node_kids, c.parent_int,
                                                     c.parent_leaf, c.flags);
         int32_t* ctrl = c.dev_counters + kCtrRounds;
-        launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
-        const unsigned nbInit = nbI < 4096 ? nbI : 4096;
-        k_bu_init<<<dim3(nbInit), dim3(kBlock), 0, s>>>(c.node_kids, nI, c.flags, c.frontier0, ctrl);
-        int blocks = (int)((nI / 4 + kBlock - 1) / kBlock);
-        blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
-        for (int r = 0; r < kMaxRounds; ++r) {
-            int32_t* fin = (r & 1) ? c.frontier1 : c.frontier0;
-            int32_t* fout = (r & 1) ? c.frontier0 : c.frontier1;
-            k_bu_round<<<dim3(blocks), dim3(kBlock), 0, s>>>(r, c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
-                                                             c.flags, (NodeSum*)c.node_sum, fin, fout, ctrl);
-        }
+        e = launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
+        if (e != hipSuccess) return e;
+        const unsigned nbLocal = (unsigned)((n + kBuChunk - 1) / kBuChunk);
         const double qp = c.p.g_constant * c.m_plus, qm = c.p.g_constant * c.m_minus;
-        if (c.p.precision == JTP_FP64)
-            k_mono_f64<<<dim3(nbI), dim3(kBlock), 0, s>>>((const NodeSum*)c.node_sum, c.node_geo, nI, qp, qm,
-                                                          (double4*)c.node_mono);
-        else
-            k_mono_f32<<<dim3(nbI), dim3(kBlock), 0, s>>>((const NodeSum*)c.node_sum, c.node_geo, nI, (float)qp,
-                                                          (float)qm, (uint4*)c.node_mono);
+        NodeSum* sums = (NodeSum*)c.node_sum;
+        if (c.p.precision == JTP_FP64) {
+            k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+                                                                     c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
+            k_bu_top<JTP_FP64><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
+                                                              sums, c.node_mono, qp, qm, c.frontier0, c.frontier1, ctrl);
+        } else {
+            k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+                                                                     c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
+            k_bu_top<JTP_FP32><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
+                                                              sums, c.node_mono, qp, qm, c.frontier0, c.frontier1, ctrl);
+        }
     }
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * kGroupLevelBits, kNCell4, c.cell4, s);
     if (e != hipSuccess) return e;

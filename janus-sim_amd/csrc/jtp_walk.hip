@@ -52,7 +52,7 @@ struct WalkArgs {
     double inv2rs;  // 1 / (2 r_s)
     double qp, qm;  // signed particle charges sigma G |m| for species +1 / -1
     SplitPoly poly;
-    float tscale;    // particles: t = rho * tscale - 1
+    float tscale;    // particles: t = rho^2 * tscale - 1 (erfc + Gaussian) or rho * tscale - 1
     float tscale_m;  // monopoles
     float escale;    // exp(-x^2) = exp2(r2s * escale)
 };
@@ -63,45 +63,59 @@ template <typename R> struct EntryT;
 template <> struct EntryT<float> { using T = float4; };
 template <> struct EntryT<double> { using T = double4; };
 
-// Pair evaluation: returns the force factor f such that a += f * d.  CUT applies the hard
-// cutoff r < r_cut (particle entries, and monopoles under the x^4 law whose weight turns
-// negative beyond r_cut); monopoles under the erfc laws keep the smooth tail of S.
+// Pair evaluation: returns the force factor f such that a += f * d for the displacement d.
+// CUT applies the hard cutoff r < r_cut (particle entries, and monopoles under the x^4 law whose
+// weight turns negative beyond r_cut); monopoles under the erfc laws keep the smooth tail of S.
+// rcut2s = r_cut^2 + eps^2: the cut tested on rho^2 = r^2 + eps^2.
 template <typename R, int MODE, bool CUT>
 struct Pair;
 
+template <int DEG>
+__device__ inline float horner(const float* c, float t) {
+    float P = c[DEG];
+#pragma unroll
+    for (int d = DEG - 1; d >= 0; --d) P = __builtin_fmaf(P, t, c[d]);
+    return P;
+}
+
 template <int MODE, bool CUT>
 struct Pair<float, MODE, CUT> {
-    __device__ static inline float f(float q, float r2, const WalkArgs& a, float eps2, float rcut2) {
-        const float r2s = r2 + eps2;
+    __device__ static inline float f(float q, float dx, float dy, float dz, const WalkArgs& a, float eps2,
+                                     float rcut2, float rcut2s) {
+        if (MODE == JTP_SPLIT_POLY4) {
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float inv = __builtin_amdgcn_rsqf(r2 + eps2);
+            const float u = r2 * (1.0f / rcut2);
+            const float fv = q * (1.0f - u * u) * (inv * inv * inv);
+            return (!CUT || r2 < rcut2) ? fv : 0.0f;
+        }
+        const float r2s = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2s);
         const float inv3 = inv * inv * inv;
-        float S;
-        if (MODE == JTP_SPLIT_POLY4) {
-            const float u = r2 * (1.0f / rcut2);
-            S = 1.0f - u * u;
+        float fv;
+        if (CUT && MODE == JTP_SPLIT_ERFC_GAUSS) {
+            // particle pair: S / rho^3 = 1 / rho^3 + H(rho^2)
+            const float t = __builtin_fmaf(r2s, a.tscale, -1.0f);
+            fv = q * (inv3 + horner<kPolyDegPair>(a.poly.cp, t));
         } else if (CUT) {
-            // particle pair: rho <= sqrt(r_cut^2 + eps^2), direct interpolant of S
+            // particle pair, erfc law: direct interpolant of S(x)
             const float t = r2s * inv * a.tscale - 1.0f;
-            float P = a.poly.cp[kPolyDeg];
-#pragma unroll
-            for (int d = kPolyDeg - 1; d >= 0; --d) P = P * t + a.poly.cp[d];
-            S = P;
+            fv = q * horner<kPolyDeg>(a.poly.cp, t) * inv3;
         } else {
             // monopole: exp(-x^2) times the interpolant of the scaled factor (bounded beyond its range)
             const float t = fminf(r2s * inv * a.tscale_m - 1.0f, 1.0f);
-            float P = a.poly.cm[kPolyDeg];
-#pragma unroll
-            for (int d = kPolyDeg - 1; d >= 0; --d) P = P * t + a.poly.cm[d];
-            S = __builtin_amdgcn_exp2f(r2s * a.escale) * P;
+            constexpr int DM = MODE == JTP_SPLIT_ERFC_GAUSS ? kPolyDegMono : kPolyDeg;
+            fv = q * (__builtin_amdgcn_exp2f(r2s * a.escale) * horner<DM>(a.poly.cm, t)) * inv3;
         }
-        const float fv = q * S * inv3;
-        return (!CUT || r2 < rcut2) ? fv : 0.0f;
+        return (!CUT || r2s < rcut2s) ? fv : 0.0f;
     }
 };
 
 template <int MODE, bool CUT>
 struct Pair<double, MODE, CUT> {
-    __device__ static inline double f(double q, double r2, const WalkArgs& a, double eps2, double rcut2) {
+    __device__ static inline double f(double q, double dx, double dy, double dz, const WalkArgs& a, double eps2,
+                                      double rcut2, double) {
+        const double r2 = dx * dx + dy * dy + dz * dz;
         const double r2s = r2 + eps2;
         const double inv = rsqrt(r2s);
         const double inv3 = inv * inv * inv;
@@ -159,6 +173,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     const M* mono = (const M*)a.mono;
     const R scale = (R)a.scale;
     const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2, thetac2 = (R)a.theta_cut2;
+    const R rcut2s = (R)(a.rcut2 + a.eps2);
     const R qpart[2] = {(R)a.qm, (R)a.qp};
     const int maxapp = a.leaf > 2 ? a.leaf : 2;
     const uint64_t lt = lanemask_lt();
@@ -235,8 +250,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             for (int jj = 0; jj < itp; ++jj) {
                 const E e = L[jj * LPT + sub];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                const R f = Pair<R, MODE, true>::f(e.w, r2, a, eps2, rcut2);
+                const R f = Pair<R, MODE, true>::f(e.w, dx, dy, dz, a, eps2, rcut2, rcut2s);
                 px += f * dx;
                 py += f * dy;
                 pz += f * dz;
@@ -246,8 +260,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             for (int jj = 0; jj < itm; ++jj) {
                 const E e = L[m0 + jj * LPT];
                 const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                const R f = Pair<R, MODE, MODE == JTP_SPLIT_POLY4>::f(e.w, r2, a, eps2, rcut2);
+                const R f = Pair<R, MODE, MODE == JTP_SPLIT_POLY4>::f(e.w, dx, dy, dz, a, eps2, rcut2, rcut2s);
                 px += f * dx;
                 py += f * dy;
                 pz += f * dz;

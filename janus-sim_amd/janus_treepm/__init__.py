@@ -7,7 +7,8 @@ ctypes and mirrors the reference's Rust-side interface for this path:
                          particle arrays in host memory, FP64 results.
 * ``PoissonSolver``    - cufft_ffi.rs:71-354 (CuFFTPoisson / solve_device), device grids.
 * ``ForceContext``     - the device-pointer force call of GpuNBodyTwoPass
-                         (nbody_gpu_twopass.rs:3961-4138, 4486-4649) without the integrator.
+                         (nbody_gpu_twopass.rs:3961-4138) and its integrator step
+                         step_treepm_gpu (:4486-4649) as ``step_device``.
 
 Errors from the library raise ``JtpError`` (the reference maps a -1 status to
 ``Err(String)``, cufft_ffi.rs:97-99).  There is no CPU fallback: if the shared
@@ -31,13 +32,17 @@ GRAD_4PT, GRAD_2PT = 0, 1
 STAGE_KEYS, STAGE_PERM, STAGE_NODES, STAGE_MONO, STAGE_RHO, STAGE_PHI, STAGE_ACC_PM, STAGE_SUMS, STAGE_BOXES = (
     1, 2, 3, 4, 5, 6, 7, 8, 9)
 STAGE_RHO2, STAGE_PHI2 = 10, 11
+STEP_DKD, STEP_KDK, STEP_GRAPH = 0, 1, 0x100
+ENS_SEQUENTIAL, ENS_GRAPH = 0x1, 0x100
 
 # every symbol declared in include/janus_treepm.h
 EXPORTED = [
     "jtp_default_params", "jtp_create", "jtp_destroy", "jtp_last_error", "jtp_version", "jtp_set_species_mass",
     "jtp_compute_forces", "jtp_pos_to_fixed_f64", "jtp_pos_to_fixed_f32", "jtp_pm_solve", "jtp_morton_keys_f32",
     "jtp_get_stats", "jtp_stage_size", "jtp_stage_copy", "jtp_alloc", "jtp_free", "jtp_copy_h2d",
-    "jtp_copy_d2h", "jtp_copy_d2d", "jtp_device_synchronize", "jtp_device_count",
+    "jtp_copy_d2h", "jtp_copy_d2d", "jtp_device_synchronize", "jtp_device_count", "jtp_step",
+    "jtp_ensemble_create", "jtp_ensemble_destroy", "jtp_ensemble_last_error", "jtp_ensemble_compute_forces",
+    "jtp_ensemble_get_stats",
 ]
 
 
@@ -104,6 +109,14 @@ def load_library(path: str = LIB_PATH):
         "jtp_copy_d2d": (C.c_int, [vp, vp, sz]),
         "jtp_device_synchronize": (C.c_int, []),
         "jtp_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "jtp_step": (C.c_int, [vp, i64, vp, vp, vp, vp, i32, dbl, dbl, dbl, dbl, vp]),
+        "jtp_ensemble_create": (C.c_int, [i32, C.POINTER(Params), C.POINTER(dbl), C.POINTER(dbl), i32,
+                                          C.POINTER(vp)]),
+        "jtp_ensemble_destroy": (C.c_int, [vp]),
+        "jtp_ensemble_last_error": (C.c_char_p, [vp]),
+        "jtp_ensemble_compute_forces": (C.c_int, [vp, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                                  i32, vp]),
+        "jtp_ensemble_get_stats": (C.c_int, [vp, i32, C.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -202,6 +215,17 @@ class ForceContext:
         ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
         self._check(self.lib.jtp_compute_forces(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), ptr(d_acc), stream))
 
+    def step_device(self, n: int, d_pos_fixed, d_vel, d_species, d_acc, dt: float, hubble_plus: float = 0.0,
+                    hubble_minus: float | None = None, dtau_dt: float = 1.0, scheme: int = STEP_DKD,
+                    graph: bool = False, stream=None):
+        """One integrator step in place on device arrays (jtp_step): DKD is step_treepm_gpu's order,
+        KDK needs d_acc = a(t) on entry.  hubble_minus defaults to hubble_plus (the reference's one H)."""
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        hm = hubble_plus if hubble_minus is None else hubble_minus
+        flags = int(scheme) | (STEP_GRAPH if graph else 0)
+        self._check(self.lib.jtp_step(self.h, int(n), ptr(d_pos_fixed), ptr(d_vel), ptr(d_species), ptr(d_acc),
+                                      flags, float(dt), float(hubble_plus), float(hm), float(dtau_dt), stream))
+
     def compute_forces(self, pos_fixed: np.ndarray, species: np.ndarray) -> np.ndarray:
         """Host convenience: upload, compute, download (float32 or float64 [n,3])."""
         n = len(species)
@@ -244,6 +268,52 @@ class ForceContext:
     def close(self):
         if getattr(self, "h", None):
             self.lib.jtp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """A batch of independent boxes (jtp_ensemble_*): one parameter set and species masses per box."""
+
+    def __init__(self, params: list, m_plus: list, m_minus_abs: list, n_streams: int = 4):
+        self.lib = load_library()
+        nb = len(params)
+        arr = (Params * nb)(*params)
+        mp = (C.c_double * nb)(*[float(x) for x in m_plus])
+        mm = (C.c_double * nb)(*[float(x) for x in m_minus_abs])
+        h = C.c_void_p()
+        rc = self.lib.jtp_ensemble_create(nb, arr, mp, mm, int(n_streams), C.byref(h))
+        if rc != JTP_OK:
+            raise JtpError(rc, "jtp_ensemble_create failed")
+        self.h, self.n_boxes = h, nb
+
+    def _check(self, rc):
+        if rc != JTP_OK:
+            raise JtpError(rc, self.lib.jtp_ensemble_last_error(self.h).decode())
+
+    def compute_forces_device(self, n, d_pos, d_species, d_acc, sequential=False, graph=False, stream=None):
+        nb = self.n_boxes
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        nn = (C.c_int64 * nb)(*[int(x) for x in n])
+        pp = (C.c_void_p * nb)(*[ptr(d) for d in d_pos])
+        ss = (C.c_void_p * nb)(*[ptr(d) for d in d_species])
+        aa = (C.c_void_p * nb)(*[ptr(d) for d in d_acc])
+        flags = (ENS_SEQUENTIAL if sequential else 0) | (ENS_GRAPH if graph else 0)
+        self._check(self.lib.jtp_ensemble_compute_forces(self.h, nn, pp, ss, aa, flags, stream))
+
+    def stats(self, box: int = -1) -> dict:
+        st = Stats()
+        self._check(self.lib.jtp_ensemble_get_stats(self.h, int(box), C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.jtp_ensemble_destroy(self.h)
             self.h = None
 
     def __del__(self):

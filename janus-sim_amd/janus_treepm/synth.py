@@ -37,7 +37,7 @@ class Workload:
     softening: float
     m_plus: float
     m_minus_abs: float
-    leaf_size: int = 8
+    leaf_size: int = 4
     extra: dict = field(default_factory=dict)
 
     @property
@@ -89,7 +89,7 @@ def zeldovich_displacement(n: int, box_size: float, seed: int, index: float = -2
 
 def lattice_zeldovich(n_side: int = 64, box_size: float = 250.0, n_cells: int = 64, seed: int = 42,
                       m_plus: float = 1.0, m_minus_abs: float = 19.0, theta: float = 0.5,
-                      leaf_size: int = 8, name: str = "lattice_zeldovich") -> Workload:
+                      leaf_size: int = 4, name: str = "lattice_zeldovich") -> Workload:
     h = box_size / n_side
     psi = zeldovich_displacement(n_side, box_size, seed)
     g = (np.arange(n_side) + 0.5) * h - box_size / 2
@@ -131,7 +131,7 @@ def _cic_density(pos, box_size, G):
 
 def clustered(n_total: int = 22_000_000, box_size: float = 250.0, n_cells: int = 256, n_halos: int = 2000,
               seed: int = 7, m_plus: float = 1.0, m_minus_abs: float = 19.0, theta: float = 0.5,
-              halo_fraction: float = 0.6, leaf_size: int = 8, name: str = "clustered") -> Workload:
+              halo_fraction: float = 0.6, leaf_size: int = 4, name: str = "clustered") -> Workload:
     rng = np.random.Generator(np.random.PCG64(seed))
     L = box_size
     n_plus = n_total // 2
@@ -188,3 +188,24 @@ def clustered(n_total: int = 22_000_000, box_size: float = 250.0, n_cells: int =
     r_s = 1.25 * cell
     return Workload(name, pos, sp, L, n_cells, r_s, 4.5 * r_s, theta, default_softening(L, n_total),
                     m_plus, m_minus_abs, leaf_size, extra={"n_halos": n_halos, "median_smoothed": float(med)})
+
+
+def ensemble_c3(n_boxes: int = 48, seed: int = 1000) -> list:
+    """Uneven ensemble of lattice + Zel'dovich two-species boxes: N_b = 2 n^3 with n = round(2^u),
+    u ~ U[5, 7] (N log2-uniform in [2*32^3, 2*128^3]); G = 2^round(log2 N^(1/3)) in {32, 64, 128};
+    L ~ U[100, 500]; |m-| / m+ ~ U[5, 40] with m+ = 1; box b uses the lattice recipe with seed + b.
+    The per-box draws come from PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for b in range(n_boxes):
+        u = rng.uniform(5.0, 7.0)
+        L = float(rng.uniform(100.0, 500.0))
+        ratio = float(rng.uniform(5.0, 40.0))
+        n = int(round(2.0 ** u))
+        N = 2 * n ** 3
+        G = int(2 ** int(round(np.log2(N ** (1.0 / 3.0)))))
+        w = lattice_zeldovich(n_side=n, box_size=L, n_cells=G, seed=seed + b, m_plus=1.0, m_minus_abs=ratio,
+                              name=f"c3_box{b}")
+        out.append(w)
+    return out
+
