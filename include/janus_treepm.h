@@ -151,6 +151,21 @@ int  jtp_step(jtp_ctx* ctx, int64_t n, uint32_t* d_pos_fixed, void* d_vel, const
               void* d_acc, int32_t flags, double dt, double hubble_plus, double hubble_minus,
               double dtau_dt, void* stream);
 
+/* ---- FP64 diagnostics on the device (reference: kinetic_energy and the grid potential-energy sum,
+ * nbody_gpu_twopass.rs:5071-5077, 5293-5307, which copy to the host and sum there).
+ * Deterministic reductions (fixed order) in FP64 of d_vel (float or double per the precision):
+ * kinetic[s] = sum |m| v^2 / 2 and momentum[s] = sum |m| v per species (s = 0: +1, 1: -1;
+ * total sum_s momentum[s] is conserved by the Janus pair law); pm_energy = 1/2 sum rho phi over the
+ * mesh of the last force evaluation (mean of the two interlaced grids), the long-range potential
+ * energy including its self-energy part. */
+typedef struct jtp_diag {
+    double kinetic[2];
+    double momentum[2][3];
+    double pm_energy;
+} jtp_diag;
+int  jtp_diagnostics(jtp_ctx* ctx, int64_t n, const void* d_vel, const int8_t* d_species, jtp_diag* out,
+                     void* stream);
+
 /* ---- ensembles of independent boxes (no counterpart in the reference, which runs one box per
  * process; the batch replaces a loop of step_treepm_gpu / TreePMForce calls over boxes).
  * Each box has its own parameters (L, G, r_s, ...) and species masses and its own buffers.

@@ -277,6 +277,8 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc_bytes(c.phi, G3 * rs);
     if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.rho2, G3 * rs);
     if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.phi2, G3 * rs);
+    if (e == hipSuccess) e = dalloc_bytes(c.grad, G3 * 4 * rs);
+    if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.grad2, G3 * 4 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.spec, (size_t)G * G * c.Hc() * 2 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.twiddle, (size_t)2 * G * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)4 * G * rs);
@@ -287,6 +289,7 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_pinned, 4 * sizeof(unsigned long long), 0);
     if (e == hipSuccess) e = dalloc(c.dev_step, 1);
+    if (e == hipSuccess) e = dalloc(c.diag_scratch, kDiagScratch);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_step, sizeof(StepParams), 0);
     if (e == hipSuccess) e = pm_tables(c);
     if (e == hipSuccess) {
@@ -311,8 +314,8 @@ int jtp_destroy(jtp_ctx* h) {
     free_particle_buffers(c);
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     if (c.graph.capture_stream) (void)hipStreamDestroy(c.graph.capture_stream);
-    void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.spec, c.twiddle, c.gtab, c.cell_start, c.cell4,
-                    c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step};
+    void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.grad, c.grad2, c.spec, c.twiddle, c.gtab, c.cell_start,
+                    c.cell4, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);
@@ -520,6 +523,27 @@ int jtp_step(jtp_ctx* h, int64_t n, uint32_t* d_pos, void* d_vel, const int8_t* 
     JTP_HIP(&c, hipEventRecord(c.ev[9], s));
     JTP_HIP(&c, hipStreamSynchronize(s));
     return check_status(c, n);
+}
+
+int jtp_diagnostics(jtp_ctx* h, int64_t n, const void* d_vel, const int8_t* d_species, jtp_diag* out,
+                    void* stream) {
+    if (!h || !out) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    if (n < 0 || (n > 0 && (!d_vel || !d_species))) return fail(&c, JTP_ERR_ARG, "bad diagnostics arguments");
+    hipStream_t s = (hipStream_t)stream;
+    double* res = c.diag_scratch + 1024 * 8;
+    JTP_HIP(&c, launch_diagnostics(c, n, d_vel, d_species, c.diag_scratch, res, s));
+    double host[10];
+    JTP_HIP(&c, hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipStreamSynchronize(s));
+    out->kinetic[0] = host[0];
+    out->kinetic[1] = host[1];
+    for (int k = 0; k < 3; ++k) {
+        out->momentum[0][k] = host[2 + k];
+        out->momentum[1][k] = host[5 + k];
+    }
+    out->pm_energy = c.p.interlace ? 0.25 * (host[8] + host[9]) : 0.5 * host[8];
+    return JTP_OK;
 }
 
 int jtp_pos_to_fixed_f64(jtp_ctx* h, int64_t n, const double* d_pos, uint32_t* d_fixed, void* stream) {

@@ -132,6 +132,7 @@ struct Ctx {
     void *rho = nullptr;             // real G^3 deposited mass per cell
     void *phi = nullptr;             // real G^3 potential
     void *rho2 = nullptr, *phi2 = nullptr;  // interlaced grid (nodes at the cell centres of the first)
+    void *grad = nullptr, *grad2 = nullptr;  // real4 [G^3] stencil gradient of phi / phi2 (w unused)
     void *spec = nullptr;            // complex G^2 (G/2+1)
     void *twiddle = nullptr;         // complex G  e^{-2 pi i m/G}
     void *gtab = nullptr;            // real 2G: per-axis window/Gaussian factor, k_i^2
@@ -142,6 +143,7 @@ struct Ctx {
     double green_norm = 0.0;
     unsigned long long* host_pinned = nullptr;  // [0] interactions, [1] (n_groups | err << 32)
 
+    double* diag_scratch = nullptr;   // FP64 reduction partials (kDiagBlocks x 8) + 10 results
     StepParams* dev_step = nullptr;   // device copy of the step scalars
     StepParams* host_step = nullptr;  // pinned staging of the step scalars
     StepGraph graph;
@@ -187,6 +189,9 @@ hipError_t pm_tables(Ctx& c);                                      // twiddles +
 hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
 hipError_t launch_kick(Ctx& c, void* vel, const void* acc, const int8_t* species, int64_t n, hipStream_t s);
 hipError_t launch_drift(Ctx& c, uint32_t* pos, const void* vel, int64_t n, hipStream_t s);
+hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* species, double* scratch,
+                              double* out, hipStream_t s);
+constexpr int kDiagScratch = 1024 * 8 + 16;  // doubles
 hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
 
 // host-side: FP32 split polynomial for the context's split mode and x range

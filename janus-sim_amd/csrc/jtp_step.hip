@@ -61,3 +61,108 @@ hipError_t launch_drift(Ctx& c, uint32_t* pos, const void* vel, int64_t n, hipSt
 }
 
 }  // namespace jtp
+
+// ------------------------------------------------------------------ FP64 diagnostics
+// Deterministic two-level reductions (fixed grid, fixed in-block order) of
+//   velocities: per species kinetic energy sum |m| v^2 / 2 and momentum sum |m| v;
+//   grid: sum rho * phi over the nodes (twice the long-range potential energy of the grid).
+// The reference copies velocities and grids to the host for these sums
+// (nbody_gpu_twopass.rs:5071-5077, 5293-5307).
+namespace jtp {
+
+constexpr int kDiagBlocks = 1024;
+constexpr int kDiagVals = 8;  // ke+, ke-, p+ xyz, p- xyz
+
+template <int NV>
+__device__ inline void block_reduce_store(double (&v)[NV], double* __restrict__ out) {
+    __shared__ double lds[NV][256];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) lds[k][threadIdx.x] += lds[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < NV) out[threadIdx.x] = lds[threadIdx.x][0];
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_diag_vel(const V* __restrict__ vel, const int8_t* __restrict__ species,
+                                                  int64_t n, double mp, double mm, double* __restrict__ partial) {
+    double v[kDiagVals] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool plus = species[i] > 0;
+        const double m = plus ? mp : mm;
+        const double vx = (double)vel[3 * i], vy = (double)vel[3 * i + 1], vz = (double)vel[3 * i + 2];
+        const double ke = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        const int o = plus ? 0 : 1;
+        v[o] += ke;
+        v[2 + 3 * o] += m * vx;
+        v[3 + 3 * o] += m * vy;
+        v[4 + 3 * o] += m * vz;
+    }
+    block_reduce_store<kDiagVals>(v, partial + (int64_t)blockIdx.x * kDiagVals);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_diag_grid(const R* __restrict__ rho, const R* __restrict__ phi, int64_t m,
+                                                   double* __restrict__ partial) {
+    double v[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        v[0] += (double)rho[i] * (double)phi[i];
+    block_reduce_store<1>(v, partial + blockIdx.x);
+}
+
+// Sums `nb` partial vectors of `nv` values in block order (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ partial, int nb, int nv,
+                                                    double* __restrict__ out) {
+    __shared__ double lds[256];
+    for (int k = 0; k < nv; ++k) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += 256) s += partial[(int64_t)b * nv + k];
+        lds[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = lds[0];
+        __syncthreads();
+    }
+}
+
+// out[0..7] = ke+, ke-, p+ xyz, p- xyz; out[8] = sum rho phi of grid 1, out[9] of grid 2 (0 if absent).
+hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* species, double* scratch,
+                              double* out, hipStream_t s) {
+    double* part = scratch;  // kDiagBlocks * kDiagVals doubles
+    if (n > 0) {
+        if (c.p.precision == JTP_FP64)
+            k_diag_vel<double><<<dim3(kDiagBlocks), dim3(256), 0, s>>>((const double*)vel, species, n, c.m_plus,
+                                                                       c.m_minus, part);
+        else
+            k_diag_vel<float><<<dim3(kDiagBlocks), dim3(256), 0, s>>>((const float*)vel, species, n, c.m_plus,
+                                                                      c.m_minus, part);
+        k_diag_final<<<dim3(1), dim3(256), 0, s>>>(part, kDiagBlocks, kDiagVals, out);
+    } else {
+        launch_fill_i32((int32_t*)out, 2 * kDiagVals, 0, s);
+    }
+    const void* grids[2][2] = {{c.rho, c.phi}, {c.rho2, c.phi2}};
+    for (int g = 0; g < 2; ++g) {
+        if (!c.p.pm_enabled || (g == 1 && !c.p.interlace)) {
+            launch_fill_i32((int32_t*)(out + kDiagVals + g), 2, 0, s);
+            continue;
+        }
+        if (c.p.precision == JTP_FP64)
+            k_diag_grid<double><<<dim3(kDiagBlocks), dim3(256), 0, s>>>((const double*)grids[g][0],
+                                                                        (const double*)grids[g][1], c.G3(), part);
+        else
+            k_diag_grid<float><<<dim3(kDiagBlocks), dim3(256), 0, s>>>((const float*)grids[g][0],
+                                                                       (const float*)grids[g][1], c.G3(), part);
+        k_diag_final<<<dim3(1), dim3(256), 0, s>>>(part, kDiagBlocks, 1, out + kDiagVals + g);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace jtp

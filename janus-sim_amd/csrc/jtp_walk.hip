@@ -225,6 +225,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
 
         int sp = 1, lp = 0, lm = 0;
+        int mono_inside = 0;  // monopole entries placed on the particle side since the last flush
         int64_t steps = 0;
         if (lane == 0) S[0] = a.root;
         wave_lds_fence();
@@ -283,7 +284,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 sz += pz;
             }
             my_inter += (unsigned long long)(np_ + nm_) * (unsigned long long)nvalid;
-            my_mono += (unsigned long long)nm_ * (unsigned long long)nvalid;
+            my_mono += (unsigned long long)(nm_ + mono_inside) * (unsigned long long)nvalid;
+            mono_inside = 0;
         };
 
         while (sp > 0) {
@@ -301,7 +303,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const int32_t node = has ? S[sp + lane] : 0;
             wave_lds_fence();
 
-            int act = 0, napp = 0, napm = 0;
+            // act: 1 accept the monopoles (tail form), 4 accept them as cut pairs (node entirely within
+            // r_cut of the group box, so the cut never applies and the cheaper pair law is exact),
+            // 2 take the bucket's particles, 3 open
+            int act = 0, napp = 0, napm = 0, nmono_p = 0;
             int32_t first = 0;
             if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
                 if (lane == 0) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skip it
@@ -340,10 +345,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     ext = er > ext ? er : ext;
                 }
                 if (gap2 < rcut2) {
-                    const R th2 = far2 < rcut2 ? theta2 : thetac2;
+                    const bool inside = far2 < rcut2;
+                    const R th2 = inside ? theta2 : thetac2;
                     if (node >= 0 && ext * ext < th2 * gap2) {
-                        act = 1;
-                        napm = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
+                        const int nq = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
+                        if (inside && MODE != JTP_SPLIT_POLY4) {
+                            act = 4;
+                            napp = nq;
+                            nmono_p = nq;
+                        } else {
+                            act = 1;
+                            napm = nq;
+                        }
                     } else if (cnt <= a.leaf) {
                         act = 2;
                         napp = cnt;
@@ -361,6 +374,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             if (act == 1) {
                 int o = kListCap - 1 - offm;
                 if (mono_q(mono, node, 0) != (R)0) L[o--] = mono_entry(mono, node, 0, gc, scale);
+                if (mono_q(mono, node, 1) != (R)0) L[o] = mono_entry(mono, node, 1, gc, scale);
+            } else if (act == 4) {
+                int o = offp;
+                if (mono_q(mono, node, 0) != (R)0) L[o++] = mono_entry(mono, node, 0, gc, scale);
                 if (mono_q(mono, node, 1) != (R)0) L[o] = mono_entry(mono, node, 1, gc, scale);
             } else if (act == 2) {
                 for (int t = 0; t < napp; ++t) {
@@ -387,6 +404,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
             lp += totl & 0xffff;
             lm += totl >> 16;
+            mono_inside += wave_sum(nmono_p);
             wave_lds_fence();
             if (lp + lm >= kFlushAt) {
                 flush(lp, lm);

@@ -42,7 +42,7 @@ EXPORTED = [
     "jtp_get_stats", "jtp_stage_size", "jtp_stage_copy", "jtp_alloc", "jtp_free", "jtp_copy_h2d",
     "jtp_copy_d2h", "jtp_copy_d2d", "jtp_device_synchronize", "jtp_device_count", "jtp_step",
     "jtp_ensemble_create", "jtp_ensemble_destroy", "jtp_ensemble_last_error", "jtp_ensemble_compute_forces",
-    "jtp_ensemble_get_stats",
+    "jtp_ensemble_get_stats", "jtp_diagnostics",
 ]
 
 
@@ -61,6 +61,10 @@ class Params(C.Structure):
         ("deconvolve", C.c_int32), ("pm_enabled", C.c_int32), ("tree_enabled", C.c_int32),
         ("interlace", C.c_int32), ("theta_cut", C.c_double),
     ]
+
+
+class Diag(C.Structure):
+    _fields_ = [("kinetic", C.c_double * 2), ("momentum", (C.c_double * 3) * 2), ("pm_energy", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -117,6 +121,7 @@ def load_library(path: str = LIB_PATH):
         "jtp_ensemble_compute_forces": (C.c_int, [vp, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                                   i32, vp]),
         "jtp_ensemble_get_stats": (C.c_int, [vp, i32, C.POINTER(Stats)]),
+        "jtp_diagnostics": (C.c_int, [vp, i64, vp, vp, C.POINTER(Diag), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -225,6 +230,14 @@ class ForceContext:
         flags = int(scheme) | (STEP_GRAPH if graph else 0)
         self._check(self.lib.jtp_step(self.h, int(n), ptr(d_pos_fixed), ptr(d_vel), ptr(d_species), ptr(d_acc),
                                       flags, float(dt), float(hubble_plus), float(hm), float(dtau_dt), stream))
+
+    def diagnostics(self, n: int, d_vel, d_species, stream=None) -> dict:
+        """FP64 device reductions: kinetic energy and momentum per species, mesh potential energy."""
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        d = Diag()
+        self._check(self.lib.jtp_diagnostics(self.h, int(n), ptr(d_vel), ptr(d_species), C.byref(d), stream))
+        return {"kinetic": np.array(d.kinetic[:]), "momentum": np.array([list(r) for r in d.momentum]),
+                "pm_energy": d.pm_energy}
 
     def compute_forces(self, pos_fixed: np.ndarray, species: np.ndarray) -> np.ndarray:
         """Host convenience: upload, compute, download (float32 or float64 [n,3])."""

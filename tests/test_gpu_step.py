@@ -199,3 +199,31 @@ def test_kdk_reversibility_and_momentum(jtp):
     print("reversibility: max |dx| =", np.abs(diff).max() * L / TWO32)
     assert np.abs(diff).max() * L / TWO32 < 1e-6 * L
     assert np.allclose(-dv.download(), vel, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_diagnostics_match_host_sums(jtp, precision):
+    """jtp_diagnostics: FP64 device reductions equal host FP64 sums of the same arrays (kinetic energy and
+    momentum per species; 1/2 sum rho phi over both interlaced grids of the last force call)."""
+    print(f"case: diagnostics precision={precision}", flush=True)
+    w = synth.clustered(n_total=50_000, box_size=100.0, n_cells=32, n_halos=20, seed=34)
+    rt = np.float64 if precision == 1 else np.float32
+    rng = np.random.default_rng(4)
+    vel = rng.standard_normal((w.n, 3)).astype(rt)
+    ctx = _ctx(jtp, w, precision)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx.compute_forces(pos, w.species)
+    dv, ds = jtp.DeviceArray.from_host(vel), jtp.DeviceArray.from_host(w.species)
+    d = ctx.diagnostics(w.n, dv, ds)
+    v64 = vel.astype(np.float64)
+    for k, (sg, m) in enumerate(((1, w.m_plus), (-1, w.m_minus_abs))):
+        sel = w.species == sg
+        ke = 0.5 * m * np.sum(v64[sel] ** 2)
+        mom = m * v64[sel].sum(axis=0)
+        assert abs(d["kinetic"][k] - ke) <= 1e-12 * ke
+        assert np.allclose(d["momentum"][k], mom, rtol=1e-10, atol=1e-10 * m * np.abs(v64[sel]).sum())
+    rho, phi = ctx.stage(jtp.STAGE_RHO).astype(np.float64), ctx.stage(jtp.STAGE_PHI).astype(np.float64)
+    rho2, phi2 = ctx.stage(jtp.STAGE_RHO2).astype(np.float64), ctx.stage(jtp.STAGE_PHI2).astype(np.float64)
+    e = 0.25 * (np.dot(rho, phi) + np.dot(rho2, phi2))
+    assert abs(d["pm_energy"] - e) <= 1e-10 * abs(e)
+    assert d["pm_energy"] != 0
