@@ -71,15 +71,15 @@ def phase_summary(stats):
 
 def cpu_baseline(w_full):
     """Reference CPU TreePM (oracle restatement of treepm_force.rs / tree_short.rs / pm_grid.rs: leaf-1 octree,
-    x^4 split, r_s = r_cut/3, two-grid PM) timed on a bounded sample: the same recipe in a box of 1/64 the
-    volume at the same number density, cell size, r_s, theta and softening."""
+    x^4 split, r_s = r_cut/3, two-grid PM) timed on a bounded sample (about 10 s on 16 cores): the same
+    recipe in a box of 1/8 the volume at the same number density, cell size, r_s, theta and softening."""
     from oracle.oracle_lib import RefCpu, load
 
     orc = load()
-    n_s = w_full.n // 64
-    Ls = w_full.box_size / 4
-    Gs = w_full.n_cells // 4
-    ws = synth.clustered(n_total=n_s, box_size=Ls, n_cells=Gs, n_halos=max(1, 2000 // 64), seed=99)
+    n_s = w_full.n // 8
+    Ls = w_full.box_size / 2
+    Gs = w_full.n_cells // 2
+    ws = synth.clustered(n_total=n_s, box_size=Ls, n_cells=Gs, n_halos=max(1, 2000 // 8), seed=99)
     h = Ls / Gs
     r_s = w_full.r_s
     r_cut = 3.0 * r_s
@@ -93,7 +93,7 @@ def cpu_baseline(w_full):
         "value": n_s / t, "unit": "particle-force-updates/s", "cores": int(orc.lib.orc_num_threads()),
         "kind": "port",
         "sample": f"C++ restatement of the reference CPU TreePM (leaf-1 octree, x^4 split, r_s=r_cut/3, two-grid "
-                  f"FP64 PM) on the same recipe at 1/64 volume: N={n_s}, L={Ls:g}, PM {Gs}^3, theta={w_full.theta}, "
+                  f"FP64 PM) on the same recipe at 1/8 volume: N={n_s}, L={Ls:g}, PM {Gs}^3, theta={w_full.theta}, "
                   f"eps={w_full.softening:.4g}; one update + all forces in {t:.2f} s",
     }
 
@@ -164,6 +164,21 @@ def run_ensemble(args):
     n = [w.n for w in ws]
     ntot = sum(n)
     modes = {}
+    # baseline: the boxes one after another as separate force calls (each call synchronises and reads
+    # its status back, as a host loop over boxes would)
+    ctxs = [j.ForceContext(p, w.m_plus, w.m_minus_abs) for p, w in zip(params, ws)]
+    for _ in range(args.warmup):
+        for b, ctx in enumerate(ctxs):
+            ctx.compute_forces_device(n[b], dpos[b], dsp[b], dacc[b])
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        for b, ctx in enumerate(ctxs):
+            ctx.compute_forces_device(n[b], dpos[b], dsp[b], dacc[b])
+    dt = (time.perf_counter() - t) / args.steps
+    modes["separate_calls"] = {"ms_per_eval": dt * 1e3, "updates_per_s": ntot / dt}
+    log(f"  separate calls: {dt * 1e3:.2f} ms per evaluation of all boxes")
+    for ctx in ctxs:
+        ctx.close()
     for mode in ("sequential", "concurrent", "graph"):
         kw = dict(sequential=(mode == "sequential"), graph=(mode == "graph"))
         for _ in range(args.warmup):
@@ -177,7 +192,7 @@ def run_ensemble(args):
         modes[mode] = {"ms_per_eval": dt * 1e3, "updates_per_s": ntot / dt}
         log(f"  ensemble {mode}: {dt * 1e3:.2f} ms per evaluation of all boxes")
     st = ens.stats()
-    best = max(modes, key=lambda m: modes[m]["updates_per_s"])
+    best = max((m for m in modes if m != "separate_calls"), key=lambda m: modes[m]["updates_per_s"])
     out = {
         "metric": "TreePM particle-force-updates/s over an uneven ensemble of 48 boxes on 1 GPU",
         "value": modes[best]["updates_per_s"], "unit": "particle-force-updates/s", "n_gpus": 1,
@@ -187,6 +202,7 @@ def run_ensemble(args):
                                f"(total {ntot}), G in {{32,64,128}}, L ~ U[100,500], |m-|/m+ ~ U[5,40]",
                    "best_mode": best, "parallelism": "single GPU", "overrides": OVERRIDES},
         "modes": modes, "interactions": st["interactions"],
+        "best_over_separate_calls": modes["separate_calls"]["ms_per_eval"] / modes[best]["ms_per_eval"],
         "concurrent_over_sequential": modes["sequential"]["ms_per_eval"] / modes["concurrent"]["ms_per_eval"],
         "graph_over_sequential": modes["sequential"]["ms_per_eval"] / modes["graph"]["ms_per_eval"],
     }

@@ -135,7 +135,7 @@ void make_split_poly(Ctx& c) {
 static void free_particle_buffers(Ctx& c) {
     void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
                     c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
-                    c.frontier0, c.frontier1, c.dense_corner};
+                    c.frontier0, c.dense_list, c.dense_corner};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c.keys0 = c.keys1 = nullptr;
@@ -145,7 +145,7 @@ static void free_particle_buffers(Ctx& c) {
     c.node_geo = nullptr;
     c.node_kids = nullptr;
     c.parent_int = c.parent_leaf = c.flags = nullptr;
-    c.frontier0 = c.frontier1 = nullptr;
+    c.frontier0 = c.dense_list = nullptr;
     c.dense_corner = nullptr;
     c.node_sum = c.node_mono = nullptr;
     c.groups = nullptr;
@@ -177,7 +177,7 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.parent_leaf, cap));
     chk(dalloc(c.flags, cap));
     chk(dalloc(c.frontier0, cap));
-    chk(dalloc(c.frontier1, cap));
+    chk(dalloc(c.dense_list, cap / kDenseCell + 1));
     chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * (8 + 27) * rs));
     chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));
     chk(dalloc_bytes(c.node_mono, (size_t)cap * (rs == 8 ? 64 : 32)));
@@ -294,6 +294,10 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = pm_tables(c);
     if (e == hipSuccess) {
         for (int i = 0; i < 10 && e == hipSuccess; ++i) e = hipEventCreate(&c.ev[i]);
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreate(&c.ev_build[i]);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c.join_ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking);  // default priority
         c.events_ok = (e == hipSuccess);
     }
     if (e != hipSuccess) {
@@ -322,6 +326,11 @@ int jtp_destroy(jtp_ctx* h) {
     if (c.host_step) (void)hipHostFree(c.host_step);
     for (int i = 0; i < 10; ++i)
         if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
+    for (int i = 0; i < 2; ++i)
+        if (c.ev_build[i]) (void)hipEventDestroy(c.ev_build[i]);
+    if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
+    if (c.join_ev) (void)hipEventDestroy(c.join_ev);
+    if (c.side) (void)hipStreamDestroy(c.side);
     delete h;
     return JTP_OK;
 }
@@ -348,7 +357,21 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
     JTP_HIP(&c, radix_sort_pairs(c, n, s));
     JTP_HIP(&c, launch_gather_parts(d_pos, d_species, c.vals0, c.parts, n, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[1], s));
-    if (c.p.tree_enabled) JTP_HIP(&c, build_tree(c, n, s));
+    // The tree build and the PM phases only share read-only inputs (sorted keys and records), so the
+    // build runs on the side stream while the PM phases run on `s`; the walk waits for both.
+    const bool fork = c.p.tree_enabled && c.p.pm_enabled;
+    if (fork) {
+        JTP_HIP(&c, hipEventRecord(c.fork_ev, s));
+        JTP_HIP(&c, hipStreamWaitEvent(c.side, c.fork_ev, 0));
+        if (timed) JTP_HIP(&c, hipEventRecord(c.ev_build[0], c.side));
+        JTP_HIP(&c, build_tree(c, n, c.side));
+        if (timed) JTP_HIP(&c, hipEventRecord(c.ev_build[1], c.side));
+        JTP_HIP(&c, hipEventRecord(c.join_ev, c.side));
+    } else if (c.p.tree_enabled) {
+        if (timed) JTP_HIP(&c, hipEventRecord(c.ev_build[0], s));
+        JTP_HIP(&c, build_tree(c, n, s));
+        if (timed) JTP_HIP(&c, hipEventRecord(c.ev_build[1], s));
+    }
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[2], s));
     if (c.p.pm_enabled) JTP_HIP(&c, pm_deposit(c, n, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[3], s));
@@ -357,6 +380,8 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[4], s));
     if (c.p.pm_enabled) JTP_HIP(&c, pm_interp(c, n, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[5], s));
+    if (fork) JTP_HIP(&c, hipStreamWaitEvent(s, c.join_ev, 0));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[7], s));
     if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s));
     else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[6], s));
@@ -586,8 +611,14 @@ int jtp_get_stats(jtp_ctx* h, jtp_stats* out) {
     // the total covers the force call, or the whole step with its kicks and drifts
     float ms[6] = {-1, -1, -1, -1, -1, -1};
     if (c.last_kind < 2) {
+        // sort ev0-ev1; build on its own events (side stream); deposit ev2-ev3, FFT ev3-ev4,
+        // interpolation ev4-ev5; walk ev7-ev6 (ev7 follows the wait for the build)
         JTP_HIP(&c, hipEventSynchronize(c.ev[6]));
-        for (int i = 0; i < 6; ++i) JTP_HIP(&c, hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]));
+        JTP_HIP(&c, hipEventElapsedTime(&ms[0], c.ev[0], c.ev[1]));
+        if (c.p.tree_enabled) JTP_HIP(&c, hipEventElapsedTime(&ms[1], c.ev_build[0], c.ev_build[1]));
+        else ms[1] = 0;
+        for (int i = 2; i < 5; ++i) JTP_HIP(&c, hipEventElapsedTime(&ms[i], c.ev[i], c.ev[i + 1]));
+        JTP_HIP(&c, hipEventElapsedTime(&ms[5], c.ev[7], c.ev[6]));
     }
     out->ms_sort = ms[0];
     out->ms_build = ms[1];

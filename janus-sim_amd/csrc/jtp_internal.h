@@ -118,7 +118,8 @@ struct Ctx {
     int32_t *parent_int = nullptr;   // parent of internal node
     int32_t *parent_leaf = nullptr;  // parent of leaf
     int32_t *flags = nullptr;        // bottom-up arrival counters
-    int32_t *frontier0 = nullptr, *frontier1 = nullptr;  // bottom-up frontiers
+    int32_t *frontier0 = nullptr;    // bottom-up: nodes whose range crosses a chunk edge, ready to climb
+    int32_t *dense_list = nullptr;   // deposit: the PM cells holding more than kDenseCell particles
     void *node_sum = nullptr;        // NodeSum per internal node
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
     int2 *groups = nullptr;
@@ -152,8 +153,12 @@ struct Ctx {
     int64_t last_n = -1;
     bool have_result = false;
     int last_kind = 0;                // 0 force call, 1 eager step, 2 graph step, 3 ensemble graph
-    // ev[0..6]: force phase boundaries; ev[8], ev[9]: step begin / end
+    // ev[0..6]: force phase boundaries, ev[7]: walk start; ev[8], ev[9]: step begin / end;
+    // ev_build[2]: the tree build on the side stream
     hipEvent_t ev[10]{};
+    hipEvent_t ev_build[2]{};
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipStream_t side = nullptr;       // the tree build runs here, concurrently with the PM phases
     bool events_ok = false;
     hipStream_t last_stream = nullptr;
     int64_t n_groups_max = 0;

@@ -3,7 +3,7 @@
 //
 // Reference behaviour replaced:
 //   cic_scatter (FP64 CAS atomics into two grids), nbody_gpu_twopass.rs:85-158,
-//     and PmGrid::assign_mass, src/treepm/pm_grid.rs:63-103 -> k_deposit_tile: one net grid
+//     and PmGrid::assign_mass, src/treepm/pm_grid.rs:63-103 -> k_deposit: one net grid
 //     rho+ - rho- gathered per node from Morton-sorted particles; no float atomics.
 //   cufft_solve_device_filtered (cuFFT D2Z / apply_green_kernel / Z2D / normalize),
 //     cuda/cufft_wrapper.cu:148-208,290-295,350-398, and the rustfft passes of
@@ -124,129 +124,93 @@ template <typename R> struct Vec4T;
 template <> struct Vec4T<float> { using T = float4; };
 template <> struct Vec4T<double> { using T = double4; };
 
-// Gather deposit, one wavefront per 4x4x4 tile of grid nodes (64 Morton-consecutive nodes, one
-// per lane).  The tile's source cells (5^3 for the node grid: node n gathers cells n-1 and n per
-// axis; 6^3 for the interlaced grid, whose node at the centre of cell m gathers cells m-1, m, m+1)
-// are looked up once into LDS, and the particles of the sparse ones are staged in LDS as
-// (fx, fy, fz, q); every lane then sums its node's contributions in the fixed order
-// (dz, dy, dx, particle), reading dense cells' pre-reduced corner sums.  If the tile's sparse
-// cells hold more than CAP particles the lanes read them from global memory instead, with the
-// same arithmetic and order, so the result never depends on the path taken.
-//   node grid, cell offset d in {-1, 0} per axis: weight (d == 0 ? 1 - f : f);
-//   interlaced grid, d in {-1, 0, 1}: shifted_weights(f)[1 - d], written at node m + 1.
-template <typename R, bool SHIFTED>
-__global__ __launch_bounds__(64) void k_deposit_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
-                                                     const int32_t* __restrict__ slot, const R* __restrict__ corner,
-                                                     int logG, R qp, R qm, R* __restrict__ out) {
-    constexpr int NA = SHIFTED ? 6 : 5;
-    constexpr int NCELL = NA * NA * NA;
-    constexpr int CAP = sizeof(R) == 4 ? 1024 : 512;
-    constexpr int NSUM = SHIFTED ? 27 : 8;
-    using V = typename Vec4T<R>::T;
-    __shared__ int32_t c_s[NCELL], c_off[NCELL], c_cnt[NCELL];  // c_cnt < 0: dense, slot = -c_cnt - 1
-    __shared__ V stage[CAP];
-    const int G = 1 << logG;
-    const int32_t msk = G - 1;
-    const int lane = threadIdx.x;
-    const uint64_t m0 = (uint64_t)blockIdx.x * 64;
-    const int32_t i0 = (int32_t)compact3_21(m0), j0 = (int32_t)compact3_21(m0 >> 1), k0 = (int32_t)compact3_21(m0 >> 2);
-
-    int run = 0;
-    for (int t = 0; t < (NCELL + 63) / 64; ++t) {
-        const int c = t * 64 + lane;
-        int cnt = 0;
-        if (c < NCELL) {
-            const int a = c % NA, b = (c / NA) % NA, d = c / (NA * NA);
-            const uint32_t ci = (uint32_t)((i0 - 1 + a) & msk), cj = (uint32_t)((j0 - 1 + b) & msk),
-                           ck = (uint32_t)((k0 - 1 + d) & msk);
-            const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
-            const int32_t s0 = cs[cm], e0 = cs[cm + 1];
-            c_s[c] = s0;
-            if (e0 - s0 > kDenseCell) {
-                c_cnt[c] = -slot[cm] - 1;
-            } else {
-                c_cnt[c] = e0 - s0;
-                cnt = e0 - s0;
-            }
-        }
-        const int incl = wave_incl_scan(cnt);
-        if (c < NCELL) c_off[c] = run + incl - cnt;
-        run += __shfl(incl, 63, 64);
-    }
-    const int total = run;
-    __syncthreads();
-    const bool staged = total <= CAP;
-    if (staged) {
-        for (int idx = lane; idx < total; idx += 64) {
-            int lo = 0, hi = NCELL;  // first cell whose offset exceeds idx; the particle is in the one before
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (c_off[mid] <= idx) lo = mid + 1; else hi = mid;
-            }
-            const int c = lo - 1;
-            const uint4 r = parts[c_s[c] + (idx - c_off[c])];
-            V v;
-            v.x = frac_of<R>(r.x, logG);
-            v.y = frac_of<R>(r.y, logG);
-            v.z = frac_of<R>(r.z, logG);
-            v.w = (int32_t)r.w > 0 ? qp : qm;
-            stage[idx] = v;
-        }
-        __syncthreads();
-    }
-
-    const int li = (lane & 1) | ((lane >> 2) & 2);
-    const int lj = ((lane >> 1) & 1) | ((lane >> 3) & 2);
-    const int lk = ((lane >> 2) & 1) | ((lane >> 4) & 2);
-    constexpr int DLO = -1, DHI = SHIFTED ? 1 : 0;
+// Shifted grid (interlacing): node n' sits at the centre of cell m = n' - 1 and gathers the
+// particles of cells m-1, m, m+1 per axis with weight 1 - |x - x_node| / h, i.e. per axis
+// (d = cell offset from m, f = fraction in that cell): d = -1: max(0, f - 1/2); d = 0: 1 - |f - 1/2|;
+// d = +1: max(0, 1/2 - f).
+template <typename R>
+__global__ void k_deposit_shifted(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                                  const int32_t* __restrict__ slot, const R* __restrict__ corner27, int logG, R qp,
+                                  R qm, R* __restrict__ rho2) {
+    const int64_t G = 1ll << logG;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= G * G * G) return;
+    const int32_t i = (int32_t)compact3_21((uint64_t)m);
+    const int32_t j = (int32_t)compact3_21((uint64_t)m >> 1);
+    const int32_t k = (int32_t)compact3_21((uint64_t)m >> 2);
+    const int32_t msk = (int32_t)(G - 1);
     R acc = 0;
-    for (int dz = DLO; dz <= DHI; ++dz) {
-        for (int dy = DLO; dy <= DHI; ++dy) {
+    for (int dz = -1; dz <= 1; ++dz) {
+        for (int dy = -1; dy <= 1; ++dy) {
 #pragma unroll
-            for (int dx = DLO; dx <= DHI; ++dx) {
-                const int c = (li + 1 + dx) + NA * ((lj + 1 + dy) + NA * (lk + 1 + dz));
-                const int32_t info = c_cnt[c];
-                if (info < 0) {
-                    const int q = SHIFTED ? (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz) : (-dx) + 2 * (-dy) + 4 * (-dz);
-                    acc += corner[NSUM * (int64_t)(-info - 1) + q];
+            for (int dx = -1; dx <= 1; ++dx) {
+                const uint32_t ci = (uint32_t)((i + dx) & msk), cj = (uint32_t)((j + dy) & msk),
+                               ck = (uint32_t)((k + dz) & msk);
+                const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
+                const int32_t s = cs[cm], e = cs[cm + 1];
+                if (e - s > kDenseCell) {
+                    // this node is the centre of cell (cell - d): shifted index e = -d
+                    acc += corner27[27 * (int64_t)slot[cm] + (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz)];
                     continue;
                 }
-                const int32_t off = c_off[c], s0 = c_s[c];
-                for (int32_t p = 0; p < info; ++p) {
-                    R fx, fy, fz, q;
-                    if (staged) {
-                        const V v = stage[off + p];
-                        fx = v.x; fy = v.y; fz = v.z; q = v.w;
-                    } else {
-                        const uint4 r = parts[s0 + p];
-                        fx = frac_of<R>(r.x, logG);
-                        fy = frac_of<R>(r.y, logG);
-                        fz = frac_of<R>(r.z, logG);
-                        q = (int32_t)r.w > 0 ? qp : qm;
-                    }
-                    if (SHIFTED) {
-                        R wx[3], wy[3], wz[3];
-                        shifted_weights<R>(fx, wx);
-                        shifted_weights<R>(fy, wy);
-                        shifted_weights<R>(fz, wz);
-                        acc += q * (wx[1 - dx] * wy[1 - dy] * wz[1 - dz]);
-                    } else {
-                        const R wx = dx == 0 ? (R)1 - fx : fx;
-                        const R wy = dy == 0 ? (R)1 - fy : fy;
-                        const R wz = dz == 0 ? (R)1 - fz : fz;
-                        acc += q * (wx * wy * wz);
-                    }
+                for (int32_t p = s; p < e; ++p) {
+                    const uint4 r = parts[p];
+                    R wx[3], wy[3], wz[3];
+                    shifted_weights<R>(frac_of<R>(r.x, logG), wx);
+                    shifted_weights<R>(frac_of<R>(r.y, logG), wy);
+                    shifted_weights<R>(frac_of<R>(r.z, logG), wz);
+                    const R q = (int32_t)r.w > 0 ? qp : qm;
+                    acc += q * (wx[1 - dx] * wy[1 - dy] * wz[1 - dz]);
                 }
             }
         }
     }
-    const int32_t i = i0 + li, j = j0 + lj, k = k0 + lk;
-    if (SHIFTED) {
-        const int64_t ii = (i + 1) & msk, jj = (j + 1) & msk, kk = (k + 1) & msk;
-        out[(kk * G + jj) * G + ii] = acc;
-    } else {
-        out[((int64_t)k * G + j) * G + i] = acc;
+    const int64_t ii = (i + 1) & msk, jj = (j + 1) & msk, kk = (k + 1) & msk;
+    rho2[(kk * G + jj) * G + ii] = acc;
+}
+
+// Thread per grid node, nodes enumerated in Morton order so that the eight
+// source cells of a wavefront's nodes are contiguous particle runs.
+template <typename R>
+__global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                          const int32_t* __restrict__ slot, const R* __restrict__ corner, int logG, R qp, R qm,
+                          R* __restrict__ rho) {
+    const int64_t G = 1ll << logG;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= G * G * G) return;
+    const int32_t i = (int32_t)compact3_21((uint64_t)m);
+    const int32_t j = (int32_t)compact3_21((uint64_t)m >> 1);
+    const int32_t k = (int32_t)compact3_21((uint64_t)m >> 2);
+    const int32_t msk = (int32_t)(G - 1);
+    R acc = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 0; ++dz) {
+#pragma unroll
+        for (int dy = -1; dy <= 0; ++dy) {
+#pragma unroll
+            for (int dx = -1; dx <= 0; ++dx) {
+                const uint32_t ci = (uint32_t)((i + dx) & msk), cj = (uint32_t)((j + dy) & msk),
+                               ck = (uint32_t)((k + dz) & msk);
+                const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
+                const int32_t s = cs[cm], e = cs[cm + 1];
+                const int corner_idx = (-dx) + 2 * (-dy) + 4 * (-dz);  // node = cell corner (-dx, -dy, -dz)
+                if (e - s > kDenseCell) {
+                    acc += corner[8 * (int64_t)slot[cm] + corner_idx];
+                    continue;
+                }
+                for (int32_t p = s; p < e; ++p) {
+                    const uint4 r = parts[p];
+                    const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
+                    const R wx = dx == 0 ? (R)1 - fx : fx;
+                    const R wy = dy == 0 ? (R)1 - fy : fy;
+                    const R wz = dz == 0 ? (R)1 - fz : fz;
+                    const R q = (int32_t)r.w > 0 ? qp : qm;
+                    acc += q * (wx * wy * wz);
+                }
+            }
+        }
     }
+    rho[((int64_t)k * G + j) * G + i] = acc;
 }
 
 // ------------------------------------------------------------------ FFT core
@@ -480,7 +444,7 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * c.logG, G3, c.cell_start, s);
     if (e != hipSuccess) return e;
     int32_t* count = c.dev_counters + kCtrDense;
-    int32_t* list = c.frontier0;  // free after the tree build; holds <= n / kDenseCell cells
+    int32_t* list = c.dense_list;  // <= n / kDenseCell cells
     e = launch_fill_i32(count, 1, 0, s);
     if (e != hipSuccess) return e;
     const unsigned nb = (unsigned)((G3 + kBlock - 1) / kBlock);
@@ -489,20 +453,18 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     unsigned nbd = (unsigned)((nd_max * 64 + kBlock - 1) / kBlock);
     if (nbd > 8192) nbd = 8192;
     const int64_t nd_cap = n / kDenseCell + 1;
-    const unsigned ntile = (unsigned)(G3 / 64);  // G >= 16, so G^3 is a multiple of 64
     if (c.p.precision == JTP_FP64) {
         double* c8 = (double*)c.dense_corner;
         double* c27 = c8 + 8 * nd_cap;
         k_dense_reduce<double, false><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
                                                                       c.m_plus, -c.m_minus, c8);
-        k_deposit_tile<double, false><<<dim3(ntile), dim3(64), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8,
-                                                                       c.logG, c.m_plus, -c.m_minus, (double*)c.rho);
+        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, c.m_plus,
+                                                            -c.m_minus, (double*)c.rho);
         if (c.p.interlace) {
             k_dense_reduce<double, true><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
                                                                          c.m_plus, -c.m_minus, c27);
-            k_deposit_tile<double, true><<<dim3(ntile), dim3(64), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
-                                                                          c.logG, c.m_plus, -c.m_minus,
-                                                                          (double*)c.rho2);
+            k_deposit_shifted<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
+                                                                        c.logG, c.m_plus, -c.m_minus, (double*)c.rho2);
         }
     } else {
         float* c8 = (float*)c.dense_corner;
@@ -510,13 +472,13 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
         const float qp = (float)c.m_plus, qm = (float)-c.m_minus;
         k_dense_reduce<float, false><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG, qp,
                                                                      qm, c8);
-        k_deposit_tile<float, false><<<dim3(ntile), dim3(64), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG,
-                                                                      qp, qm, (float*)c.rho);
+        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, qp, qm,
+                                                           (float*)c.rho);
         if (c.p.interlace) {
             k_dense_reduce<float, true><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
                                                                         qp, qm, c27);
-            k_deposit_tile<float, true><<<dim3(ntile), dim3(64), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
-                                                                         c.logG, qp, qm, (float*)c.rho2);
+            k_deposit_shifted<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
+                                                                       c.logG, qp, qm, (float*)c.rho2);
         }
     }
     return hipGetLastError();

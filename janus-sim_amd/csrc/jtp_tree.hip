@@ -190,18 +190,26 @@ __device__ inline void finish_node(int32_t p, const uint4* __restrict__ parts, u
     write_mono<PREC>(p, s, mn, mx, mono, qp, qm);
 }
 
-// Bottom-up pass in two launches, with no device-scope fence (the arrival-counter climb with a fence
-// per level is what the reference does, nbody_gpu_twopass.rs:485-560).
+// Bottom-up pass in two launches, with no device-scope fence (the reference climbs with a
+// device-scope fence per level, nbody_gpu_twopass.rs:485-560; on MI355X a device-scope release
+// writes back the per-XCD L2, which made that pattern ~100x slower than this one).
 //
-// 1. k_bu_local: one workgroup per chunk of kBuChunk consecutive sorted leaves.  A Karras node whose
-//    leaf range lies inside the chunk ("local") has only local descendants and its index lies in the
-//    chunk, so the workgroup completes all of its local nodes level by level with LDS frontiers and
-//    LDS arrival counters, separated by workgroup barriers.  A local node whose parent is not local
-//    bumps the parent's global arrival counter (flags[], preset by k_karras to the number of leaf
-//    children); the arrival that completes the parent appends it to the global frontier.
-// 2. k_bu_top: one workgroup finishes the remaining nodes (those whose range crosses a chunk
-//    boundary: about the chunk count times the height of the crossing paths) level by level.
-constexpr int kBuChunk = 2048;
+// 1. k_bu_local: one workgroup per chunk of kBuChunk consecutive sorted leaves.  A Karras node
+//    whose leaf range lies inside the chunk ("local") has only local descendants and its index
+//    lies in the chunk.  Threads climb from the local nodes with two leaf children: finish the
+//    node, then bump its parent's arrival count in LDS; the arrival that completes the parent
+//    continues with it, the other stops.  Only waves of one workgroup meet at a local node, so
+//    workgroup-scope release / acquire fences order the node data.  A local node whose parent is
+//    not local bumps the parent's global arrival count (flags[], preset by k_karras to the number
+//    of leaf children); the arrival that completes that parent appends it to a global list.
+// 2. k_bu_top: one workgroup climbs the same way from that list through the remaining nodes
+//    (those whose range crosses a chunk edge), with the arrival counts in flags[]; all of their
+//    arrivals in this launch come from this workgroup, so workgroup-scope fences suffice again.
+// Each node is finished exactly once, with its children combined in the fixed order (left, right).
+constexpr int kBuChunk = 4096;
+
+__device__ inline void wg_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+__device__ inline void wg_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 
 template <int PREC>
 __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
@@ -209,13 +217,13 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
                                                   int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                   double qp, double qm, int32_t* __restrict__ gfront,
                                                   int32_t* __restrict__ ctrl) {
-    __shared__ int32_t cnt[kBuChunk];  // arrival count of a local node, -1 if the node is not local
-    __shared__ int32_t fr[2][kBuChunk];
-    __shared__ int32_t fsz[2];
+    __shared__ int32_t cnt[kBuChunk];    // arrival count of a local node, -1 if the node is not local
+    __shared__ int32_t start[kBuChunk];  // local nodes with two leaf children
+    __shared__ int32_t nstart;
     const int tid = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * kBuChunk;
     const int64_t lastLeaf = base + kBuChunk - 1;
-    if (tid < 2) fsz[tid] = 0;
+    if (tid == 0) nstart = 0;
     __syncthreads();
     for (int k = tid; k < kBuChunk; k += blockDim.x) {
         const int64_t i = base + k;
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
             const int64_t first = geo[2 * i].w, last = geo[2 * i + 1].w;
             if (first >= base && last <= lastLeaf) {
                 c = leaves;
-                if (leaves == 2) fr[0][atomicAdd(&fsz[0], 1)] = (int32_t)i;
+                if (leaves == 2) start[atomicAdd(&nstart, 1)] = (int32_t)i;
             } else if (leaves == 2) {
                 gfront[atomicAdd(&ctrl[0], 1)] = (int32_t)i;
             }
@@ -235,31 +243,28 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
         cnt[k] = c;
     }
     __syncthreads();
-    int cur = 0;
-    while (true) {
-        const int size = fsz[cur];
-        if (size == 0) break;  // block-uniform: every thread read fsz after the same barrier
-        __syncthreads();       // everyone has read fsz[cur] before it can be reused
-        if (tid == 0) fsz[cur ^ 1] = 0;
-        __syncthreads();
-        for (int k = tid; k < size; k += blockDim.x) {
-            const int32_t p = fr[cur][k];
+    const int ns = nstart;
+    for (int t = tid; t < ns; t += blockDim.x) {
+        int32_t p = start[t];
+        while (true) {
             finish_node<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
             if (p == 0) {
                 ctrl[kMaxRounds + 1] = 1;  // root complete
-                continue;
+                break;
             }
             const int32_t par = parent_int[p];
-            if (par < 0 || par >= nI) continue;
+            if (par < 0 || par >= nI) break;
             const int64_t lp = (int64_t)par - base;
             if (lp >= 0 && lp < kBuChunk && cnt[lp] >= 0) {
-                if (atomicAdd(&cnt[lp], 1) == 1) fr[cur ^ 1][atomicAdd(&fsz[cur ^ 1], 1)] = par;
-            } else if (atomicAdd(&flags[par], 1) == 1) {
-                gfront[atomicAdd(&ctrl[0], 1)] = par;
+                wg_release();
+                if (atomicAdd(&cnt[lp], 1) != 1) break;
+                wg_acquire();
+                p = par;
+            } else {
+                if (atomicAdd(&flags[par], 1) == 1) gfront[atomicAdd(&ctrl[0], 1)] = par;
+                break;
             }
         }
-        __syncthreads();
-        cur ^= 1;
     }
 }
 
@@ -267,37 +272,24 @@ template <int PREC>
 __global__ __launch_bounds__(1024) void k_bu_top(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
                                                  const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
                                                  int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
-                                                 double qp, double qm, int32_t* __restrict__ f0,
-                                                 int32_t* __restrict__ f1, int32_t* __restrict__ ctrl) {
-    __shared__ int32_t fsz[2];
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        fsz[0] = ctrl[0];
-        fsz[1] = 0;
-    }
-    __syncthreads();
-    int cur = 0;
-    // Every node enters a frontier at most once, so the loop ends after at most nI passes.
-    while (true) {
-        const int size = fsz[cur];
-        if (size == 0) break;
-        int32_t* fin = cur ? f1 : f0;
-        int32_t* fout = cur ? f0 : f1;
-        __syncthreads();
-        if (tid == 0) fsz[cur ^ 1] = 0;
-        __syncthreads();
-        for (int k = tid; k < size; k += blockDim.x) {
-            const int32_t p = fin[k];
+                                                 double qp, double qm, const int32_t* __restrict__ starts,
+                                                 int32_t* __restrict__ ctrl) {
+    const int ns = ctrl[0];
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+        int32_t p = starts[t];
+        while (true) {
             finish_node<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
             if (p == 0) {
                 ctrl[kMaxRounds + 1] = 1;
-                continue;
+                break;
             }
             const int32_t par = parent_int[p];
-            if (par >= 0 && par < nI && atomicAdd(&flags[par], 1) == 1) fout[atomicAdd(&fsz[cur ^ 1], 1)] = par;
+            if (par < 0 || par >= nI) break;
+            wg_release();
+            if (atomicAdd(&flags[par], 1) != 1) break;
+            wg_acquire();
+            p = par;
         }
-        __syncthreads();
-        cur ^= 1;
     }
 }
 
@@ -404,12 +396,12 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
             k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
             k_bu_top<JTP_FP64><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
-                                                              sums, c.node_mono, qp, qm, c.frontier0, c.frontier1, ctrl);
+                                                              sums, c.node_mono, qp, qm, c.frontier0, ctrl);
         } else {
             k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
             k_bu_top<JTP_FP32><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
-                                                              sums, c.node_mono, qp, qm, c.frontier0, c.frontier1, ctrl);
+                                                              sums, c.node_mono, qp, qm, c.frontier0, ctrl);
         }
     }
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * kGroupLevelBits, kNCell4, c.cell4, s);

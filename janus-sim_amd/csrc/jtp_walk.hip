@@ -23,10 +23,13 @@
 
 namespace jtp {
 
-constexpr int kStackCap = 640;          // logical stack bound per wave
-constexpr int kStackPhys = kStackCap + 192;
-constexpr int kListCap = 512;
-constexpr int kFlushAt = 256;
+// LDS per wave (FP32): 384 x 16 B list + 576 x 4 B stack = 8.3 KB, so four 4-wave workgroups fit
+// a CU (16 waves, the 124-VGPR limit); pops shrink when the stack nears kStackCap, so a smaller
+// stack costs parallelism in deep traversals, never correctness.
+constexpr int kStackCap = 448;          // logical stack bound per wave
+constexpr int kStackPhys = kStackCap + 128;
+constexpr int kListCap = 384;
+constexpr int kFlushAt = 192;
 constexpr int kPad = 4;  // list slots kept free for the lane-sharing padding
 
 struct WalkArgs {
@@ -303,8 +306,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const int32_t node = has ? S[sp + lane] : 0;
             wave_lds_fence();
 
-            // act: 1 accept the monopoles (tail form), 4 accept them as cut pairs (node entirely within
-            // r_cut of the group box, so the cut never applies and the cheaper pair law is exact),
+            // act: 1 accept the monopoles (tail form), 4 accept them as cut pairs (FP32: node entirely
+            // within r_cut of the group box, so the cut never applies and the cheaper pair law is exact),
             // 2 take the bucket's particles, 3 open
             int act = 0, napp = 0, napm = 0, nmono_p = 0;
             int32_t first = 0;
@@ -349,7 +352,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     const R th2 = inside ? theta2 : thetac2;
                     if (node >= 0 && ext * ext < th2 * gap2) {
                         const int nq = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
-                        if (inside && MODE != JTP_SPLIT_POLY4) {
+                        if (sizeof(R) == 4 && inside && MODE != JTP_SPLIT_POLY4) {
                             act = 4;
                             napp = nq;
                             nmono_p = nq;
