@@ -135,7 +135,7 @@ void make_split_poly(Ctx& c) {
 static void free_particle_buffers(Ctx& c) {
     void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
                     c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
-                    c.frontier0, c.dense_list, c.dense_corner};
+                    c.frontier0, c.frontier1, c.dense_list, c.dense_corner};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c.keys0 = c.keys1 = nullptr;
@@ -145,7 +145,7 @@ static void free_particle_buffers(Ctx& c) {
     c.node_geo = nullptr;
     c.node_kids = nullptr;
     c.parent_int = c.parent_leaf = c.flags = nullptr;
-    c.frontier0 = c.dense_list = nullptr;
+    c.frontier0 = c.frontier1 = c.dense_list = nullptr;
     c.dense_corner = nullptr;
     c.node_sum = c.node_mono = nullptr;
     c.groups = nullptr;
@@ -177,6 +177,7 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.parent_leaf, cap));
     chk(dalloc(c.flags, cap));
     chk(dalloc(c.frontier0, cap));
+    chk(dalloc(c.frontier1, cap));
     chk(dalloc(c.dense_list, cap / kDenseCell + 1));
     chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * (8 + 27) * rs));
     chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));

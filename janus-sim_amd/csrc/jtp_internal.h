@@ -119,6 +119,7 @@ struct Ctx {
     int32_t *parent_leaf = nullptr;  // parent of leaf
     int32_t *flags = nullptr;        // bottom-up arrival counters
     int32_t *frontier0 = nullptr;    // bottom-up: nodes whose range crosses a chunk edge, ready to climb
+    int32_t *frontier1 = nullptr;    // bottom-up: nodes whose range crosses a superchunk edge, ready to climb
     int32_t *dense_list = nullptr;   // deposit: the PM cells holding more than kDenseCell particles
     void *node_sum = nullptr;        // NodeSum per internal node
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node

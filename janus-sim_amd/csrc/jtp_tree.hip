@@ -78,6 +78,7 @@ struct ChildData {
     uint32_t mn[3], mx[3];
     uint64_t sp[3], sm[3];
     uint32_t np, nm;
+    uint32_t first, last;  // sorted leaf range
 };
 
 __device__ inline void load_child(int32_t c, const uint4* parts, uint4* geo, NodeSum* sums, ChildData& o) {
@@ -94,8 +95,11 @@ __device__ inline void load_child(int32_t c, const uint4* parts, uint4* geo, Nod
         }
         o.np = plus ? 1u : 0u;
         o.nm = plus ? 0u : 1u;
+        o.first = o.last = (uint32_t)~c;
     } else {
         const uint4 g0 = geo[2 * c], g1 = geo[2 * c + 1];
+        o.first = g0.w;
+        o.last = g1.w;
         o.mn[0] = g0.x; o.mn[1] = g0.y; o.mn[2] = g0.z;
         o.mx[0] = g1.x; o.mx[1] = g1.y; o.mx[2] = g1.z;
         const NodeSum s = sums[c];
@@ -161,33 +165,57 @@ __device__ inline void write_mono(int32_t p, const NodeSum& s, const uint32_t mn
     }
 }
 
-// Combines the two children of internal node p in the fixed order (left, right): exact integer sums,
-// bounding box (the .w words keep the node's range) and walk records.  The result does not depend on
-// which thread or pass completes the node.
+// Node p from its two children in the fixed order (left, right): exact integer sums, bounding
+// box, leaf range and walk records, written out; returns the node's data for its parent.  The
+// result does not depend on which thread or pass completes the node.
 template <int PREC>
-__device__ inline void finish_node(int32_t p, const uint4* __restrict__ parts, uint4* __restrict__ geo,
-                                   const int2* __restrict__ kids, NodeSum* __restrict__ sums, void* mono,
-                                   double qp, double qm) {
+__device__ inline ChildData combine_write(int32_t p, const ChildData& a, const ChildData& b, uint4* __restrict__ geo,
+                                          NodeSum* __restrict__ sums, void* mono, double qp, double qm) {
+    ChildData o;
+    NodeSum s;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        s.sp[ax] = o.sp[ax] = a.sp[ax] + b.sp[ax];
+        s.sm[ax] = o.sm[ax] = a.sm[ax] + b.sm[ax];
+        o.mn[ax] = min(a.mn[ax], b.mn[ax]);
+        o.mx[ax] = max(a.mx[ax], b.mx[ax]);
+    }
+    s.np = o.np = a.np + b.np;
+    s.nm = o.nm = a.nm + b.nm;
+    s.pad[0] = s.pad[1] = 0;
+    o.first = a.first;
+    o.last = b.last;
+    sums[p] = s;
+    geo[2 * p] = make_uint4(o.mn[0], o.mn[1], o.mn[2], o.first);
+    geo[2 * p + 1] = make_uint4(o.mx[0], o.mx[1], o.mx[2], o.last);
+    write_mono<PREC>(p, s, o.mn, o.mx, mono, qp, qm);
+    return o;
+}
+
+// A node whose two children are leaves: both loaded.
+template <int PREC>
+__device__ inline ChildData finish_start(int32_t p, const uint4* __restrict__ parts, uint4* __restrict__ geo,
+                                         const int2* __restrict__ kids, NodeSum* __restrict__ sums, void* mono,
+                                         double qp, double qm) {
     const int2 k = kids[p];
     ChildData a, b;
     load_child(k.x, parts, geo, sums, a);
     load_child(k.y, parts, geo, sums, b);
-    NodeSum s;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        s.sp[ax] = a.sp[ax] + b.sp[ax];
-        s.sm[ax] = a.sm[ax] + b.sm[ax];
-    }
-    s.np = a.np + b.np;
-    s.nm = a.nm + b.nm;
-    s.pad[0] = s.pad[1] = 0;
-    sums[p] = s;
-    const uint32_t mn[3] = {min(a.mn[0], b.mn[0]), min(a.mn[1], b.mn[1]), min(a.mn[2], b.mn[2])};
-    const uint32_t mx[3] = {max(a.mx[0], b.mx[0]), max(a.mx[1], b.mx[1]), max(a.mx[2], b.mx[2])};
-    const uint32_t first = geo[2 * p].w, last = geo[2 * p + 1].w;
-    geo[2 * p] = make_uint4(mn[0], mn[1], mn[2], first);
-    geo[2 * p + 1] = make_uint4(mx[0], mx[1], mx[2], last);
-    write_mono<PREC>(p, s, mn, mx, mono, qp, qm);
+    return combine_write<PREC>(p, a, b, geo, sums, mono, qp, qm);
+}
+
+// The second arrival at node `par` climbs with the data of its child `p` in registers and loads
+// only the sibling.
+template <int PREC>
+__device__ inline ChildData finish_up(int32_t par, int32_t p, const ChildData& cur, const uint4* __restrict__ parts,
+                                      uint4* __restrict__ geo, const int2* __restrict__ kids,
+                                      NodeSum* __restrict__ sums, void* mono, double qp, double qm) {
+    const int2 k = kids[par];
+    const bool left = k.x == p;
+    ChildData sib;
+    load_child(left ? k.y : k.x, parts, geo, sums, sib);
+    return left ? combine_write<PREC>(par, cur, sib, geo, sums, mono, qp, qm)
+                : combine_write<PREC>(par, sib, cur, geo, sums, mono, qp, qm);
 }
 
 // Bottom-up pass in two launches, with no device-scope fence (the reference climbs with a
@@ -202,9 +230,11 @@ __device__ inline void finish_node(int32_t p, const uint4* __restrict__ parts, u
 //    workgroup-scope release / acquire fences order the node data.  A local node whose parent is
 //    not local bumps the parent's global arrival count (flags[], preset by k_karras to the number
 //    of leaf children); the arrival that completes that parent appends it to a global list.
-// 2. k_bu_top: one workgroup climbs the same way from that list through the remaining nodes
-//    (those whose range crosses a chunk edge), with the arrival counts in flags[]; all of their
-//    arrivals in this launch come from this workgroup, so workgroup-scope fences suffice again.
+// 2. k_bu_mid: the same one level up, one workgroup per superchunk of 64 chunks, with the arrival
+//    counts in flags[]; all arrivals in this launch at a node inside a superchunk come from that
+//    superchunk's workgroup, so workgroup-scope fences suffice again.
+// 3. k_bu_top: one workgroup climbs from the second list through the remaining nodes (those whose
+//    range crosses a superchunk edge).
 // Each node is finished exactly once, with its children combined in the fixed order (left, right).
 constexpr int kBuChunk = 4096;
 
@@ -246,8 +276,8 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
     const int ns = nstart;
     for (int t = tid; t < ns; t += blockDim.x) {
         int32_t p = start[t];
+        ChildData cur = finish_start<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
         while (true) {
-            finish_node<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
             if (p == 0) {
                 ctrl[kMaxRounds + 1] = 1;  // root complete
                 break;
@@ -259,9 +289,56 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
                 wg_release();
                 if (atomicAdd(&cnt[lp], 1) != 1) break;
                 wg_acquire();
+                cur = finish_up<PREC>(par, p, cur, parts, geo, kids, sums, mono, qp, qm);
                 p = par;
             } else {
                 if (atomicAdd(&flags[par], 1) == 1) gfront[atomicAdd(&ctrl[0], 1)] = par;
+                break;
+            }
+        }
+    }
+}
+
+// Middle level: one 1024-thread workgroup per superchunk of kBuSuper leaves climbs from the listed
+// nodes whose leaf range lies inside its superchunk.  Such a node's descendants lie in the same
+// superchunk and were completed either by the first launch or by this workgroup, so workgroup
+// fences suffice; a finished node whose parent crosses the superchunk edge is handed on (second
+// arrival) to the final list.
+constexpr int64_t kBuSuper = (int64_t)kBuChunk * 64;
+
+template <int PREC>
+__global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
+                                                 const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+                                                 int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
+                                                 double qp, double qm, const int32_t* __restrict__ starts,
+                                                 int32_t* __restrict__ next, int32_t* __restrict__ ctrl) {
+    const int64_t lo = (int64_t)blockIdx.x * kBuSuper, hi = lo + kBuSuper - 1;  // leaf range of the superchunk
+    const int ns = ctrl[0];
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+        int32_t p = starts[t];
+        if (p < lo || p > hi) continue;  // a node's index lies in its own range
+        if ((int64_t)geo[2 * p].w < lo || (int64_t)geo[2 * p + 1].w > hi) {
+            next[atomicAdd(&ctrl[1], 1)] = p;  // ready, but it crosses the superchunk edge
+            continue;
+        }
+        ChildData cur = finish_start<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
+        while (true) {
+            if (p == 0) {
+                ctrl[kMaxRounds + 1] = 1;
+                break;
+            }
+            const int32_t par = parent_int[p];
+            if (par < 0 || par >= nI) break;
+            const bool inside = par >= lo && par <= hi && (int64_t)geo[2 * par].w >= lo &&
+                                (int64_t)geo[2 * par + 1].w <= hi;
+            if (inside) {
+                wg_release();
+                if (atomicAdd(&flags[par], 1) != 1) break;
+                wg_acquire();
+                cur = finish_up<PREC>(par, p, cur, parts, geo, kids, sums, mono, qp, qm);
+                p = par;
+            } else {
+                if (atomicAdd(&flags[par], 1) == 1) next[atomicAdd(&ctrl[1], 1)] = par;
                 break;
             }
         }
@@ -274,11 +351,12 @@ __global__ __launch_bounds__(1024) void k_bu_top(const uint4* __restrict__ parts
                                                  int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                  double qp, double qm, const int32_t* __restrict__ starts,
                                                  int32_t* __restrict__ ctrl) {
-    const int ns = ctrl[0];
+    const int ns = ctrl[1];
     for (int t = threadIdx.x; t < ns; t += blockDim.x) {
         int32_t p = starts[t];
+        // a listed node's children are complete: both loaded
+        ChildData cur = finish_start<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
         while (true) {
-            finish_node<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
             if (p == 0) {
                 ctrl[kMaxRounds + 1] = 1;
                 break;
@@ -288,6 +366,7 @@ __global__ __launch_bounds__(1024) void k_bu_top(const uint4* __restrict__ parts
             wg_release();
             if (atomicAdd(&flags[par], 1) != 1) break;
             wg_acquire();
+            cur = finish_up<PREC>(par, p, cur, parts, geo, kids, sums, mono, qp, qm);
             p = par;
         }
     }
@@ -390,18 +469,25 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
         e = launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
         if (e != hipSuccess) return e;
         const unsigned nbLocal = (unsigned)((n + kBuChunk - 1) / kBuChunk);
+        const unsigned nbSuper = (unsigned)((n + kBuSuper - 1) / kBuSuper);
         const double qp = c.p.g_constant * c.m_plus, qm = c.p.g_constant * c.m_minus;
         NodeSum* sums = (NodeSum*)c.node_sum;
         if (c.p.precision == JTP_FP64) {
             k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
+            k_bu_mid<JTP_FP64><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+                                                                    c.flags, sums, c.node_mono, qp, qm, c.frontier0,
+                                                                    c.frontier1, ctrl);
             k_bu_top<JTP_FP64><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
-                                                              sums, c.node_mono, qp, qm, c.frontier0, ctrl);
+                                                              sums, c.node_mono, qp, qm, c.frontier1, ctrl);
         } else {
             k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
+            k_bu_mid<JTP_FP32><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+                                                                    c.flags, sums, c.node_mono, qp, qm, c.frontier0,
+                                                                    c.frontier1, ctrl);
             k_bu_top<JTP_FP32><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
-                                                              sums, c.node_mono, qp, qm, c.frontier0, ctrl);
+                                                              sums, c.node_mono, qp, qm, c.frontier1, ctrl);
         }
     }
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * kGroupLevelBits, kNCell4, c.cell4, s);

@@ -311,6 +311,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             // 2 take the bucket's particles, 3 open
             int act = 0, napp = 0, napm = 0, nmono_p = 0;
             int32_t first = 0;
+            bool has0 = false, has1 = false;  // the accepted node's monopoles with nonzero charge
+            E pre0, pre1;                     // FP32: their list entries, built when accepted
             if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
                 if (lane == 0) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skip it
             } else if (has) {
@@ -330,6 +332,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     nmn[2] = nmx[2] = pr.z;
                     cnt = 1;
                 }
+                // 32-bit integer box arithmetic: |centre offset| <= 2^31, group half-extent <= 2^27 (a
+                // group lies in one level-4 cell) and node half-extent <= 2^31, so adc, t and gap fit
                 R gap2 = 0, far2 = 0, ext = 0;
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) {
@@ -337,12 +341,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     const uint32_t nc = nmn[ax] + e / 2;
                     const uint32_t nh = e / 2 + (e & 1u);
                     const int32_t dc = (int32_t)(nc - gc[ax]);
-                    const int64_t adc = dc < 0 ? -(int64_t)dc : (int64_t)dc;
-                    int64_t gap = adc - (int64_t)gh[ax] - (int64_t)nh;
-                    gap = gap > 0 ? gap : 0;
+                    const uint32_t adc = dc < 0 ? 0u - (uint32_t)dc : (uint32_t)dc;
+                    const uint32_t t = gh[ax] + nh;
+                    const uint32_t gap = adc > t ? adc - t : 0u;
                     const R gr_ = (R)gap * scale;
                     gap2 += gr_ * gr_;
-                    const R fr_ = (R)(adc + (int64_t)gh[ax] + (int64_t)nh) * scale;
+                    const R fr_ = ((R)adc + (R)t) * scale;
                     far2 += fr_ * fr_;
                     const R er = (R)e * scale;
                     ext = er > ext ? er : ext;
@@ -351,7 +355,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     const bool inside = far2 < rcut2;
                     const R th2 = inside ? theta2 : thetac2;
                     if (node >= 0 && ext * ext < th2 * gap2) {
-                        const int nq = (mono_q(mono, node, 0) != (R)0 ? 1 : 0) + (mono_q(mono, node, 1) != (R)0 ? 1 : 0);
+                        if (sizeof(R) == 4) {
+                            pre0 = mono_entry(mono, node, 0, gc, scale);
+                            pre1 = mono_entry(mono, node, 1, gc, scale);
+                            has0 = pre0.w != (R)0;
+                            has1 = pre1.w != (R)0;
+                        } else {
+                            has0 = mono_q(mono, node, 0) != (R)0;
+                            has1 = mono_q(mono, node, 1) != (R)0;
+                        }
+                        const int nq = (has0 ? 1 : 0) + (has1 ? 1 : 0);
                         if (sizeof(R) == 4 && inside && MODE != JTP_SPLIT_POLY4) {
                             act = 4;
                             napp = nq;
@@ -374,14 +387,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const int offp = lp + (incl & 0xffff) - napp;
             const int offm = lm + (incl >> 16) - napm;
             const int totl = __shfl(incl, 63, 64);
-            if (act == 1) {
-                int o = kListCap - 1 - offm;
-                if (mono_q(mono, node, 0) != (R)0) L[o--] = mono_entry(mono, node, 0, gc, scale);
-                if (mono_q(mono, node, 1) != (R)0) L[o] = mono_entry(mono, node, 1, gc, scale);
-            } else if (act == 4) {
-                int o = offp;
-                if (mono_q(mono, node, 0) != (R)0) L[o++] = mono_entry(mono, node, 0, gc, scale);
-                if (mono_q(mono, node, 1) != (R)0) L[o] = mono_entry(mono, node, 1, gc, scale);
+            if (act == 1 || act == 4) {  // tail-form monopoles grow down from the top, cut-law ones up
+                int o = act == 1 ? kListCap - 1 - offm : offp;
+                const int d = act == 1 ? -1 : 1;
+                if (sizeof(R) == 4) {
+                    if (has0) {
+                        L[o] = pre0;
+                        o += d;
+                    }
+                    if (has1) L[o] = pre1;
+                } else {
+                    if (has0) {
+                        L[o] = mono_entry(mono, node, 0, gc, scale);
+                        o += d;
+                    }
+                    if (has1) L[o] = mono_entry(mono, node, 1, gc, scale);
+                }
             } else if (act == 2) {
                 for (int t = 0; t < napp; ++t) {
                     const uint4 pr = a.parts[first + t];

@@ -96,3 +96,36 @@ def test_compat_vs_reference_cpu(jtp, orc):
     e = _rel(acc, a_ref)
     print("compat vs reference CPU: median %.2e max %.2e" % (np.median(e), np.max(e)))
     assert np.median(e) < 1e-9 and np.max(e) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_config0_full_force_and_step(jtp, orc, precision):
+    """The small configuration: N = 2 x 64^3 lattice + Zel'dovich (P(k) ~ k^-2.5, rms 0.3 cell, m- on the
+    half-cell-offset lattice with the opposite displacement), L = 250, PM 64^3, r_s = 1.25 cells,
+    r_cut = 4.5 r_s, theta = 0.5, leaf 8, m+ = 1, |m-| = 19: one force evaluation against the FP64 Ewald
+    sum on 300 targets, then one DKD step (finite, momentum of the kick conserved)."""
+    print(f"case: config 0 precision={precision}", flush=True)
+    w = synth.lattice_zeldovich(n_side=64, box_size=250.0, n_cells=64, seed=42, leaf_size=8)
+    assert w.n == 2 * 64 ** 3
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    p = jtp.default_params(w.box_size, w.n_cells)
+    p.r_s, p.r_cut, p.softening, p.leaf_size, p.precision = w.r_s, w.r_cut, w.softening, 8, precision
+    ctx = jtp.ForceContext(p, w.m_plus, w.m_minus_abs)
+    acc = ctx.compute_forces(pos, w.species).astype(np.float64)
+    rng = np.random.default_rng(7)
+    tg = rng.choice(w.n, 300, replace=False)
+    ref = orc.ewald(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, targets=tg,
+                    rc=w.box_size / 3)
+    e = _rel(acc[tg], ref)
+    print("config 0 vs Ewald: median %.2e p99 %.2e" % (np.median(e), np.percentile(e, 99)))
+    assert np.median(e) < 1e-2 and np.percentile(e, 99) < 1e-1
+    rt = np.float64 if precision == 1 else np.float32
+    n = w.n
+    dp, dv = jtp.DeviceArray.from_host(pos), jtp.DeviceArray.from_host(np.zeros((n, 3), rt))
+    ds, da = jtp.DeviceArray.from_host(w.species), jtp.DeviceArray((n, 3), rt)
+    ctx.step_device(n, dp, dv, ds, da, 0.01, scheme=jtp.STEP_DKD)
+    v = dv.download().astype(np.float64)
+    assert np.all(np.isfinite(v))
+    m = np.where(w.species > 0, w.m_plus, w.m_minus_abs)[:, None]
+    mom = (m * v).sum(axis=0)
+    assert np.abs(mom).max() < 1e-4 * (m * np.abs(v)).sum()
