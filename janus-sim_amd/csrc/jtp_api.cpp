@@ -285,6 +285,10 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)4 * G * rs);
     if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
+    if (e == hipSuccess) e = dalloc(c.cellnode, (size_t)4096);
+    c.start_level = 2;  // L/4 > L/5 >= r_cut
+    while (c.start_level < kGroupLevelBits && p->box_size / (double)(1 << (c.start_level + 1)) >= p->r_cut)
+        ++c.start_level;
     if (e == hipSuccess) e = dalloc(c.dev_counters, kCtrTotal);
     if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
@@ -320,7 +324,7 @@ int jtp_destroy(jtp_ctx* h) {
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     if (c.graph.capture_stream) (void)hipStreamDestroy(c.graph.capture_stream);
     void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.grad, c.grad2, c.spec, c.twiddle, c.gtab, c.cell_start,
-                    c.cell4, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch};
+                    c.cell4, c.cellnode, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);

@@ -125,6 +125,8 @@ struct Ctx {
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
     int2 *groups = nullptr;
     int32_t *cell4 = nullptr;        // level-4 cell offsets (4097)
+    int32_t *cellnode = nullptr;     // node holding each Morton cell of level start_level (<= 4096)
+    int start_level = 4;             // finest level whose cell edge L / 2^level is >= r_cut (2..4)
     int32_t *dev_counters = nullptr; // kCtr* layout
     unsigned long long *dev_inter = nullptr;  // interaction counter
     // PM

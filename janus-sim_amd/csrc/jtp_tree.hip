@@ -406,6 +406,32 @@ hipError_t launch_cell_start(const uint64_t* keys, int64_t n, int shift, int64_t
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ start cells of the walk
+// The keys of one Morton cell of level `lvl` (8^lvl cells, edge L / 2^lvl) are a contiguous sorted
+// range [s, e), and in a binary radix tree they form exactly one subtree: the node whose range is
+// [s, e - 1] (a leaf if e - s == 1).  A Karras node's index is one of its range ends, so that node
+// is s or e - 1.  cellnode[c] = that node (~leaf for a single particle, kNoNode if the cell is
+// empty); the walk seeds each group's traversal with the 27 cells around the group's cell.
+constexpr int32_t kNoNode = (int32_t)0x80000000;
+
+__global__ void k_cell_nodes(const int32_t* __restrict__ cell4, int lvl, int64_t nI, const uint4* __restrict__ geo,
+                             int32_t* __restrict__ cellnode, int32_t* __restrict__ counters) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncell = 1 << (3 * lvl);
+    if (c >= ncell) return;
+    const int f = 1 << (3 * (kGroupLevelBits - lvl));  // level-4 cells per cell of this level
+    const int32_t s = cell4[c * f], e = cell4[(c + 1) * f];
+    int32_t node = kNoNode;
+    if (e - s == 1) {
+        node = ~s;
+    } else if (e - s >= 2) {
+        if (s < nI && (int32_t)geo[2 * s].w == s && (int32_t)geo[2 * s + 1].w == e - 1) node = s;
+        else if (e - 1 < nI && (int32_t)geo[2 * (e - 1)].w == s && (int32_t)geo[2 * (e - 1) + 1].w == e - 1) node = e - 1;
+        else counters[kCtrErr] = 4;  // no node holds exactly this cell: malformed hierarchy
+    }
+    cellnode[c] = node;
+}
+
 // ------------------------------------------------------------------ walk groups
 // Groups are runs of <= gs consecutive sorted particles inside one level-4 Morton cell
 // (edge L/16).  goff[c] = index of the first group of cell c; counters[0] = number of groups.
@@ -497,6 +523,11 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
     if (c.n_groups_max > 0)
         k_groups<<<dim3((unsigned)((c.n_groups_max + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
             c.cell4, goff, c.p.group_size, c.n_groups_max, c.groups);
+    if (n >= 2) {
+        const int lvl = c.start_level;
+        k_cell_nodes<<<dim3(((1u << (3 * lvl)) + 255) / 256), dim3(256), 0, s>>>(c.cell4, lvl, n - 1, c.node_geo,
+                                                                              c.cellnode, c.dev_counters);
+    }
     return hipGetLastError();
 }
 

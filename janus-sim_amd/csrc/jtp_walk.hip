@@ -31,6 +31,7 @@ constexpr int kStackPhys = kStackCap + 128;
 constexpr int kListCap = 384;
 constexpr int kFlushAt = 192;
 constexpr int kPad = 4;  // list slots kept free for the lane-sharing padding
+constexpr int32_t kNoNodeW = (int32_t)0x80000000;  // empty start cell
 
 struct WalkArgs {
     const uint4* parts;
@@ -46,6 +47,8 @@ struct WalkArgs {
     int64_t n;
     int32_t root;
     int32_t leaf;
+    const int32_t* cellnode;  // node of each Morton cell of level start_level (nullptr: start at the root)
+    int32_t start_level;
     double theta_cut2;  // theta_cut^2 for nodes whose box reaches beyond r_cut
     int64_t max_steps;
     double scale;   // length per fixed-point unit, L / 2^32
@@ -178,7 +181,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2, thetac2 = (R)a.theta_cut2;
     const R rcut2s = (R)(a.rcut2 + a.eps2);
     const R qpart[2] = {(R)a.qm, (R)a.qp};
-    const int maxapp = a.leaf > 2 ? a.leaf : 2;
+    // entries one popped node can append, rounded up to a power of two so the pop bound is a shift
+    int app_shift = 1;
+    while ((1 << app_shift) < a.leaf) ++app_shift;
     const uint64_t lt = lanemask_lt();
     unsigned long long my_inter = 0, my_mono = 0;
     const int32_t n_groups = a.counters[kCtrNGroups];
@@ -230,7 +235,25 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         int sp = 1, lp = 0, lm = 0;
         int mono_inside = 0;  // monopole entries placed on the particle side since the last flush
         int64_t steps = 0;
-        if (lane == 0) S[0] = a.root;
+        if (a.cellnode) {
+            // seed with the 27 cells around the group's cell: a cell's edge is >= r_cut, so every
+            // source within r_cut of the group lies in them (the group lies inside one cell)
+            const int lv = a.start_level, sh = 32 - lv, msk = (1 << lv) - 1;
+            int32_t nd = kNoNodeW;
+            if (lane < 27) {
+                const int cx = (int)((gc[0] >> sh) + (lane % 3) - 1) & msk;
+                const int cy = (int)((gc[1] >> sh) + ((lane / 3) % 3) - 1) & msk;
+                const int cz = (int)((gc[2] >> sh) + (lane / 9) - 1) & msk;
+                const uint64_t cm = spread3_21((uint32_t)cx) | (spread3_21((uint32_t)cy) << 1) |
+                                    (spread3_21((uint32_t)cz) << 2);
+                nd = a.cellnode[cm];
+            }
+            const uint64_t have = __ballot(nd != kNoNodeW);
+            if (nd != kNoNodeW) S[__popcll(have & lt)] = nd;
+            sp = __popcll(have);
+        } else if (lane == 0) {
+            S[0] = a.root;
+        }
         wave_lds_fence();
 
         // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
@@ -297,7 +320,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 break;
             }
             int k = sp < 64 ? sp : 64;
-            const int kl = (kListCap - 2 * kPad - lp - lm) / maxapp;
+            const int kl = (kListCap - 2 * kPad - lp - lm) >> app_shift;
             k = k < kl ? k : kl;
             const int ks = kStackCap - sp > 1 ? kStackCap - sp : 1;
             k = k < ks ? k : ks;
@@ -312,7 +335,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             int act = 0, napp = 0, napm = 0, nmono_p = 0;
             int32_t first = 0;
             bool has0 = false, has1 = false;  // the accepted node's monopoles with nonzero charge
-            E pre0, pre1;                     // FP32: their list entries, built when accepted
+            E pre0 = {}, pre1 = {};           // FP32: their list entries, built when accepted
             if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
                 if (lane == 0) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skip it
             } else if (has) {
@@ -426,8 +449,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 S[sp + 2 * rank + 1] = kd.x;
             }
             if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
-            lp += totl & 0xffff;
-            lm += totl >> 16;
+            // the counts are wave-uniform: keep them in scalar registers
+            lp = __builtin_amdgcn_readfirstlane(lp + (totl & 0xffff));
+            lm = __builtin_amdgcn_readfirstlane(lm + (totl >> 16));
+            sp = __builtin_amdgcn_readfirstlane(sp);
             mono_inside += wave_sum(nmono_p);
             wave_lds_fence();
             if (lp + lm >= kFlushAt) {
@@ -525,6 +550,8 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     a.n = n;
     a.root = n >= 2 ? 0 : ~0;
     a.leaf = c.p.leaf_size;
+    a.cellnode = n >= 2 ? c.cellnode : nullptr;
+    a.start_level = c.start_level;
     a.max_steps = 4 * n + 1024;
     const double tc = c.p.theta_cut < c.p.theta ? c.p.theta_cut : c.p.theta;
     a.theta_cut2 = tc * tc;  // each step pops >= 1 of the <= 2n-1 nodes
