@@ -68,54 +68,107 @@ __device__ inline void shifted_weights(R f, R w[3]) {
     w[2] = f < h ? (R)0 : f - h;        // node at the centre of the cell above
 }
 
-template <typename R, bool SHIFTED>
+template <typename R>
 __global__ __launch_bounds__(256) void k_dense_reduce(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
                                                       const int32_t* __restrict__ list,
                                                       const int32_t* __restrict__ count, int logG, R qp, R qm,
                                                       R* __restrict__ corner) {
-    constexpr int NC = SHIFTED ? 27 : 8;
     const int lane = threadIdx.x & 63;
     const int32_t nd = *count;
     for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd;
          k += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int32_t c = list[k];
         const int32_t s = cs[c], e = cs[c + 1];
-        R acc[NC];
+        R acc[8];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) acc[q] = 0;
+        for (int q = 0; q < 8; ++q) acc[q] = 0;
         for (int32_t p = s + lane; p < e; p += 64) {
             const uint4 r = parts[p];
             const R q = (int32_t)r.w > 0 ? qp : qm;
-            if (SHIFTED) {
-                R wx[3], wy[3], wz[3];
-                shifted_weights<R>(frac_of<R>(r.x, logG), wx);
-                shifted_weights<R>(frac_of<R>(r.y, logG), wy);
-                shifted_weights<R>(frac_of<R>(r.z, logG), wz);
+            R w[8];
+            cic_corners<R>(r, logG, q, w);
 #pragma unroll
-                for (int cz = 0; cz < 3; ++cz)
-#pragma unroll
-                    for (int cy = 0; cy < 3; ++cy)
-#pragma unroll
-                        for (int cx = 0; cx < 3; ++cx) acc[cx + 3 * cy + 9 * cz] += q * (wx[cx] * wy[cy] * wz[cz]);
-            } else {
-                R w[8];
-                cic_corners<R>(r, logG, q, w);
-#pragma unroll
-                for (int q2 = 0; q2 < 8; ++q2) acc[q2] += w[q2];
-            }
+            for (int q2 = 0; q2 < 8; ++q2) acc[q2] += w[q2];
         }
 #pragma unroll
-        for (int q = 0; q < NC; ++q) {
+        for (int q = 0; q < 8; ++q) {
             R v = acc[q];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
             acc[q] = v;
         }
-        if (lane < NC) {
+        if (lane < 8) {
             R v = acc[0];
 #pragma unroll
-            for (int q = 1; q < NC; ++q) v = lane == q ? acc[q] : v;
-            corner[NC * (int64_t)k + lane] = v;
+            for (int q = 1; q < 8; ++q) v = lane == q ? acc[q] : v;
+            corner[8 * (int64_t)k + lane] = v;
+        }
+    }
+}
+
+// Both grids of an interlaced deposit in one pass over a dense cell's particles: the 8 node-grid
+// corner sums (as k_dense_reduce) and the 27 sums for the shifted grid's nodes at the centres of
+// cells c-1, c, c+1 per axis, index (ex+1) + 3 (ey+1) + 9 (ez+1) for the centre of cell c + e.
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_reduce_both(const uint4* __restrict__ parts,
+                                                           const int32_t* __restrict__ cs,
+                                                           const int32_t* __restrict__ list,
+                                                           const int32_t* __restrict__ count, int logG, R qp, R qm,
+                                                           R* __restrict__ corner8, R* __restrict__ corner27) {
+    const int lane = threadIdx.x & 63;
+    const int32_t nd = *count;
+    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd;
+         k += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const int32_t c = list[k];
+        const int32_t s = cs[c], e = cs[c + 1];
+        R a8[8], a27[27];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a8[q] = 0;
+#pragma unroll
+        for (int q = 0; q < 27; ++q) a27[q] = 0;
+        for (int32_t p = s + lane; p < e; p += 64) {
+            const uint4 r = parts[p];
+            const R q = (int32_t)r.w > 0 ? qp : qm;
+            R w[8];
+            cic_corners<R>(r, logG, q, w);
+#pragma unroll
+            for (int q2 = 0; q2 < 8; ++q2) a8[q2] += w[q2];
+            R wx[3], wy[3], wz[3];
+            shifted_weights<R>(frac_of<R>(r.x, logG), wx);
+            shifted_weights<R>(frac_of<R>(r.y, logG), wy);
+            shifted_weights<R>(frac_of<R>(r.z, logG), wz);
+#pragma unroll
+            for (int cz = 0; cz < 3; ++cz)
+#pragma unroll
+                for (int cy = 0; cy < 3; ++cy)
+#pragma unroll
+                    for (int cx = 0; cx < 3; ++cx) a27[cx + 3 * cy + 9 * cz] += q * (wx[cx] * wy[cy] * wz[cz]);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            R v = a8[q];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            a8[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            R v = a27[q];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            a27[q] = v;
+        }
+        if (lane < 8) {
+            R v = a8[0];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) v = lane == q ? a8[q] : v;
+            corner8[8 * (int64_t)k + lane] = v;
+        }
+        if (lane < 27) {
+            R v = a27[0];
+#pragma unroll
+            for (int q = 1; q < 27; ++q) v = lane == q ? a27[q] : v;
+            corner27[27 * (int64_t)k + lane] = v;
         }
     }
 }
@@ -124,14 +177,16 @@ template <typename R> struct Vec4T;
 template <> struct Vec4T<float> { using T = float4; };
 template <> struct Vec4T<double> { using T = double4; };
 
-// Shifted grid (interlacing): node n' sits at the centre of cell m = n' - 1 and gathers the
-// particles of cells m-1, m, m+1 per axis with weight 1 - |x - x_node| / h, i.e. per axis
-// (d = cell offset from m, f = fraction in that cell): d = -1: max(0, f - 1/2); d = 0: 1 - |f - 1/2|;
-// d = +1: max(0, 1/2 - f).
+// Both grids in one gather: the thread of cell m = (i, j, k) reads the particles of cells m-1..m+1
+// per axis once and sums (a) the node-grid node (i, j, k) from cells i-1, i per axis, in the order
+// of k_deposit, and (b) the interlaced node at the centre of cell m, written at (i+1, j+1, k+1): per
+// axis, with d the cell offset from m and f the fraction in that cell, weight max(0, f - 1/2) for
+// d = -1, 1 - |f - 1/2| for d = 0 and max(0, 1/2 - f) for d = +1 (shifted_weights).
 template <typename R>
-__global__ void k_deposit_shifted(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
-                                  const int32_t* __restrict__ slot, const R* __restrict__ corner27, int logG, R qp,
-                                  R qm, R* __restrict__ rho2) {
+__global__ void k_deposit_both(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                               const int32_t* __restrict__ slot, const R* __restrict__ corner8,
+                               const R* __restrict__ corner27, int logG, R qp, R qm, R* __restrict__ rho,
+                               R* __restrict__ rho2) {
     const int64_t G = 1ll << logG;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= G * G * G) return;
@@ -139,34 +194,44 @@ __global__ void k_deposit_shifted(const uint4* __restrict__ parts, const int32_t
     const int32_t j = (int32_t)compact3_21((uint64_t)m >> 1);
     const int32_t k = (int32_t)compact3_21((uint64_t)m >> 2);
     const int32_t msk = (int32_t)(G - 1);
-    R acc = 0;
+    R acc1 = 0, acc2 = 0;
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
+                const bool both = dx <= 0 && dy <= 0 && dz <= 0;  // also a source cell of node (i, j, k)
                 const uint32_t ci = (uint32_t)((i + dx) & msk), cj = (uint32_t)((j + dy) & msk),
                                ck = (uint32_t)((k + dz) & msk);
                 const uint64_t cm = spread3_21(ci) | (spread3_21(cj) << 1) | (spread3_21(ck) << 2);
                 const int32_t s = cs[cm], e = cs[cm + 1];
                 if (e - s > kDenseCell) {
-                    // this node is the centre of cell (cell - d): shifted index e = -d
-                    acc += corner27[27 * (int64_t)slot[cm] + (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz)];
+                    const int64_t sl = slot[cm];
+                    acc2 += corner27[27 * sl + (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz)];
+                    if (both) acc1 += corner8[8 * sl + (-dx) + 2 * (-dy) + 4 * (-dz)];
                     continue;
                 }
                 for (int32_t p = s; p < e; ++p) {
                     const uint4 r = parts[p];
-                    R wx[3], wy[3], wz[3];
-                    shifted_weights<R>(frac_of<R>(r.x, logG), wx);
-                    shifted_weights<R>(frac_of<R>(r.y, logG), wy);
-                    shifted_weights<R>(frac_of<R>(r.z, logG), wz);
+                    const R fx = frac_of<R>(r.x, logG), fy = frac_of<R>(r.y, logG), fz = frac_of<R>(r.z, logG);
                     const R q = (int32_t)r.w > 0 ? qp : qm;
-                    acc += q * (wx[1 - dx] * wy[1 - dy] * wz[1 - dz]);
+                    R wx[3], wy[3], wz[3];
+                    shifted_weights<R>(fx, wx);
+                    shifted_weights<R>(fy, wy);
+                    shifted_weights<R>(fz, wz);
+                    acc2 += q * (wx[1 - dx] * wy[1 - dy] * wz[1 - dz]);
+                    if (both) {
+                        const R ux = dx == 0 ? (R)1 - fx : fx;
+                        const R uy = dy == 0 ? (R)1 - fy : fy;
+                        const R uz = dz == 0 ? (R)1 - fz : fz;
+                        acc1 += q * (ux * uy * uz);
+                    }
                 }
             }
         }
     }
+    rho[((int64_t)k * G + j) * G + i] = acc1;
     const int64_t ii = (i + 1) & msk, jj = (j + 1) & msk, kk = (k + 1) & msk;
-    rho2[(kk * G + jj) * G + ii] = acc;
+    rho2[(kk * G + jj) * G + ii] = acc2;
 }
 
 // Thread per grid node, nodes enumerated in Morton order so that the eight
@@ -456,29 +521,32 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     if (c.p.precision == JTP_FP64) {
         double* c8 = (double*)c.dense_corner;
         double* c27 = c8 + 8 * nd_cap;
-        k_dense_reduce<double, false><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                      c.m_plus, -c.m_minus, c8);
-        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, c.m_plus,
-                                                            -c.m_minus, (double*)c.rho);
         if (c.p.interlace) {
-            k_dense_reduce<double, true><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                         c.m_plus, -c.m_minus, c27);
-            k_deposit_shifted<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
-                                                                        c.logG, c.m_plus, -c.m_minus, (double*)c.rho2);
+            k_dense_reduce_both<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                        c.m_plus, -c.m_minus, c8, c27);
+            k_deposit_both<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
+                                                                     c.logG, c.m_plus, -c.m_minus, (double*)c.rho,
+                                                                     (double*)c.rho2);
+        } else {
+            k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                          c.m_plus, -c.m_minus, c8);
+            k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG,
+                                                                c.m_plus, -c.m_minus, (double*)c.rho);
         }
     } else {
         float* c8 = (float*)c.dense_corner;
         float* c27 = c8 + 8 * nd_cap;
         const float qp = (float)c.m_plus, qm = (float)-c.m_minus;
-        k_dense_reduce<float, false><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG, qp,
-                                                                     qm, c8);
-        k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, qp, qm,
-                                                           (float*)c.rho);
         if (c.p.interlace) {
-            k_dense_reduce<float, true><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                        qp, qm, c27);
-            k_deposit_shifted<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c27,
-                                                                       c.logG, qp, qm, (float*)c.rho2);
+            k_dense_reduce_both<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                       qp, qm, c8, c27);
+            k_deposit_both<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
+                                                                    c.logG, qp, qm, (float*)c.rho, (float*)c.rho2);
+        } else {
+            k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
+                                                                         qp, qm, c8);
+            k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, qp,
+                                                               qm, (float*)c.rho);
         }
     }
     return hipGetLastError();
