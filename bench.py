@@ -210,6 +210,86 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def run_multistep(args):
+    """Large single box, multi-step: N = 2 x 256^3 + 7.2e6 m+ (40,754,432), L = 300, PM 512^3, theta = 0.5,
+    the clustered recipe with 8000 halos; 100 KDK steps of the FP32 form (hipGraph replay, FP64 drift)
+    and the same run in the FP64 form, from rest.  Reports the per-step time split, the HBM footprint
+    and the relative drift of total energy (kinetic + potential, FP64 reductions) and momentum."""
+    n_plus = 256 ** 3 + 7_200_000
+    n_total = n_plus + 256 ** 3
+    log(f"generating the multistep workload (N={n_total}, PM 512^3) ...")
+    t0 = time.perf_counter()
+    w = synth.clustered(n_total=n_total, n_plus=n_plus, box_size=300.0, n_cells=512, n_halos=8000, seed=11)
+    log(f"workload generated in {time.perf_counter() - t0:.1f} s (N={w.n})")
+    n = w.n
+    pos0 = j.to_fixed(w.pos, w.box_size)
+    steps = args.multisteps
+
+    def params(prec):
+        p = j.default_params(w.box_size, w.n_cells)
+        p.r_s, p.r_cut, p.theta, p.softening, p.leaf_size = w.r_s, w.r_cut, w.theta, w.softening, w.leaf_size
+        p.theta_cut = 0.7 * w.theta
+        p.precision = prec
+        return p
+
+    energy_ctx = j.ForceContext(params(j.FP64), w.m_plus, w.m_minus_abs)
+
+    energy_ms = []
+
+    def energy(dp, ds, dv, ctx):
+        t = time.perf_counter()
+        wpot = energy_ctx.potential_energy(n, dp, ds)
+        energy_ms.append((time.perf_counter() - t) * 1e3)
+        d = ctx.diagnostics(n, dv, ds)
+        mom = d["momentum"].sum(axis=0)
+        return wpot + d["kinetic"].sum(), mom, wpot
+
+    out = {"metric": "TreePM KDK steps at N=4.08e7, PM 512^3, 1 GPU", "n_particles": n, "steps": steps,
+           "pm_grid": 512, "box": 300.0, "n_halos": 8000, "forms": {}}
+    dt = None
+    for prec, label in ((j.FP32, "fp32"), (j.FP64, "fp64")):
+        rt = np.float64 if prec == j.FP64 else np.float32
+        free0, total = j.device_memory()
+        ctx = j.ForceContext(params(prec), w.m_plus, w.m_minus_abs)
+        dp, ds = j.DeviceArray.from_host(pos0), j.DeviceArray.from_host(w.species)
+        dv, da = j.DeviceArray.from_host(np.zeros((n, 3), rt)), j.DeviceArray((n, 3), rt)
+        ctx.compute_forces_device(n, dp, ds, da)
+        free1, _ = j.device_memory()
+        if dt is None:
+            a = np.linalg.norm(da.download().astype(np.float64), axis=1)
+            h = w.box_size / w.n_cells
+            dt = float(np.sqrt(2.0 * h / np.percentile(a, 99)) / steps)  # p99 particle moves ~1 cell in total
+        e0, p0, w0 = energy(dp, ds, dv, ctx)
+        ctx.step_device(n, dp, dv, ds, da, dt, scheme=j.STEP_KDK, graph=False)  # one timed eager step
+        split = ctx.stats()
+        j.load_library().jtp_device_synchronize()
+        t = time.perf_counter()
+        for k in range(steps - 1):
+            ctx.step_device(n, dp, dv, ds, da, dt, scheme=j.STEP_KDK, graph=True)
+            if (k + 1) % 20 == 0:
+                log(f"  {label}: step {k + 2}/{steps}")
+        j.load_library().jtp_device_synchronize()
+        ms = (time.perf_counter() - t) / max(1, steps - 1) * 1e3
+        e1, p1, w1 = energy(dp, ds, dv, ctx)
+        d = ctx.diagnostics(n, dv, ds)
+        mscale = float(np.abs(d["momentum"]).sum())
+        out["forms"][label] = {
+            "ms_per_step": ms, "dt": dt, "first_step_split_ms": split,
+            "hbm_footprint_bytes": int(free0 - free1) + 0,
+            "energy_start": e0, "energy_end": e1, "potential_start": w0, "potential_end": w1,
+            "kinetic_end": float(d["kinetic"].sum()),
+            "rel_energy_drift": abs(e1 - e0) / abs(e0),
+            "momentum_end": p1.tolist(), "rel_momentum_drift": float(np.abs(p1 - p0).max() / max(mscale, 1e-300)),
+        }
+        log(f"  {label}: {ms:.1f} ms/step, dE/E = {out['forms'][label]['rel_energy_drift']:.2e}")
+        ctx.close()
+        for arr in (dp, ds, dv, da):
+            arr.free()
+    out["fp32_over_fp64"] = out["forms"]["fp64"]["ms_per_step"] / out["forms"]["fp32"]["ms_per_step"]
+    out["ms_potential_energy"] = min(energy_ms[1:]) if len(energy_ms) > 1 else None  # FP64, theta = 0 short range
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,7 +299,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="clustered_22M", choices=sorted(WORKLOADS) + ["ensemble_c3"])
+    ap.add_argument("--workload", default="clustered_22M",
+                    choices=sorted(WORKLOADS) + ["ensemble_c3", "multistep_41M"])
+    ap.add_argument("--multisteps", type=int, default=100, help="KDK steps of the multistep_41M workload")
     ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's streaming copy kernel on MIB MiB, 5 times (HBM counter calibration)")
@@ -246,6 +328,9 @@ def main():
 
     if args.workload == "ensemble_c3":
         run_ensemble(args)
+        return
+    if args.workload == "multistep_41M":
+        run_multistep(args)
         return
 
     t0 = time.perf_counter()

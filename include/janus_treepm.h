@@ -166,6 +166,16 @@ typedef struct jtp_diag {
 int  jtp_diagnostics(jtp_ctx* ctx, int64_t n, const void* d_vel, const int8_t* d_species, jtp_diag* out,
                      void* stream);
 
+/* Total potential energy of the two-species system, FP64 (for energy-drift diagnostics; needs a
+ * JTP_FP64 context with JTP_SPLIT_ERFC_GAUSS, tree and PM enabled).  Runs one force evaluation whose
+ * walk also sums the short-range potential -sum_j q_j erfc(rho/2r_s)/rho, interpolates the mesh
+ * potential at the particles and removes the mesh self-energy of each particle exactly:
+ * W = 1/2 sum_i |m_i| sigma_i (Phi_mesh(x_i) + Phi_short_i) - 1/2 sum_i m_i^2 S_i.
+ * (The reference estimates PE from the grid only: potential_energy_binding_sampled,
+ * nbody_gpu_twopass.rs:5280-5307.) */
+int  jtp_potential_energy(jtp_ctx* ctx, int64_t n, const uint32_t* d_pos_fixed, const int8_t* d_species,
+                          double* energy, void* stream);
+
 /* ---- ensembles of independent boxes (no counterpart in the reference, which runs one box per
  * process; the batch replaces a loop of step_treepm_gpu / TreePMForce calls over boxes).
  * Each box has its own parameters (L, G, r_s, ...) and species masses and its own buffers.
@@ -217,6 +227,7 @@ int   jtp_copy_d2h(void* h_dst, const void* d_src, size_t bytes);
 int   jtp_copy_d2d(void* d_dst, const void* d_src, size_t bytes);
 int   jtp_device_synchronize(void);
 int   jtp_device_count(int* out);
+int   jtp_device_memory(size_t* free_bytes, size_t* total_bytes);  /* hipMemGetInfo of the current device */
 
 #ifdef __cplusplus
 }

@@ -324,7 +324,8 @@ int jtp_destroy(jtp_ctx* h) {
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     if (c.graph.capture_stream) (void)hipStreamDestroy(c.graph.capture_stream);
     void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.grad, c.grad2, c.spec, c.twiddle, c.gtab, c.cell_start,
-                    c.cell4, c.cellnode, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch};
+                    c.cell4, c.cellnode, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch,
+                    c.pot_short, c.acc_scratch, c.kself};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);
@@ -355,7 +356,7 @@ int jtp_set_species_mass(jtp_ctx* h, double m_plus, double m_minus_abs) {
 // Enqueues one force evaluation and the copies of its status words to pinned host memory; no
 // synchronisation, so it can be captured into a graph.  `timed` brackets the phases with events.
 static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t* d_species, void* d_acc,
-                          hipStream_t s, bool timed) {
+                          hipStream_t s, bool timed, double* pot = nullptr) {
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[0], s));
     JTP_HIP(&c, launch_fill_i32((int32_t*)c.dev_inter, 4, 0, s));
     JTP_HIP(&c, launch_morton_keys(d_pos, c.keys0, c.vals0, n, s));
@@ -387,7 +388,7 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[5], s));
     if (fork) JTP_HIP(&c, hipStreamWaitEvent(s, c.join_ev, 0));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[7], s));
-    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s));
+    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s, pot));
     else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[6], s));
     // status: interaction counts, group count, walk error flag, root flag
@@ -573,6 +574,89 @@ int jtp_diagnostics(jtp_ctx* h, int64_t n, const void* d_vel, const int8_t* d_sp
         out->momentum[1][k] = host[5 + k];
     }
     out->pm_energy = c.p.interlace ? 0.25 * (host[8] + host[9]) : 0.5 * host[8];
+    return JTP_OK;
+}
+
+int jtp_potential_energy(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_t* d_species, double* w_out,
+                         void* stream) {
+    if (!h || !w_out) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    if (c.p.precision != JTP_FP64 || c.p.split_mode != JTP_SPLIT_ERFC_GAUSS || !c.p.tree_enabled || !c.p.pm_enabled)
+        return fail(&c, JTP_ERR_ARG, "potential energy needs an FP64 context with the erfc + Gaussian split, tree and PM");
+    if (n < 0 || n > (int64_t)0x7fffffff - 8192) return fail(&c, JTP_ERR_ARG, "n out of range");
+    if (n > 0 && (!d_pos || !d_species)) return fail(&c, JTP_ERR_ARG, "null device pointer");
+    *w_out = 0;
+    hipStream_t s = (hipStream_t)stream;
+    c.have_result = false;
+    c.last_n = n;
+    c.last_kind = 0;
+    if (n == 0) {
+        c.have_result = true;
+        return JTP_OK;
+    }
+    int rc = prepare(c, n);
+    if (rc != JTP_OK) return rc;
+    if (c.pot_cap < c.cap_n) {
+        JTP_HIP(&c, dalloc(c.pot_short, (size_t)c.cap_n));
+        JTP_HIP(&c, dalloc(c.acc_scratch, (size_t)3 * c.cap_n));
+        c.pot_cap = c.cap_n;
+    }
+    if (!c.kself_ready) {
+        // the mesh potential of a unit mass at node 0, at the 27 nodes around it
+        JTP_HIP(&c, dalloc(c.kself, 27));
+        JTP_HIP(&c, launch_fill_i32((int32_t*)c.rho, 2 * c.G3(), 0, s));
+        const double one = 1.0;
+        JTP_HIP(&c, hipStreamSynchronize(s));
+        JTP_HIP(&c, hipMemcpy(c.rho, &one, sizeof(double), hipMemcpyHostToDevice));
+        JTP_HIP(&c, pm_fft_solve(c, c.rho, c.phi, s, 0));
+        JTP_HIP(&c, hipStreamSynchronize(s));
+        double k27[27];
+        const int64_t G = c.G();
+        for (int d = 0; d < 27; ++d) {
+            const int64_t dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+            const int64_t node = (((dz + G) % G) * G + (dy + G) % G) * G + (dx + G) % G;
+            JTP_HIP(&c, hipMemcpy(&k27[d], (const double*)c.phi + node, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        JTP_HIP(&c, hipMemcpy(c.kself, k27, sizeof(k27), hipMemcpyHostToDevice));
+        c.kself_ready = true;
+    }
+    rc = enqueue_forces(c, n, d_pos, d_species, c.acc_scratch, s, true, c.pot_short);
+    if (rc != JTP_OK) return rc;
+    // The force's Green function is matched to the gradient stencil, which distorts the potential at
+    // a few cells; the energy uses the potential of the plain deconvolved Gaussian-split Green function
+    // (the same one the self-energy kernel was computed with).
+    JTP_HIP(&c, pm_fft_solve(c, c.rho, c.phi, s, 0));
+    if (c.p.interlace) JTP_HIP(&c, pm_fft_solve(c, c.rho2, c.phi2, s, 0));
+    double* res = c.diag_scratch + 1024 * 8;
+    JTP_HIP(&c, launch_energy_terms(c, n, c.kself, c.pot_short, c.diag_scratch, res, s));
+    double host[5];
+    JTP_HIP(&c, hipMemcpyAsync(host, res, sizeof(host), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipStreamSynchronize(s));
+    rc = check_status(c, n);
+    if (rc != JTP_OK) return rc;
+    // The mesh drops the k = 0 mode (a uniform background of net density -Q/V); its long-range part
+    // is thereby accounted for, but the short-range sum runs over particles only, so the background's
+    // short-range potential G (Q/V) 4 pi r_s^2 (the integral of erfc(r / 2 r_s) / r) is added:
+    // W_bg = 1/2 Q x 4 pi G r_s^2 Q / V (the Ewald background term pi Q^2 / (V alpha^2), alpha = 1/2 r_s).
+    const double L = c.p.box_size, Q = host[3], V = L * L * L;
+    const double w_bg = 2.0 * M_PI * c.p.g_constant * c.p.r_s * c.p.r_s * Q * Q / V;
+    // Subtracting the mesh's self-potential S_i removes each particle's long-range interaction with
+    // its own periodic images too, which belongs to the periodic energy.  A lone mass in the box has
+    // W = 1/2 G m^2 xi / L (xi = 2.837297479, the simple cubic Madelung constant; the gravitational
+    // sign is +); of that, the background's short-range share is 4 pi r_s^2 / V (in w_bg) and the
+    // images' short-range share, -sum_n erfc(|n| L / 2 r_s) / |n| L, lies beyond r_cut <= L / 5 and
+    // is not part of the cut force law, so the long-range share is xi / L - 4 pi r_s^2 / V plus the
+    // image sum (below 1e-12 of it for r_s <= L / 10).
+    double img = 2.837297479480619 / L - 4.0 * M_PI * c.p.r_s * c.p.r_s / V;
+    for (int i = -2; i <= 2; ++i)
+        for (int j = -2; j <= 2; ++j)
+            for (int k = -2; k <= 2; ++k) {
+                if (i == 0 && j == 0 && k == 0) continue;
+                const double r = L * std::sqrt((double)(i * i + j * j + k * k));
+                img += std::erfc(r * 0.5 / c.p.r_s) / r;
+            }
+    const double w_img = 0.5 * c.p.g_constant * host[4] * img;
+    *w_out = 0.5 * (host[0] + host[1] - host[2]) + w_bg + w_img;
     return JTP_OK;
 }
 
@@ -787,6 +871,10 @@ int jtp_copy_d2d(void* d, const void* s, size_t b) {
     if (b && (!d || !s)) return JTP_ERR_ARG;
     if (device_copy(d, s, b, nullptr) != hipSuccess) return JTP_ERR_HIP;
     return hipStreamSynchronize(nullptr) == hipSuccess ? JTP_OK : JTP_ERR_HIP;
+}
+int jtp_device_memory(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return JTP_ERR_ARG;
+    return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? JTP_OK : JTP_ERR_HIP;
 }
 int jtp_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? JTP_OK : JTP_ERR_HIP; }
 int jtp_device_count(int* out) {

@@ -148,6 +148,11 @@ struct Ctx {
     unsigned long long* host_pinned = nullptr;  // [0] interactions, [1] (n_groups | err << 32)
 
     double* diag_scratch = nullptr;   // FP64 reduction partials (kDiagBlocks x 8) + 10 results
+    double* pot_short = nullptr;      // energy diagnostic: short-range potential per particle (cap_n)
+    double* acc_scratch = nullptr;    // energy diagnostic: forces of that evaluation (3 cap_n)
+    int64_t pot_cap = 0;
+    double* kself = nullptr;          // 27 values of the mesh potential of a unit mass, offsets {-1,0,1}^3
+    bool kself_ready = false;
     StepParams* dev_step = nullptr;   // device copy of the step scalars
     StepParams* host_step = nullptr;  // pinned staging of the step scalars
     StepGraph graph;
@@ -190,16 +195,21 @@ hipError_t launch_cell_start(const uint64_t* keys, int64_t n, int shift, int64_t
                              hipStream_t s);
 
 hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s);           // grid <- sum q W
-hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s);  // phi <- Green * rho
+// phi <- Green * rho; matched = 1 multiplies by the gradient-stencil match (the force's Green
+// function when deconvolving; -1 = as the context's force), 0 leaves the potential unmatched
+hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched = -1);
 hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s);            // pm_acc <- -sigma grad phi
 hipError_t pm_tables(Ctx& c);                                      // twiddles + Green tables (sync)
 
-hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
+// pot != nullptr (FP64 contexts, erfc + Gaussian law): also writes the short-range potential
+hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot = nullptr);
 hipError_t launch_kick(Ctx& c, void* vel, const void* acc, const int8_t* species, int64_t n, hipStream_t s);
 hipError_t launch_drift(Ctx& c, uint32_t* pos, const void* vel, int64_t n, hipStream_t s);
 hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* species, double* scratch,
                               double* out, hipStream_t s);
 constexpr int kDiagScratch = 1024 * 8 + 16;  // doubles
+hipError_t launch_energy_terms(Ctx& c, int64_t n, const double* kself, const double* pot_short, double* scratch,
+                               double* out, hipStream_t s);
 hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s);
 
 // host-side: FP32 split polynomial for the context's split mode and x range

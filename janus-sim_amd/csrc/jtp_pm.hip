@@ -482,7 +482,7 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
 }
 
 template <typename R>
-static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s) {
+static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
     const int G = (int)c.G(), H = G / 2 + 1, logG = c.logG;
     const int npair = kFftPoints / G;
     const int KB = kFftPoints / G;
@@ -493,15 +493,16 @@ static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s) {
     k_fft_x_r2c<R><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, logG, tw);
     k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 0);
     k_fft_z_green<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, (const R*)c.gtab,
-                                                                     (R)c.green_norm, c.p.k_min, c.p.deconvolve);
+                                                                     (R)c.green_norm, c.p.k_min, matched);
     k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 1);
     k_fft_x_c2r<R><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, logG, tw);
     return hipGetLastError();
 }
 
-hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s) {
-    if (c.p.precision == JTP_FP64) return fft_solve_t<double>(c, (const double*)rho, (double*)phi, s);
-    return fft_solve_t<float>(c, (const float*)rho, (float*)phi, s);
+hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched) {
+    if (matched < 0) matched = c.p.deconvolve;  // the force's Green function
+    if (c.p.precision == JTP_FP64) return fft_solve_t<double>(c, (const double*)rho, (double*)phi, s, matched);
+    return fft_solve_t<float>(c, (const float*)rho, (float*)phi, s, matched);
 }
 
 hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {

@@ -166,3 +166,85 @@ hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* 
 }
 
 }  // namespace jtp
+
+// ------------------------------------------------------------------ potential energy (FP64)
+// out = {sum |m| sigma Phi_long, sum |m| sigma Phi_short, sum m^2 S, Q = sum |m| sigma}, from which
+// w_i = |m_i| sigma_i (Phi_long(x_i) + Phi_short_i) - m_i^2 S_i, W = 1/2 sum_i w_i, where Phi_long is
+// the CIC interpolation of the mesh potential (mean of the two interlaced grids), Phi_short the
+// walk's short-range potential, and S_i = sum_{a,b} w_a w_b K(n_a - n_b) the mesh potential a unit
+// mass deposited with particle i's CIC weights produces at the same weights (K: the potential of a
+// unit mass at one node, at offsets in {-1, 0, 1}^3), i.e. the self-energy the mesh includes.
+namespace jtp {
+
+__device__ inline void cic_w(uint32_t u, int logG, int& i0, double w[2]) {
+    const int sh = 32 - logG;
+    i0 = (int)(u >> sh);
+    const double f = (double)(u & ((1u << sh) - 1u)) * (1.0 / (double)(1ull << sh));
+    w[0] = 1.0 - f;
+    w[1] = f;
+}
+
+__device__ inline void grid_terms(const uint4 r, uint32_t shift, const double* __restrict__ phi, int logG,
+                                  const double* __restrict__ kself, double& pot, double& self) {
+    const int G = 1 << logG;
+    int ix, iy, iz;
+    double wx[2], wy[2], wz[2];
+    cic_w(r.x + shift, logG, ix, wx);
+    cic_w(r.y + shift, logG, iy, wy);
+    cic_w(r.z + shift, logG, iz, wz);
+    double p = 0, sf = 0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
+        const double wa = wx[ax] * wy[ay] * wz[az];
+        const int64_t node = ((int64_t)((iz + az) & (G - 1)) * G + ((iy + ay) & (G - 1))) * G + ((ix + ax) & (G - 1));
+        p += wa * phi[node];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+            const double wb = wx[bx] * wy[by] * wz[bz];
+            sf += wa * wb * kself[(ax - bx + 1) + 3 * (ay - by + 1) + 9 * (az - bz + 1)];
+        }
+    }
+    pot = p;
+    self = sf;
+}
+
+__global__ __launch_bounds__(256) void k_energy_terms(const uint4* __restrict__ parts, const int32_t* __restrict__ perm,
+                                                      int64_t n, const double* __restrict__ phi,
+                                                      const double* __restrict__ phi2, int logG,
+                                                      const double* __restrict__ kself,
+                                                      const double* __restrict__ pot_short, double mp, double mm,
+                                                      double* __restrict__ partial) {
+    double v[5] = {0, 0, 0, 0, 0};  // |m| sigma Phi_long, |m| sigma Phi_short, m^2 S, |m| sigma, m^2
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 r = parts[i];
+        const bool plus = (int32_t)r.w > 0;
+        const double m = plus ? mp : mm, sg = plus ? 1.0 : -1.0;
+        double p1, s1;
+        grid_terms(r, 0u, phi, logG, kself, p1, s1);
+        if (phi2) {
+            double p2, s2;
+            grid_terms(r, 1u << (31 - logG), phi2, logG, kself, p2, s2);
+            p1 = 0.5 * (p1 + p2);
+            s1 = 0.5 * (s1 + s2);
+        }
+        v[0] += m * sg * p1;
+        v[1] += m * sg * pot_short[perm[i]];
+        v[2] += m * m * s1;
+        v[3] += m * sg;
+        v[4] += m * m;
+    }
+    block_reduce_store<5>(v, partial + (int64_t)blockIdx.x * 5);
+}
+
+hipError_t launch_energy_terms(Ctx& c, int64_t n, const double* kself, const double* pot_short, double* scratch,
+                               double* out, hipStream_t s) {
+    k_energy_terms<<<dim3(kDiagBlocks), dim3(256), 0, s>>>(c.parts, c.vals0, n, (const double*)c.phi,
+                                                           c.p.interlace ? (const double*)c.phi2 : nullptr, c.logG,
+                                                           kself, pot_short, c.m_plus, c.m_minus, scratch);
+    k_diag_final<<<dim3(1), dim3(256), 0, s>>>(scratch, kDiagBlocks, 5, out);
+    return hipGetLastError();
+}
+
+}  // namespace jtp

@@ -42,6 +42,7 @@ struct WalkArgs {
     int32_t* counters;  // kCtr* layout: group count, error flag, per-XCD queue heads
     unsigned long long* inter;
     const void* pm_acc;  // sorted order, nullptr when the PM part is disabled
+    double* pot;         // FP64 energy diagnostic: short-range potential per particle (caller order)
     const int32_t* perm;
     void* out;
     int64_t n;
@@ -139,6 +140,19 @@ struct Pair<double, MODE, CUT> {
     }
 };
 
+// Short-range potential of a pair under the erfc + Gaussian law (FP64 energy diagnostic):
+// q erfc(rho / 2 r_s) / rho, whose gradient is the pair force; zero for the target itself (r = 0)
+// and beyond the cut where the force is cut.
+template <bool CUT>
+__device__ inline double pair_potential(double q, double dx, double dy, double dz, const WalkArgs& a, double eps2,
+                                        double rcut2) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r2s = r2 + eps2;
+    const double inv = rsqrt(r2s);
+    const double x = r2s * inv * a.inv2rs;
+    return (r2 > 0.0 && (!CUT || r2 < rcut2)) ? q * erfc(x) * inv : 0.0;
+}
+
 template <typename R>
 __device__ inline R fixed_rel(uint32_t u, uint32_t c, R scale) {
     return (R)(int32_t)(u - c) * scale;
@@ -164,7 +178,7 @@ __device__ inline double mono_q(const double4* mono, int32_t node, int s) { retu
 
 // LPT lanes share one target: a group holds up to 64 / LPT targets and each target's
 // interaction list is split over its LPT lanes (entries j = sub, sub + LPT, ...).
-template <typename R, int MODE, int WAVES, int LPT>
+template <typename R, int MODE, int WAVES, int LPT, bool POT = false>
 __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     constexpr int GL = 64 / LPT;  // targets per wavefront
     using E = typename EntryT<R>::T;
@@ -231,6 +245,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         const R tx = fixed_rel<R>(rec.x, gc[0], scale), ty = fixed_rel<R>(rec.y, gc[1], scale),
                 tz = fixed_rel<R>(rec.z, gc[2], scale);
         R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
+        double spot = 0;  // POT: short-range potential sum
 
         int sp = 1, lp = 0, lm = 0;
         int mono_inside = 0;  // monopole entries placed on the particle side since the last flush
@@ -273,6 +288,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 wave_lds_fence();
             }
             R px = 0, py = 0, pz = 0;
+            double ppot = 0;
 #pragma unroll 4
             for (int jj = 0; jj < itp; ++jj) {
                 const E e = L[jj * LPT + sub];
@@ -281,6 +297,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 px += f * dx;
                 py += f * dy;
                 pz += f * dz;
+                if (POT) ppot += pair_potential<true>((double)e.w, dx, dy, dz, a, (double)eps2, (double)rcut2);
             }
             const int m0 = kListCap - itm * LPT + sub;
 #pragma unroll 4
@@ -291,13 +308,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 px += f * dx;
                 py += f * dy;
                 pz += f * dz;
+                if (POT) ppot += pair_potential<false>((double)e.w, dx, dy, dz, a, (double)eps2, (double)rcut2);
             }
 #pragma unroll
             for (int off = GL; off < 64; off <<= 1) {  // combine the LPT partial sums of a target
                 px += __shfl_xor(px, off, 64);
                 py += __shfl_xor(py, off, 64);
                 pz += __shfl_xor(pz, off, 64);
+                if (POT) ppot += __shfl_xor(ppot, off, 64);
             }
+            if (POT) spot += ppot;
             if (sizeof(R) == 4) {  // compensated accumulation of the chunk partials
                 R y = px - cx, t = sx + y;
                 cx = (t - sx) - y;
@@ -479,6 +499,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             out[3 * o] = ox;
             out[3 * o + 1] = oy;
             out[3 * o + 2] = oz;
+            if (POT) a.pot[o] = -spot;  // Phi_short = -sum_j q_j erfc(rho / 2 r_s) / rho
         }
     }
     if (lane == 0 && my_inter) {
@@ -515,7 +536,7 @@ hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int WAVES, int LPT>
+template <typename R, int MODE, int WAVES, int LPT, bool POT = false>
 static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
@@ -524,7 +545,7 @@ static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
     const int64_t cap = (int64_t)cus * 16;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    k_walk<R, MODE, WAVES, LPT><<<dim3((unsigned)blocks), dim3(64 * WAVES), 0, s>>>(a);
+    k_walk<R, MODE, WAVES, LPT, POT><<<dim3((unsigned)blocks), dim3(64 * WAVES), 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -535,7 +556,7 @@ static hipError_t launch_walk_mode(int mode, const WalkArgs& a, int64_t gmax, hi
     return launch_walk<R, JTP_SPLIT_POLY4, WAVES, LPT>(a, gmax, s);
 }
 
-hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
+hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
     WalkArgs a{};
     a.parts = c.parts;
     a.geo = c.node_geo;
@@ -570,6 +591,19 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     const int64_t gmax = c.n_groups_max;
     const int mode = c.p.split_mode;
     const int lpt = lanes_per_target(c.p.group_size);
+    if (c.p.precision == JTP_FP64 && pot) {  // energy diagnostic: erfc + Gaussian law only
+        // The short-range kernel is not harmonic (its Laplacian is 4 pi G times the Gaussian density
+        // of the split, > 0 off the origin), so a monopole evaluated at a node's centre of mass falls
+        // short of the node's mean potential by ~ <d^2>/6 times that Laplacian: a bias of one sign that
+        // the forces do not show but the potential sums up (about 2 % of the short-range potential
+        // at theta = 0.5 on a uniform lattice).  The potential therefore opens every node (theta = 0).
+        a.theta2 = 0.0;
+        a.theta_cut2 = 0.0;
+        a.pot = pot;
+        if (lpt == 4) return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2, 4, true>(a, gmax, s);
+        if (lpt == 2) return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2, 2, true>(a, gmax, s);
+        return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2, 1, true>(a, gmax, s);
+    }
     if (c.p.precision == JTP_FP64) {
         if (lpt == 4) return launch_walk_mode<double, 2, 4>(mode, a, gmax, s);
         if (lpt == 2) return launch_walk_mode<double, 2, 2>(mode, a, gmax, s);

@@ -42,7 +42,7 @@ EXPORTED = [
     "jtp_get_stats", "jtp_stage_size", "jtp_stage_copy", "jtp_alloc", "jtp_free", "jtp_copy_h2d",
     "jtp_copy_d2h", "jtp_copy_d2d", "jtp_device_synchronize", "jtp_device_count", "jtp_step",
     "jtp_ensemble_create", "jtp_ensemble_destroy", "jtp_ensemble_last_error", "jtp_ensemble_compute_forces",
-    "jtp_ensemble_get_stats", "jtp_diagnostics",
+    "jtp_ensemble_get_stats", "jtp_diagnostics", "jtp_potential_energy", "jtp_device_memory",
 ]
 
 
@@ -122,6 +122,8 @@ def load_library(path: str = LIB_PATH):
                                                   i32, vp]),
         "jtp_ensemble_get_stats": (C.c_int, [vp, i32, C.POINTER(Stats)]),
         "jtp_diagnostics": (C.c_int, [vp, i64, vp, vp, C.POINTER(Diag), vp]),
+        "jtp_potential_energy": (C.c_int, [vp, i64, vp, vp, C.POINTER(dbl), vp]),
+        "jtp_device_memory": (C.c_int, [C.POINTER(sz), C.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -129,6 +131,16 @@ def load_library(path: str = LIB_PATH):
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def device_memory() -> tuple:
+    """(free, total) bytes of the current device."""
+    lib = load_library()
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    rc = lib.jtp_device_memory(C.byref(f), C.byref(t))
+    if rc != JTP_OK:
+        raise JtpError(rc, "hipMemGetInfo failed")
+    return f.value, t.value
 
 
 def device_count() -> int:
@@ -238,6 +250,15 @@ class ForceContext:
         self._check(self.lib.jtp_diagnostics(self.h, int(n), ptr(d_vel), ptr(d_species), C.byref(d), stream))
         return {"kinetic": np.array(d.kinetic[:]), "momentum": np.array([list(r) for r in d.momentum]),
                 "pm_energy": d.pm_energy}
+
+    def potential_energy(self, n: int, d_pos_fixed, d_species, stream=None) -> float:
+        """FP64 total potential energy (FP64 context, erfc + Gaussian split): mesh + short range, mesh
+        self-energy removed."""
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        w = C.c_double(0.0)
+        self._check(self.lib.jtp_potential_energy(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), C.byref(w),
+                                                  stream))
+        return w.value
 
     def compute_forces(self, pos_fixed: np.ndarray, species: np.ndarray) -> np.ndarray:
         """Host convenience: upload, compute, download (float32 or float64 [n,3])."""

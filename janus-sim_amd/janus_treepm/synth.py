@@ -131,10 +131,16 @@ def _cic_density(pos, box_size, G):
 
 def clustered(n_total: int = 22_000_000, box_size: float = 250.0, n_cells: int = 256, n_halos: int = 2000,
               seed: int = 7, m_plus: float = 1.0, m_minus_abs: float = 19.0, theta: float = 0.5,
-              halo_fraction: float = 0.6, leaf_size: int = 4, name: str = "clustered") -> Workload:
+              halo_fraction: float = 0.6, leaf_size: int = 4, name: str = "clustered",
+              n_plus: int | None = None) -> Workload:
+    """Clustered two-species box: 60 % of m+ in NFW halos (dN/dM ~ M^-1.9 on [1e-5, 1e-2] of the m+
+    mass, concentration U[5, 15], r_200 at 200 x the mean m+ density), the rest uniform; m- rejection
+    sampled where the CIC density of m+ smoothed by a 2-cell Gaussian is below its median.  n_plus
+    defaults to half of n_total."""
     rng = np.random.Generator(np.random.PCG64(seed))
     L = box_size
-    n_plus = n_total // 2
+    if n_plus is None:
+        n_plus = n_total // 2
     n_minus = n_total - n_plus
     # halo masses ~ M^-1.9 on [1e-5, 1e-2] of the total m+ mass (inverse CDF of a power law)
     a, lo, hi = -1.9, 1e-5, 1e-2
