@@ -373,15 +373,24 @@ def main():
                        "traffic": traffic, "traffic_source": traffic_src, "interactions": inter,
                        "interactions_mono": inter_mono, "flops_per_interaction": FLOPS_PER_INTERACTION,
                        "walk_ms": walk_ms}
+    out["phases_ms"] = ph32
+    # PM kernels alone: in the step the tree build shares the GPU with them (side stream), so their
+    # HBM rates are taken from PM-only evaluations (tree disabled) of the same particles.
+    saved = dict(OVERRIDES)
+    OVERRIDES["tree_enabled"] = 0
+    _, st_pm, _ = run_form(w, d_pos, d_sp, j.FP32, 2, 5)
+    OVERRIDES.clear()
+    OVERRIDES.update(saved)
+    ph_pm = phase_summary(st_pm)
     G3 = w.n_cells ** 3
     grids = 2 if OVERRIDES.get("interlace", 1) else 1  # interlaced PM runs every grid phase twice
     hbm = {
-        "deposit": (w.n * 13 + grids * 4 * G3) / (ph32["ms_deposit"] * 1e-3) / 1e9,
-        "fft": (grids * 40 * G3) / (ph32["ms_fft"] * 1e-3) / 1e9,
-        "interp": (w.n * 25 + grids * 4 * G3) / (ph32["ms_interp"] * 1e-3) / 1e9,
+        "deposit": (w.n * 13 + grids * 4 * G3) / (ph_pm["ms_deposit"] * 1e-3) / 1e9,
+        "fft": (grids * 40 * G3) / (ph_pm["ms_fft"] * 1e-3) / 1e9,
+        "interp": (w.n * 25 + grids * 4 * G3) / (ph_pm["ms_interp"] * 1e-3) / 1e9,
     }
-    out["phases_ms"] = ph32
-    out["phase_hbm_gbs"] = {k: {"achieved": v, "frac": v / PEAK_HBM_GBS} for k, v in hbm.items()}
+    out["phase_hbm_gbs"] = {k: {"achieved": v, "frac": v / PEAK_HBM_GBS, "ms": ph_pm["ms_" + k]}
+                            for k, v in hbm.items()}
 
     if not args.no_fp64:
         t64, st64, _ = run_form(w, d_pos, d_sp, j.FP64, 1, args.fp64_steps)

@@ -234,6 +234,163 @@ __global__ void k_deposit_both(const uint4* __restrict__ parts, const int32_t* _
     rho2[(kk * G + jj) * G + ii] = acc2;
 }
 
+// FP32 form: one workgroup per tile of 8^3 cells (a Morton-aligned block), owning the 8^3 nodes of
+// each grid that take mass from its cells: node-grid nodes 0..7 per axis (cell corners) and the
+// interlaced nodes at the centres of cells 0..7.  Those nodes also take mass from the one-cell
+// shell around the tile (cells -1 .. 8 per axis), so the workgroup reads the particles of all
+// 10^3 cells and keeps only the contributions to its own nodes: every node is written by one
+// workgroup with a plain store, no global atomics and no zeroing pass.  Within the workgroup the
+// contributions are summed in LDS as 64-bit fixed point: integer addition is associative, so the
+// sums do not depend on the order in which the LDS atomics land and the grid is bit-reproducible.
+// Charges arrive pre-scaled by 2^k with |q| 2^k <= 2^32, so a contribution q W 2^k rounds to an
+// integer of at most 2^32 and a node sum of n < 2^31 particles fits in 63 bits; the unit 2^-k is
+// 2^-32 of the largest |q| or finer.  Dense cells (> kDenseCell particles) enter through their
+// corner sums from k_dense_reduce*, computed with the same pre-scaled charges.  Each thread takes a
+// contiguous run of the tile's sparse particles, so the lanes of a wavefront sit in different
+// cells and seldom hit one LDS address together.  Tiles are dealt to the 8 XCDs in contiguous
+// Morton ranges (blocks are dispatched round-robin over the XCDs), so the shell cells a tile reads
+// were mostly read by its predecessor on the same XCD and sit in that XCD's L2.
+constexpr int kTile = 8;
+constexpr int kShell = kTile + 2;                      // cells -1 .. 8 per axis
+constexpr int kShellCells = kShell * kShell * kShell;  // 1000
+
+__device__ inline long long fix_round(float v) { return __float2ll_rn(v); }
+
+__device__ inline void lds_add(unsigned long long* a, long long v) {
+    if (v != 0) atomicAdd(a, (unsigned long long)v);
+}
+
+template <bool BOTH>
+__global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                                                      const int32_t* __restrict__ slot,
+                                                      const float* __restrict__ corner8,
+                                                      const float* __restrict__ corner27, int logG, float qp,
+                                                      float qm, double unit, float* __restrict__ rho,
+                                                      float* __restrict__ rho2) {
+    __shared__ unsigned long long a1[kTile * kTile * kTile];           // node-grid nodes 0..7
+    __shared__ unsigned long long a2[BOTH ? kTile * kTile * kTile : 1];  // centres of cells 0..7
+    __shared__ int32_t cst[kShellCells];      // particle start of each shell cell
+    __shared__ int32_t pre[kShellCells + 1];  // prefix of the sparse cells' counts
+    __shared__ int32_t wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t G = 1ll << logG;
+    const int32_t msk = (int32_t)G - 1;
+    const unsigned ntile = gridDim.x;
+    const unsigned tile = (blockIdx.x & 7u) * (ntile >> 3) + (blockIdx.x >> 3);
+    const uint64_t base = (uint64_t)tile << 9;  // Morton index of the tile's first cell
+    const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
+                  oz = (int32_t)compact3_21(base >> 2);
+    for (int i = tid; i < kTile * kTile * kTile; i += 256) {
+        a1[i] = 0;
+        if (BOTH) a2[i] = 0;
+    }
+    __syncthreads();
+    // shell cells 4 tid .. 4 tid + 3 (s = (cz + 1) 100 + (cy + 1) 10 + cx + 1)
+    int cnt[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int sc = 4 * tid + u;
+        if (sc >= kShellCells) continue;
+        const int cx = sc % kShell - 1, cy = (sc / kShell) % kShell - 1, cz = sc / (kShell * kShell) - 1;
+        const uint64_t cm = spread3_21((uint32_t)((ox + cx) & msk)) | (spread3_21((uint32_t)((oy + cy) & msk)) << 1) |
+                            (spread3_21((uint32_t)((oz + cz) & msk)) << 2);
+        const int32_t st = cs[cm], k = cs[cm + 1] - st;
+        cst[sc] = st;
+        if (k <= kDenseCell) {
+            cnt[u] = k;
+            continue;
+        }
+        const int64_t sl = slot[cm];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // node-grid node = cell + (q & 1, q >> 1 & 1, q >> 2)
+            const int nx = cx + (q & 1), ny = cy + ((q >> 1) & 1), nz = cz + (q >> 2);
+            if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
+                lds_add(&a1[(nz * kTile + ny) * kTile + nx], __double2ll_rn((double)corner8[8 * sl + q]));
+        }
+        if (BOTH)
+            for (int q = 0; q < 27; ++q) {  // centre of cell c + e, e = (q % 3, q / 3 % 3, q / 9) - 1
+                const int nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
+                if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
+                    lds_add(&a2[(nz * kTile + ny) * kTile + nx], __double2ll_rn((double)corner27[27 * sl + q]));
+            }
+    }
+    // block exclusive scan of the sparse counts in shell-cell order
+    const int mine = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+    int incl = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wv; ++w) woff += wsum[w];
+    int run = woff + incl - mine;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int sc = 4 * tid + u;
+        if (sc < kShellCells) pre[sc] = run;
+        run += cnt[u];
+    }
+    if (tid == 0) pre[kShellCells] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int32_t total = pre[kShellCells];
+    const int32_t per = (total + 255) >> 8;
+    int32_t t = tid * per;
+    const int32_t t_end = min(total, t + per);
+    if (t < t_end) {
+        int lo = 0, hi = kShellCells - 1;  // last shell cell with pre <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        int32_t cell_end = pre[lo + 1];
+        for (; t < t_end; ++t) {
+            while (t >= cell_end) cell_end = pre[++lo + 1];
+            const int cx = lo % kShell - 1, cy = (lo / kShell) % kShell - 1, cz = lo / (kShell * kShell) - 1;
+            const uint4 r = parts[cst[lo] + (t - pre[lo])];
+            const float fx = frac_of<float>(r.x, logG), fy = frac_of<float>(r.y, logG), fz = frac_of<float>(r.z, logG);
+            const float q = (int32_t)r.w > 0 ? qp : qm;
+            const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+                const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+                if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
+                    lds_add(&a1[(nz * kTile + ny) * kTile + nx], fix_round(q * (wx[dx] * wy[dy] * wz[dz])));
+            }
+            if (BOTH) {
+                // the two interlaced nodes around the particle per axis: centres of cells (c-1, c) if
+                // f < 1/2, else (c, c+1); the centre of cell m is slot m of a2
+                const bool lox = fx < 0.5f, loy = fy < 0.5f, loz = fz < 0.5f;
+                const int bx = cx - (lox ? 1 : 0), by = cy - (loy ? 1 : 0), bz = cz - (loz ? 1 : 0);
+                const float vx[2] = {lox ? 0.5f - fx : 1.5f - fx, lox ? 0.5f + fx : fx - 0.5f};
+                const float vy[2] = {loy ? 0.5f - fy : 1.5f - fy, loy ? 0.5f + fy : fy - 0.5f};
+                const float vz[2] = {loz ? 0.5f - fz : 1.5f - fz, loz ? 0.5f + fz : fz - 0.5f};
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+                    const int nx = bx + dx, ny = by + dy, nz = bz + dz;
+                    if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
+                        lds_add(&a2[(nz * kTile + ny) * kTile + nx], fix_round(q * (vx[dx] * vy[dy] * vz[dz])));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // node (x, y, z) of the tile: the node grid's node at cell corner o + l, the interlaced grid's
+    // node at the centre of cell o + l, stored at index o + l + 1
+    for (int i = tid; i < kTile * kTile * kTile; i += 256) {
+        const int lx = i % kTile, ly = (i / kTile) % kTile, lz = i / (kTile * kTile);
+        rho[((int64_t)(oz + lz) * G + (oy + ly)) * G + (ox + lx)] = (float)((double)(long long)a1[i] * unit);
+        if (BOTH)
+            rho2[((int64_t)((oz + lz + 1) & msk) * G + ((oy + ly + 1) & msk)) * G + ((ox + lx + 1) & msk)] =
+                (float)((double)(long long)a2[i] * unit);
+    }
+}
+
 // Thread per grid node, nodes enumerated in Morton order so that the eight
 // source cells of a wavefront's nodes are contiguous particle runs.
 template <typename R>
@@ -537,17 +694,24 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     } else {
         float* c8 = (float*)c.dense_corner;
         float* c27 = c8 + 8 * nd_cap;
-        const float qp = (float)c.m_plus, qm = (float)-c.m_minus;
+        // charges scaled by 2^k, |q| 2^k <= 2^32 (exact: a power of two); the grids' unit is 2^-k
+        int ex = 0;
+        (void)std::frexp(c.m_plus > c.m_minus ? c.m_plus : c.m_minus, &ex);  // max |q| <= 2^ex
+        const int k = 32 - ex;
+        const float qp = (float)std::ldexp(c.m_plus, k), qm = (float)-std::ldexp(c.m_minus, k);
+        const double unit = std::ldexp(1.0, -k);
+        const unsigned ntile = (unsigned)(G3 >> 9);  // 8^3-cell tiles (G >= 16: a multiple of 8)
         if (c.p.interlace) {
             k_dense_reduce_both<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
                                                                        qp, qm, c8, c27);
-            k_deposit_both<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
-                                                                    c.logG, qp, qm, (float*)c.rho, (float*)c.rho2);
+            k_deposit_tile<true><<<dim3(ntile), dim3(256), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
+                                                                   c.logG, qp, qm, unit, (float*)c.rho,
+                                                                   (float*)c.rho2);
         } else {
             k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
                                                                          qp, qm, c8);
-            k_deposit<float><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG, qp,
-                                                               qm, (float*)c.rho);
+            k_deposit_tile<false><<<dim3(ntile), dim3(256), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, nullptr,
+                                                                    c.logG, qp, qm, unit, (float*)c.rho, nullptr);
         }
     }
     return hipGetLastError();
