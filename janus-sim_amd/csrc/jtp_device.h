@@ -53,6 +53,23 @@ __device__ inline int wave_append(bool pred, int32_t* counter) {
     return pred ? base + (int)__popcll(m & lanemask_lt()) : -1;
 }
 
+// Reserves `count` consecutive slots per lane of a global list (one atomic per wavefront); returns
+// the lane's first slot (meaningless where count is 0).  Every lane of the wavefront must call it.
+__device__ inline int wave_reserve(int count, int32_t* counter) {
+    int incl = count;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane_id() >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0) return -1;
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    return base + incl - count;
+}
+
 // Appends the threads with `pred` to a global list using one atomic per 256-thread block (the
 // throughput of same-address atomics, not their count per wavefront, limits frontier appends).
 // Every thread of the block must call it; `lds` holds 6 int32 of scratch.

@@ -20,7 +20,8 @@ constexpr int kSortTile = kBlock * kSortIPT;
 constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Morton cell (L/16)
 constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
 constexpr int kFftPoints = 4096;     // complex points held in LDS per FFT workgroup
-constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by one wavefront
+constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by wavefronts,
+constexpr int kDenseChunk = 1024;    // one per chunk of this many particles
 constexpr int kMaxRounds = 128;      // size of the bottom-up control block (frontier count ... root flag)
 constexpr int kXcds = 8;             // MI355X accelerator complex dies; workgroup i runs on XCD i % 8
 // device counter block (int32): offsets
@@ -132,7 +133,7 @@ struct Ctx {
     // PM
     int32_t *cell_start = nullptr;   // G^3 + 1
     int32_t *dense_slot = nullptr;   // G^3: slot of a dense cell's corner sums (written for dense cells only)
-    void *dense_corner = nullptr;    // real [n / kDenseCell + 1][8] corner sums of dense cells
+    void *dense_corner = nullptr;    // real [n / kDenseCell + 1][8 + 27] corner sums of dense-cell chunks
     void *rho = nullptr;             // real G^3 deposited mass per cell
     void *phi = nullptr;             // real G^3 potential
     void *rho2 = nullptr, *phi2 = nullptr;  // interlaced grid (nodes at the cell centres of the first)

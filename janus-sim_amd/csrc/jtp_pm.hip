@@ -33,7 +33,7 @@ __device__ inline R frac_of(uint32_t u, int logG) {
 // Deterministic CIC deposit ("sort + segmented reduction").  The particles are
 // in Morton order and G is a power of two, so every PM cell is a contiguous run
 // (cell_start).  Cells holding more than kDenseCell particles (halo cores hold up
-// to ~1e5) are first reduced by one wavefront each into their 8 corner sums, in a
+// to ~1e5) are first reduced by wavefronts into their 8 corner sums, in a
 // fixed lane/tree order; then every grid node gathers, in a fixed order, the
 // corner sums of dense source cells and the particles of the others.
 template <typename R>
@@ -44,18 +44,47 @@ __device__ inline void cic_corners(const uint4 r, int logG, R q, R w[8]) {
     for (int c = 0; c < 8; ++c) w[c] = q * (wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2]);
 }
 
+// A dense cell is reduced in chunks of kDenseChunk particles (one wavefront each, so a halo core
+// of ~1e4 particles is not one long serial loop); slot[c] is its first chunk, list[] maps each
+// chunk back to its cell, and k_dense_combine adds a cell's chunk sums, in chunk order, into its
+// first chunk, where the consumers read them.
+__device__ inline int dense_chunks(int32_t cnt) { return (cnt + kDenseChunk - 1) / kDenseChunk; }
+
 __global__ void k_dense_mark(const int32_t* __restrict__ cs, int64_t ncells, int32_t* __restrict__ slot,
                              int32_t* __restrict__ list, int32_t* __restrict__ count) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool dense = c < ncells && cs[c + 1] - cs[c] > kDenseCell;
-    const int k = wave_append(dense, count);
-    if (k >= 0) {
+    const int32_t cnt = c < ncells ? cs[c + 1] - cs[c] : 0;
+    const int nch = cnt > kDenseCell ? dense_chunks(cnt) : 0;
+    const int k = wave_reserve(nch, count);
+    if (nch > 0) {
         slot[c] = k;
-        list[k] = (int32_t)c;
+        for (int i = 0; i < nch; ++i) list[k + i] = (int32_t)c;
     }
 }
 
-// One wavefront per dense cell: lane l takes particles s + l, s + l + 64, ...; then a
+// Thread per (chunk, value): the first chunk of a cell sums the value over the cell's chunks in order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_combine(const int32_t* __restrict__ cs,
+                                                       const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ slot,
+                                                       const int32_t* __restrict__ count, int nv,
+                                                       R* __restrict__ vals) {
+    const int32_t nd = *count;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)nd * nv;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = t / nv;
+        const int q = (int)(t - k * nv);
+        const int32_t c = list[k];
+        if (slot[c] != k) continue;
+        const int nch = dense_chunks(cs[c + 1] - cs[c]);
+        if (nch == 1) continue;
+        R v = vals[k * nv + q];
+        for (int i = 1; i < nch; ++i) v += vals[(k + i) * nv + q];
+        vals[k * nv + q] = v;
+    }
+}
+
+// One wavefront per dense-cell chunk: lane l takes particles s + l, s + l + 64, ...; then a
 // fixed butterfly reduction across lanes.  Unshifted corner index c = dx + 2 dy + 4 dz
 // (offset of the node from the cell's lower corner).  With interlacing, also the 27 sums
 // for the shifted grid, whose nodes sit at the centres of cells c-1, c, c+1 per axis:
@@ -71,6 +100,7 @@ __device__ inline void shifted_weights(R f, R w[3]) {
 template <typename R>
 __global__ __launch_bounds__(256) void k_dense_reduce(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
                                                       const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ slot,
                                                       const int32_t* __restrict__ count, int logG, R qp, R qm,
                                                       R* __restrict__ corner) {
     const int lane = threadIdx.x & 63;
@@ -78,10 +108,11 @@ __global__ __launch_bounds__(256) void k_dense_reduce(const uint4* __restrict__ 
     for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd;
          k += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int32_t c = list[k];
-        const int32_t s = cs[c], e = cs[c + 1];
+        const int32_t s = cs[c] + (int32_t)(k - slot[c]) * kDenseChunk, e = min(cs[c + 1], s + kDenseChunk);
         R acc[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[q] = 0;
+#pragma unroll 4
         for (int32_t p = s + lane; p < e; p += 64) {
             const uint4 r = parts[p];
             const R q = (int32_t)r.w > 0 ? qp : qm;
@@ -113,6 +144,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_dense_reduce_both(const uint4* __restrict__ parts,
                                                            const int32_t* __restrict__ cs,
                                                            const int32_t* __restrict__ list,
+                                                           const int32_t* __restrict__ slot,
                                                            const int32_t* __restrict__ count, int logG, R qp, R qm,
                                                            R* __restrict__ corner8, R* __restrict__ corner27) {
     const int lane = threadIdx.x & 63;
@@ -120,12 +152,13 @@ __global__ __launch_bounds__(256) void k_dense_reduce_both(const uint4* __restri
     for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd;
          k += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const int32_t c = list[k];
-        const int32_t s = cs[c], e = cs[c + 1];
+        const int32_t s = cs[c] + (int32_t)(k - slot[c]) * kDenseChunk, e = min(cs[c + 1], s + kDenseChunk);
         R a8[8], a27[27];
 #pragma unroll
         for (int q = 0; q < 8; ++q) a8[q] = 0;
 #pragma unroll
         for (int q = 0; q < 27; ++q) a27[q] = 0;
+#pragma unroll 4
         for (int32_t p = s + lane; p < e; p += 64) {
             const uint4 r = parts[p];
             const R q = (int32_t)r.w > 0 ? qp : qm;
@@ -242,9 +275,9 @@ __global__ void k_deposit_both(const uint4* __restrict__ parts, const int32_t* _
 // workgroup with a plain store, no global atomics and no zeroing pass.  Within the workgroup the
 // contributions are summed in LDS as 64-bit fixed point: integer addition is associative, so the
 // sums do not depend on the order in which the LDS atomics land and the grid is bit-reproducible.
-// Charges arrive pre-scaled by 2^k with |q| 2^k <= 2^32, so a contribution q W 2^k rounds to an
-// integer of at most 2^32 and a node sum of n < 2^31 particles fits in 63 bits; the unit 2^-k is
-// 2^-32 of the largest |q| or finer.  Dense cells (> kDenseCell particles) enter through their
+// Charges arrive pre-scaled by 2^k with |q| 2^k <= 2^30, so a contribution q W 2^k rounds to an
+// int32 and a node sum of n < 2^31 particles fits in 61 bits; the unit 2^-k is 2^-30 of the
+// largest |q| or finer (FP32 resolves 2^-24 of it).  Dense cells (> kDenseCell particles) enter through their
 // corner sums from k_dense_reduce*, computed with the same pre-scaled charges.  Each thread takes a
 // contiguous run of the tile's sparse particles, so the lanes of a wavefront sit in different
 // cells and seldom hit one LDS address together.  Tiles are dealt to the 8 XCDs in contiguous
@@ -253,12 +286,18 @@ __global__ void k_deposit_both(const uint4* __restrict__ parts, const int32_t* _
 constexpr int kTile = 8;
 constexpr int kShell = kTile + 2;                      // cells -1 .. 8 per axis
 constexpr int kShellCells = kShell * kShell * kShell;  // 1000
+constexpr int kN1 = kTile + 3;                         // node-grid nodes -1 .. 9 per axis
+constexpr int kN2 = kTile + 4;                         // centres of cells -2 .. 9 per axis
+// the largest LDS index a shell particle reaches: cell 8 (index 9) plus corner offset 1 per axis in
+// a1; cell 8 above its centre (index 10) plus offset 1 per axis in a2
+static_assert(((kTile + 2) * kN1 + kTile + 2) * kN1 + kTile + 2 < kN1 * kN1 * kN1, "a1 too small");
+static_assert(((kTile + 3) * kN2 + kTile + 3) * kN2 + kTile + 3 < kN2 * kN2 * kN2, "a2 too small");
 
-__device__ inline long long fix_round(float v) { return __float2ll_rn(v); }
+// |v| <= 2^30: one hardware float -> int32 conversion, then sign extension (a float -> int64
+// conversion has no instruction and costs ~20)
+__device__ inline long long fix_round(float v) { return (long long)__float2int_rn(v); }
 
-__device__ inline void lds_add(unsigned long long* a, long long v) {
-    if (v != 0) atomicAdd(a, (unsigned long long)v);
-}
+__device__ inline void lds_add(unsigned long long* a, long long v) { atomicAdd(a, (unsigned long long)v); }
 
 template <bool BOTH>
 __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
@@ -267,8 +306,11 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
                                                       const float* __restrict__ corner27, int logG, float qp,
                                                       float qm, double unit, float* __restrict__ rho,
                                                       float* __restrict__ rho2) {
-    __shared__ unsigned long long a1[kTile * kTile * kTile];           // node-grid nodes 0..7
-    __shared__ unsigned long long a2[BOTH ? kTile * kTile * kTile : 1];  // centres of cells 0..7
+    // every node the shell's particles reach, so that the corner offsets are compile-time constants
+    // (LDS instruction offsets) and no contribution needs a bounds test; only the tile's own nodes
+    // are stored at the end
+    __shared__ unsigned long long a1[kN1 * kN1 * kN1];           // node-grid nodes -1 .. 9, index l + 1
+    __shared__ unsigned long long a2[BOTH ? kN2 * kN2 * kN2 : 1];  // centres of cells -2 .. 9, index l + 2
     __shared__ int32_t cst[kShellCells];      // particle start of each shell cell
     __shared__ int32_t pre[kShellCells + 1];  // prefix of the sparse cells' counts
     __shared__ int32_t wsum[4];
@@ -280,10 +322,9 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
     const uint64_t base = (uint64_t)tile << 9;  // Morton index of the tile's first cell
     const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
                   oz = (int32_t)compact3_21(base >> 2);
-    for (int i = tid; i < kTile * kTile * kTile; i += 256) {
-        a1[i] = 0;
-        if (BOTH) a2[i] = 0;
-    }
+    for (int i = tid; i < kN1 * kN1 * kN1; i += 256) a1[i] = 0;
+    if (BOTH)
+        for (int i = tid; i < kN2 * kN2 * kN2; i += 256) a2[i] = 0;
     __syncthreads();
     // shell cells 4 tid .. 4 tid + 3 (s = (cz + 1) 100 + (cy + 1) 10 + cx + 1)
     int cnt[4] = {0, 0, 0, 0};
@@ -301,18 +342,18 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
             continue;
         }
         const int64_t sl = slot[cm];
+        const int b1 = ((cz + 1) * kN1 + cy + 1) * kN1 + cx + 1;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {  // node-grid node = cell + (q & 1, q >> 1 & 1, q >> 2)
-            const int nx = cx + (q & 1), ny = cy + ((q >> 1) & 1), nz = cz + (q >> 2);
-            if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
-                lds_add(&a1[(nz * kTile + ny) * kTile + nx], __double2ll_rn((double)corner8[8 * sl + q]));
+        for (int q = 0; q < 8; ++q)  // node-grid node = cell + (q & 1, q >> 1 & 1, q >> 2)
+            lds_add(&a1[b1 + ((q >> 2) * kN1 + ((q >> 1) & 1)) * kN1 + (q & 1)],
+                    __double2ll_rn((double)corner8[8 * sl + q]));
+        if (BOTH) {
+            const int b2 = ((cz + 1) * kN2 + cy + 1) * kN2 + cx + 1;  // centre of cell c - 1
+#pragma unroll
+            for (int q = 0; q < 27; ++q)  // centre of cell c + e, e = (q % 3, q / 3 % 3, q / 9) - 1
+                lds_add(&a2[b2 + ((q / 9) * kN2 + (q / 3) % 3) * kN2 + q % 3],
+                        __double2ll_rn((double)corner27[27 * sl + q]));
         }
-        if (BOTH)
-            for (int q = 0; q < 27; ++q) {  // centre of cell c + e, e = (q % 3, q / 3 % 3, q / 9) - 1
-                const int nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
-                if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
-                    lds_add(&a2[(nz * kTile + ny) * kTile + nx], __double2ll_rn((double)corner27[27 * sl + q]));
-            }
     }
     // block exclusive scan of the sparse counts in shell-cell order
     const int mine = cnt[0] + cnt[1] + cnt[2] + cnt[3];
@@ -353,28 +394,28 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
             const uint4 r = parts[cst[lo] + (t - pre[lo])];
             const float fx = frac_of<float>(r.x, logG), fy = frac_of<float>(r.y, logG), fz = frac_of<float>(r.z, logG);
             const float q = (int32_t)r.w > 0 ? qp : qm;
-            const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                const int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-                if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
-                    lds_add(&a1[(nz * kTile + ny) * kTile + nx], fix_round(q * (wx[dx] * wy[dy] * wz[dz])));
-            }
-            if (BOTH) {
-                // the two interlaced nodes around the particle per axis: centres of cells (c-1, c) if
-                // f < 1/2, else (c, c+1); the centre of cell m is slot m of a2
-                const bool lox = fx < 0.5f, loy = fy < 0.5f, loz = fz < 0.5f;
-                const int bx = cx - (lox ? 1 : 0), by = cy - (loy ? 1 : 0), bz = cz - (loz ? 1 : 0);
-                const float vx[2] = {lox ? 0.5f - fx : 1.5f - fx, lox ? 0.5f + fx : fx - 0.5f};
-                const float vy[2] = {loy ? 0.5f - fy : 1.5f - fy, loy ? 0.5f + fy : fy - 0.5f};
-                const float vz[2] = {loz ? 0.5f - fz : 1.5f - fz, loz ? 0.5f + fz : fz - 0.5f};
+            {
+                const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {q * (1.0f - fz), q * fz};
+                unsigned long long* a = &a1[((cz + 1) * kN1 + cy + 1) * kN1 + cx + 1];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                    const int nx = bx + dx, ny = by + dy, nz = bz + dz;
-                    if ((unsigned)nx < kTile && (unsigned)ny < kTile && (unsigned)nz < kTile)
-                        lds_add(&a2[(nz * kTile + ny) * kTile + nx], fix_round(q * (vx[dx] * vy[dy] * vz[dz])));
+                    lds_add(a + (dz * kN1 + dy) * kN1 + dx, fix_round(wz[dz] * wy[dy] * wx[dx]));
+                }
+            }
+            if (BOTH) {
+                // the two interlaced nodes around the particle per axis: centres of cells (c-1, c) if
+                // f < 1/2, else (c, c+1); the centre of cell m has index m + 2
+                const bool lox = fx < 0.5f, loy = fy < 0.5f, loz = fz < 0.5f;
+                const int bx = cx + (lox ? 1 : 2), by = cy + (loy ? 1 : 2), bz = cz + (loz ? 1 : 2);
+                const float vx[2] = {lox ? 0.5f - fx : 1.5f - fx, lox ? 0.5f + fx : fx - 0.5f};
+                const float vy[2] = {loy ? 0.5f - fy : 1.5f - fy, loy ? 0.5f + fy : fy - 0.5f};
+                const float vz[2] = {q * (loz ? 0.5f - fz : 1.5f - fz), q * (loz ? 0.5f + fz : fz - 0.5f)};
+                unsigned long long* a = &a2[(bz * kN2 + by) * kN2 + bx];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+                    lds_add(a + (dz * kN2 + dy) * kN2 + dx, fix_round(vz[dz] * vy[dy] * vx[dx]));
                 }
             }
         }
@@ -384,10 +425,11 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
     // node at the centre of cell o + l, stored at index o + l + 1
     for (int i = tid; i < kTile * kTile * kTile; i += 256) {
         const int lx = i % kTile, ly = (i / kTile) % kTile, lz = i / (kTile * kTile);
-        rho[((int64_t)(oz + lz) * G + (oy + ly)) * G + (ox + lx)] = (float)((double)(long long)a1[i] * unit);
+        rho[((int64_t)(oz + lz) * G + (oy + ly)) * G + (ox + lx)] =
+            (float)((double)(long long)a1[((lz + 1) * kN1 + ly + 1) * kN1 + lx + 1] * unit);
         if (BOTH)
             rho2[((int64_t)((oz + lz + 1) & msk) * G + ((oy + ly + 1) & msk)) * G + ((ox + lx + 1) & msk)] =
-                (float)((double)(long long)a2[i] * unit);
+                (float)((double)(long long)a2[((lz + 2) * kN2 + ly + 2) * kN2 + lx + 2] * unit);
     }
 }
 
@@ -680,36 +722,42 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
         double* c8 = (double*)c.dense_corner;
         double* c27 = c8 + 8 * nd_cap;
         if (c.p.interlace) {
-            k_dense_reduce_both<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                        c.m_plus, -c.m_minus, c8, c27);
+            k_dense_reduce_both<double><<<dim3(nbd), dim3(256), 0, s>>>(
+                c.parts, c.cell_start, list, c.dense_slot, count, c.logG, c.m_plus, -c.m_minus, c8, c27);
+            k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
+            k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 27, c27);
             k_deposit_both<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
                                                                      c.logG, c.m_plus, -c.m_minus, (double*)c.rho,
                                                                      (double*)c.rho2);
         } else {
-            k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                          c.m_plus, -c.m_minus, c8);
+            k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot,
+                                                                   count, c.logG, c.m_plus, -c.m_minus, c8);
+            k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
             k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG,
                                                                 c.m_plus, -c.m_minus, (double*)c.rho);
         }
     } else {
         float* c8 = (float*)c.dense_corner;
         float* c27 = c8 + 8 * nd_cap;
-        // charges scaled by 2^k, |q| 2^k <= 2^32 (exact: a power of two); the grids' unit is 2^-k
+        // charges scaled by 2^k, |q| 2^k <= 2^30 (exact: a power of two); the grids' unit is 2^-k
         int ex = 0;
         (void)std::frexp(c.m_plus > c.m_minus ? c.m_plus : c.m_minus, &ex);  // max |q| <= 2^ex
-        const int k = 32 - ex;
+        const int k = 30 - ex;
         const float qp = (float)std::ldexp(c.m_plus, k), qm = (float)-std::ldexp(c.m_minus, k);
         const double unit = std::ldexp(1.0, -k);
         const unsigned ntile = (unsigned)(G3 >> 9);  // 8^3-cell tiles (G >= 16: a multiple of 8)
         if (c.p.interlace) {
-            k_dense_reduce_both<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                       qp, qm, c8, c27);
+            k_dense_reduce_both<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot,
+                                                                       count, c.logG, qp, qm, c8, c27);
+            k_dense_combine<float><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
+            k_dense_combine<float><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 27, c27);
             k_deposit_tile<true><<<dim3(ntile), dim3(256), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
                                                                    c.logG, qp, qm, unit, (float*)c.rho,
                                                                    (float*)c.rho2);
         } else {
-            k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, count, c.logG,
-                                                                         qp, qm, c8);
+            k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot,
+                                                                  count, c.logG, qp, qm, c8);
+            k_dense_combine<float><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
             k_deposit_tile<false><<<dim3(ntile), dim3(256), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, nullptr,
                                                                     c.logG, qp, qm, unit, (float*)c.rho, nullptr);
         }
