@@ -168,13 +168,25 @@ int  jtp_diagnostics(jtp_ctx* ctx, int64_t n, const void* d_vel, const int8_t* d
 
 /* Total potential energy of the two-species system, FP64 (for energy-drift diagnostics; needs a
  * JTP_FP64 context with JTP_SPLIT_ERFC_GAUSS, tree and PM enabled).  Runs one force evaluation whose
- * walk also sums the short-range potential -sum_j q_j erfc(rho/2r_s)/rho, interpolates the mesh
- * potential at the particles and removes the mesh self-energy of each particle exactly:
- * W = 1/2 sum_i |m_i| sigma_i (Phi_mesh(x_i) + Phi_short_i) - 1/2 sum_i m_i^2 S_i.
+ * walk also sums the short-range potential -sum_j q_j erfc(rho/2r_s)/rho exactly (every node opened),
+ * interpolates the mesh potential at the particles and removes the mesh self-energy of each particle:
+ * W = 1/2 sum_i |m_i| sigma_i (Phi_mesh(x_i) + Phi_short_i) - 1/2 sum_i m_i^2 S_i
+ *     + 2 pi G r_s^2 Q^2 / V (background) + 1/2 G sum_i m_i^2 C (own periodic images),
+ * Q = sum_i |m_i| sigma_i, C = 2.837297479 / L - 4 pi r_s^2 / V + sum_n erfc(|n| L / 2 r_s) / |n| L.
  * (The reference estimates PE from the grid only: potential_energy_binding_sampled,
  * nbody_gpu_twopass.rs:5280-5307.) */
 int  jtp_potential_energy(jtp_ctx* ctx, int64_t n, const uint32_t* d_pos_fixed, const int8_t* d_species,
                           double* energy, void* stream);
+
+/* Segregation of the two species on an n_cells^3 nearest-grid-point count grid (1 <= n_cells <= 1024;
+ * cell of a fixed-point coordinate u: floor(u n_cells / 2^32)):
+ *   out[0] = Var(n+ - n-) / (mean+ + mean-)          ("Diff/Pois")
+ *   out[1] = corr(delta+, delta-), delta = n / mean - 1
+ *   out[2] = (Var(n-) - mean-) / mean-               (excess variance of the m- counts)
+ * Replaces compute_segregation_metrics, src/bin/janus_mu19_production.rs:219-273 (host loops over
+ * f32 copies of the positions): counts with integer atomics, FP64 moments reduced in fixed order. */
+int  jtp_segregation(jtp_ctx* ctx, int64_t n, const uint32_t* d_pos_fixed, const int8_t* d_species,
+                     int32_t n_cells, double out[3], void* stream);
 
 /* ---- ensembles of independent boxes (no counterpart in the reference, which runs one box per
  * process; the batch replaces a loop of step_treepm_gpu / TreePMForce calls over boxes).
@@ -228,6 +240,54 @@ int   jtp_copy_d2d(void* d_dst, const void* d_src, size_t bytes);
 int   jtp_device_synchronize(void);
 int   jtp_device_count(int* out);
 int   jtp_device_memory(size_t* free_bytes, size_t* total_bytes);  /* hipMemGetInfo of the current device */
+
+/* ---- snapshots
+ * The reference's two on-disk formats (little-endian):
+ *   JTP_SNAP_PRODUCTION  save_snapshot, src/bin/janus_mu19_production.rs:199-216: u32 N, f32 L, u32 step,
+ *                        f32 z, then per particle 3 f32 position, 3 f32 velocity, i8 sign (25 bytes, packed).
+ *   JTP_SNAP_V3          src/snapshot_v3.rs:14-16,108-157,313-335: a 408-byte header (magic "JANUSV3\n",
+ *                        version 3, header_size, n_total, a, t_gyr, l_box, h0, mu, omega_b, base masses and
+ *                        softenings, n_split_max, seed_ic, z_init, n_stars, z_start_run, sfr, rho_max,
+ *                        256-byte run label), then per particle 3 f32 position, 3 f32 velocity, f32 mass,
+ *                        f32 epsilon, u8 sign (1 = m+, 255 = m-), u8 split_level, u8 is_star, u8 flags (36 B).
+ * Positions on disk are f32 in [-L/2, L/2) (this library's box convention), converted from / to the
+ * fixed-point device arrays on the GPU; velocities are stored as f32 of (device value x velocity_scale)
+ * and read back divided by it (device f32 or f64, vel_f64 = 0 / 1).  Records are packed and unpacked
+ * on the GPU in chunks of 2^20 and the file moves through pinned host buffers.  Writing v3 fills
+ * mass and epsilon per species from the metadata and split_level = is_star = flags = 0 (as the
+ * reference's writers do); reading v3 loads position, velocity and sign only (one mass per species
+ * drives the force path).  A v3 header_size above 408 is skipped over (snapshot_v3.rs:425-431).
+ * Errors of these calls (no context) are reported by jtp_io_last_error(). */
+#define JTP_SNAP_PRODUCTION 1
+#define JTP_SNAP_V3         3
+typedef struct jtp_snap_meta {
+    int32_t format;            /* JTP_SNAP_* */
+    int64_t n;                 /* particle count (written: the arrays' length; read: the file's) */
+    double box_size;           /* L (v3: l_box) */
+    uint32_t step;             /* production format */
+    double z;                  /* production: redshift; v3 read: 1/a - 1 */
+    double velocity_scale;     /* file velocity = device velocity x velocity_scale (default 1) */
+    double a, t_gyr, h0, mu, omega_b;                                      /* v3 header */
+    double m_part_plus_base, m_part_minus_base, eps_plus_base, eps_minus_base;
+    uint32_t n_split_max, seed_ic;
+    double z_init;
+    uint64_t n_stars;
+    double z_start_run, sfr, rho_max;
+    char run_label[256];
+    float mass_plus, mass_minus, eps_plus, eps_minus;                      /* v3 per-particle fields */
+} jtp_snap_meta;
+
+/* Defaults of SnapshotHeaderV3::default (snapshot_v3.rs:48-76), velocity_scale 1, the given format and L. */
+void  jtp_snap_meta_default(jtp_snap_meta* meta, int32_t format, double box_size);
+/* Writes meta->n particles (meta->format) to path.  Replaces save_snapshot / write_snapshot_v3. */
+int   jtp_snapshot_write(const char* path, const jtp_snap_meta* meta, const uint32_t* d_pos_fixed,
+                         const void* d_vel, int32_t vel_f64, const int8_t* d_species, void* stream);
+/* Header only (read_header_only, snapshot_v3.rs:442-463; magic and version checked). */
+int   jtp_snapshot_read_meta(const char* path, int32_t format, jtp_snap_meta* meta);
+/* Loads n particles (n must equal the file's count) into device arrays.  Replaces read_snapshot_v3. */
+int   jtp_snapshot_read(const char* path, int32_t format, int64_t n, uint32_t* d_pos_fixed, void* d_vel,
+                        int32_t vel_f64, int8_t* d_species, double velocity_scale, void* stream);
+const char* jtp_io_last_error(void);
 
 #ifdef __cplusplus
 }

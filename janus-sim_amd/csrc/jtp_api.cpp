@@ -325,7 +325,7 @@ int jtp_destroy(jtp_ctx* h) {
     if (c.graph.capture_stream) (void)hipStreamDestroy(c.graph.capture_stream);
     void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.grad, c.grad2, c.spec, c.twiddle, c.gtab, c.cell_start,
                     c.cell4, c.cellnode, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch,
-                    c.pot_short, c.acc_scratch, c.kself};
+                    c.pot_short, c.acc_scratch, c.kself, c.seg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c.host_pinned) (void)hipHostFree(c.host_pinned);
@@ -574,6 +574,39 @@ int jtp_diagnostics(jtp_ctx* h, int64_t n, const void* d_vel, const int8_t* d_sp
         out->momentum[1][k] = host[5 + k];
     }
     out->pm_energy = c.p.interlace ? 0.25 * (host[8] + host[9]) : 0.5 * host[8];
+    return JTP_OK;
+}
+
+int jtp_segregation(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_t* d_species, int32_t n_cells,
+                    double out[3], void* stream) {
+    if (!h || !out) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    if (n_cells < 1 || n_cells > 1024) return fail(&c, JTP_ERR_ARG, "n_cells must be in [1, 1024]");
+    if (n < 0 || n > (int64_t)0x7fffffff) return fail(&c, JTP_ERR_ARG, "n out of range");
+    if (n > 0 && (!d_pos || !d_species)) return fail(&c, JTP_ERR_ARG, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t cells = (int64_t)n_cells * n_cells * n_cells;
+    if (c.seg_cap < cells) {
+        JTP_HIP(&c, hipStreamSynchronize(s));
+        if (c.seg) (void)hipFree(c.seg);
+        c.seg = nullptr;
+        c.seg_cap = 0;
+        JTP_HIP(&c, dalloc(c.seg, (size_t)(2 * cells + 2)));
+        c.seg_cap = cells;
+    }
+    double* res = c.diag_scratch + 1024 * 8;
+    JTP_HIP(&c, launch_segregation(n, d_pos, d_species, n_cells, c.seg, c.diag_scratch, res, s));
+    double v[9];
+    JTP_HIP(&c, hipMemcpyAsync(v, res, sizeof(v), hipMemcpyDeviceToHost, s));
+    JTP_HIP(&c, hipStreamSynchronize(s));
+    // janus_mu19_production.rs:240-272
+    const double C = (double)cells, mp = v[7] / C, mm = v[8] / C;
+    const double var_diff = v[0] / C, pois = mp + mm;
+    out[0] = pois > 0 ? var_diff / pois : 0.0;
+    const double var_p = v[3] / C - (v[1] / C) * (v[1] / C), var_m = v[4] / C - (v[2] / C) * (v[2] / C);
+    const double cov = v[5] / C - (v[1] / C) * (v[2] / C);
+    out[1] = (var_p > 0 && var_m > 0) ? cov / (std::sqrt(var_p) * std::sqrt(var_m)) : 0.0;
+    out[2] = mm > 0 ? (v[6] / C - mm) / mm : 0.0;
     return JTP_OK;
 }
 

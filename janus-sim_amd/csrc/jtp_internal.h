@@ -153,6 +153,8 @@ struct Ctx {
     double* acc_scratch = nullptr;    // energy diagnostic: forces of that evaluation (3 cap_n)
     int64_t pot_cap = 0;
     double* kself = nullptr;          // 27 values of the mesh potential of a unit mass, offsets {-1,0,1}^3
+    int32_t* seg = nullptr;           // segregation counts: int32 [cells][2] + 2 totals
+    int64_t seg_cap = 0;              // cells the buffer holds
     bool kself_ready = false;
     StepParams* dev_step = nullptr;   // device copy of the step scalars
     StepParams* host_step = nullptr;  // pinned staging of the step scalars
@@ -209,6 +211,8 @@ hipError_t launch_drift(Ctx& c, uint32_t* pos, const void* vel, int64_t n, hipSt
 hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* species, double* scratch,
                               double* out, hipStream_t s);
 constexpr int kDiagScratch = 1024 * 8 + 16;  // doubles
+hipError_t launch_segregation(int64_t n, const uint32_t* pos, const int8_t* species, int32_t n_cells,
+                              int32_t* counts, double* scratch, double* out, hipStream_t s);
 hipError_t launch_energy_terms(Ctx& c, int64_t n, const double* kself, const double* pot_short, double* scratch,
                                double* out, hipStream_t s);
 hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s);

@@ -146,12 +146,14 @@ hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* 
                                                                       c.m_minus, part);
         k_diag_final<<<dim3(1), dim3(256), 0, s>>>(part, kDiagBlocks, kDiagVals, out);
     } else {
-        launch_fill_i32((int32_t*)out, 2 * kDiagVals, 0, s);
+        const hipError_t e = launch_fill_i32((int32_t*)out, 2 * kDiagVals, 0, s);
+        if (e != hipSuccess) return e;
     }
     const void* grids[2][2] = {{c.rho, c.phi}, {c.rho2, c.phi2}};
     for (int g = 0; g < 2; ++g) {
         if (!c.p.pm_enabled || (g == 1 && !c.p.interlace)) {
-            launch_fill_i32((int32_t*)(out + kDiagVals + g), 2, 0, s);
+            const hipError_t e = launch_fill_i32((int32_t*)(out + kDiagVals + g), 2, 0, s);
+            if (e != hipSuccess) return e;
             continue;
         }
         if (c.p.precision == JTP_FP64)
@@ -244,6 +246,74 @@ hipError_t launch_energy_terms(Ctx& c, int64_t n, const double* kself, const dou
                                                            c.p.interlace ? (const double*)c.phi2 : nullptr, c.logG,
                                                            kself, pot_short, c.m_plus, c.m_minus, scratch);
     k_diag_final<<<dim3(1), dim3(256), 0, s>>>(scratch, kDiagBlocks, 5, out);
+    return hipGetLastError();
+}
+
+}  // namespace jtp
+
+// ------------------------------------------------------------------ segregation metrics
+// Counts per nearest-grid-point cell and species (integer atomics: exact, order-free), totals per
+// species; then, per cell, with mean = total / cells: (d - (mean+ - mean-))^2 for d = n+ - n-,
+// delta+, delta-, delta+^2, delta-^2, delta+ delta-, (n- - mean-)^2, reduced in FP64 in fixed order.
+namespace jtp {
+
+__device__ inline int64_t seg_cell(uint32_t u, int32_t nc) { return (int64_t)(((uint64_t)u * (uint32_t)nc) >> 32); }
+
+__global__ __launch_bounds__(256) void k_seg_count(const uint32_t* __restrict__ pos,
+                                                   const int8_t* __restrict__ species, int64_t n, int32_t nc,
+                                                   int32_t* __restrict__ counts, int32_t* __restrict__ totals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const bool plus = live && species[i] > 0;
+    if (live) {
+        const int64_t cell =
+            (seg_cell(pos[3 * i + 2], nc) * nc + seg_cell(pos[3 * i + 1], nc)) * nc + seg_cell(pos[3 * i], nc);
+        atomicAdd(&counts[2 * cell + (plus ? 0 : 1)], 1);
+    }
+    const uint64_t mp = __ballot(plus), mm = __ballot(live && !plus);
+    if ((threadIdx.x & 63) == 0) {
+        if (mp) atomicAdd(&totals[0], (int32_t)__popcll(mp));
+        if (mm) atomicAdd(&totals[1], (int32_t)__popcll(mm));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_moments(const int32_t* __restrict__ counts,
+                                                     const int32_t* __restrict__ totals, int64_t cells,
+                                                     double* __restrict__ partial) {
+    const double mp = (double)totals[0] / (double)cells, mm = (double)totals[1] / (double)cells;
+    const double ed = mp - mm;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (int64_t)gridDim.x * blockDim.x) {
+        const double np = (double)counts[2 * c], nm = (double)counts[2 * c + 1];
+        const double dd = np - nm - ed;
+        const double dp = mp > 0 ? (np - mp) / mp : 0.0, dm = mm > 0 ? (nm - mm) / mm : 0.0;
+        v[0] += dd * dd;
+        v[1] += dp;
+        v[2] += dm;
+        v[3] += dp * dp;
+        v[4] += dm * dm;
+        v[5] += dp * dm;
+        v[6] += (nm - mm) * (nm - mm);
+    }
+    block_reduce_store<7>(v, partial + (int64_t)blockIdx.x * 7);
+}
+
+// out[0..6] = the sums above, out[7], out[8] = the species totals (as doubles)
+__global__ void k_seg_totals(const int32_t* __restrict__ totals, double* __restrict__ out) {
+    if (threadIdx.x < 2) out[7 + threadIdx.x] = (double)totals[threadIdx.x];
+}
+
+hipError_t launch_segregation(int64_t n, const uint32_t* pos, const int8_t* species, int32_t n_cells,
+                              int32_t* counts, double* scratch, double* out, hipStream_t s) {
+    const int64_t cells = (int64_t)n_cells * n_cells * n_cells;
+    hipError_t e = launch_fill_i32(counts, 2 * cells + 2, 0, s);
+    if (e != hipSuccess) return e;
+    int32_t* totals = counts + 2 * cells;
+    if (n > 0)
+        k_seg_count<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(pos, species, n, n_cells, counts, totals);
+    k_seg_moments<<<dim3(kDiagBlocks), dim3(256), 0, s>>>(counts, totals, cells, scratch);
+    k_diag_final<<<dim3(1), dim3(256), 0, s>>>(scratch, kDiagBlocks, 7, out);
+    k_seg_totals<<<dim3(1), dim3(64), 0, s>>>(totals, out);
     return hipGetLastError();
 }
 

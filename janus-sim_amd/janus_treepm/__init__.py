@@ -43,6 +43,8 @@ EXPORTED = [
     "jtp_copy_d2h", "jtp_copy_d2d", "jtp_device_synchronize", "jtp_device_count", "jtp_step",
     "jtp_ensemble_create", "jtp_ensemble_destroy", "jtp_ensemble_last_error", "jtp_ensemble_compute_forces",
     "jtp_ensemble_get_stats", "jtp_diagnostics", "jtp_potential_energy", "jtp_device_memory",
+    "jtp_segregation", "jtp_snap_meta_default", "jtp_snapshot_write", "jtp_snapshot_read_meta", "jtp_snapshot_read",
+    "jtp_io_last_error",
 ]
 
 
@@ -65,6 +67,22 @@ class Params(C.Structure):
 
 class Diag(C.Structure):
     _fields_ = [("kinetic", C.c_double * 2), ("momentum", (C.c_double * 3) * 2), ("pm_energy", C.c_double)]
+
+
+SNAP_PRODUCTION, SNAP_V3 = 1, 3
+
+
+class SnapMeta(C.Structure):
+    """jtp_snap_meta (include/janus_treepm.h)."""
+    _fields_ = [
+        ("format", C.c_int32), ("n", C.c_int64), ("box_size", C.c_double), ("step", C.c_uint32), ("z", C.c_double),
+        ("velocity_scale", C.c_double), ("a", C.c_double), ("t_gyr", C.c_double), ("h0", C.c_double),
+        ("mu", C.c_double), ("omega_b", C.c_double), ("m_part_plus_base", C.c_double),
+        ("m_part_minus_base", C.c_double), ("eps_plus_base", C.c_double), ("eps_minus_base", C.c_double),
+        ("n_split_max", C.c_uint32), ("seed_ic", C.c_uint32), ("z_init", C.c_double), ("n_stars", C.c_uint64),
+        ("z_start_run", C.c_double), ("sfr", C.c_double), ("rho_max", C.c_double), ("run_label", C.c_char * 256),
+        ("mass_plus", C.c_float), ("mass_minus", C.c_float), ("eps_plus", C.c_float), ("eps_minus", C.c_float),
+    ]
 
 
 class Stats(C.Structure):
@@ -124,6 +142,12 @@ def load_library(path: str = LIB_PATH):
         "jtp_diagnostics": (C.c_int, [vp, i64, vp, vp, C.POINTER(Diag), vp]),
         "jtp_potential_energy": (C.c_int, [vp, i64, vp, vp, C.POINTER(dbl), vp]),
         "jtp_device_memory": (C.c_int, [C.POINTER(sz), C.POINTER(sz)]),
+        "jtp_segregation": (C.c_int, [vp, i64, vp, vp, i32, C.POINTER(dbl), vp]),
+        "jtp_snap_meta_default": (None, [C.POINTER(SnapMeta), i32, dbl]),
+        "jtp_snapshot_write": (C.c_int, [C.c_char_p, C.POINTER(SnapMeta), vp, vp, i32, vp, vp]),
+        "jtp_snapshot_read_meta": (C.c_int, [C.c_char_p, i32, C.POINTER(SnapMeta)]),
+        "jtp_snapshot_read": (C.c_int, [C.c_char_p, i32, i64, vp, vp, i32, vp, dbl, vp]),
+        "jtp_io_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -141,6 +165,41 @@ def device_memory() -> tuple:
     if rc != JTP_OK:
         raise JtpError(rc, "hipMemGetInfo failed")
     return f.value, t.value
+
+
+def snap_meta(fmt: int, box_size: float, **fields) -> SnapMeta:
+    """Metadata with the library's defaults (SnapshotHeaderV3::default), then `fields`."""
+    m = SnapMeta()
+    load_library().jtp_snap_meta_default(C.byref(m), int(fmt), float(box_size))
+    for k, v in fields.items():
+        setattr(m, k, v.encode() if k == "run_label" and isinstance(v, str) else v)
+    return m
+
+
+def _io_check(rc):
+    if rc != JTP_OK:
+        raise JtpError(rc, load_library().jtp_io_last_error().decode())
+
+
+def write_snapshot(path: str, meta: SnapMeta, d_pos_fixed, d_vel, d_species, vel_f64: bool = False, stream=None):
+    """Write meta.n particles from device arrays (jtp_snapshot_write)."""
+    ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+    _io_check(load_library().jtp_snapshot_write(path.encode(), C.byref(meta), ptr(d_pos_fixed), ptr(d_vel),
+                                                int(vel_f64), ptr(d_species), stream))
+
+
+def read_snapshot_meta(path: str, fmt: int) -> SnapMeta:
+    m = SnapMeta()
+    _io_check(load_library().jtp_snapshot_read_meta(path.encode(), int(fmt), C.byref(m)))
+    return m
+
+
+def read_snapshot(path: str, fmt: int, n: int, d_pos_fixed, d_vel, d_species, vel_f64: bool = False,
+                  velocity_scale: float = 1.0, stream=None):
+    """Load the file's n particles into device arrays (jtp_snapshot_read)."""
+    ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+    _io_check(load_library().jtp_snapshot_read(path.encode(), int(fmt), int(n), ptr(d_pos_fixed), ptr(d_vel),
+                                               int(vel_f64), ptr(d_species), float(velocity_scale), stream))
 
 
 def device_count() -> int:
@@ -259,6 +318,14 @@ class ForceContext:
         self._check(self.lib.jtp_potential_energy(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), C.byref(w),
                                                   stream))
         return w.value
+
+    def segregation(self, n: int, d_pos_fixed, d_species, n_cells: int, stream=None) -> tuple:
+        """(Diff/Pois, corr(delta+, delta-), excess variance of m-) on an n_cells^3 count grid."""
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        out = (C.c_double * 3)()
+        self._check(self.lib.jtp_segregation(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), int(n_cells), out,
+                                             stream))
+        return tuple(out)
 
     def compute_forces(self, pos_fixed: np.ndarray, species: np.ndarray) -> np.ndarray:
         """Host convenience: upload, compute, download (float32 or float64 [n,3])."""

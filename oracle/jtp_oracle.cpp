@@ -18,6 +18,9 @@
 //   orc_ewald              FP64 Ewald direct summation with Plummer correction of the nearest image
 //   orc_refcpu_*           TreePMForce (treepm_force.rs:35-131) + TreePMTree (tree_short.rs:49-247) +
 //                          PmGrid two-grid solve, restated for the CPU baseline and compat parity
+//   orc_segregation*       compute_segregation_metrics (src/bin/janus_mu19_production.rs:219-273): the
+//                          _f32_ref form bins f32 positions exactly as the reference does, the fixed form
+//                          bins 32-bit fixed-point coordinates by floor(u n / 2^32)
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -796,5 +799,72 @@ void orc_refcpu_forces(void* h, int64_t nt, const int64_t* tidx, int pm, int tre
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+// ------------------------------------------------------------------ segregation metrics
+// janus_mu19_production.rs:238-272, from per-cell counts
+static void seg_metrics(const std::vector<int32_t>& np, const std::vector<int32_t>& nm, double* out) {
+    const size_t nc = np.size();
+    double tp = 0, tm = 0;
+    for (size_t i = 0; i < nc; ++i) {
+        tp += np[i];
+        tm += nm[i];
+    }
+    const double mean_p = tp / (double)nc, mean_m = tm / (double)nc;
+    double var_diff = 0;
+    for (size_t i = 0; i < nc; ++i) {
+        const double d = (double)np[i] - (double)nm[i] - (mean_p - mean_m);
+        var_diff += d * d;
+    }
+    var_diff /= (double)nc;
+    const double pois = mean_p + mean_m;
+    out[0] = pois > 0 ? var_diff / pois : 0.0;
+    double sp = 0, sm = 0, spp = 0, smm = 0, spm = 0;
+    for (size_t i = 0; i < nc; ++i) {
+        const double dp = mean_p > 0 ? ((double)np[i] - mean_p) / mean_p : 0.0;
+        const double dm = mean_m > 0 ? ((double)nm[i] - mean_m) / mean_m : 0.0;
+        sp += dp;
+        sm += dm;
+        spp += dp * dp;
+        smm += dm * dm;
+        spm += dp * dm;
+    }
+    const double n = (double)nc;
+    const double vp = spp / n - (sp / n) * (sp / n), vm = smm / n - (sm / n) * (sm / n);
+    const double cov = spm / n - (sp / n) * (sm / n);
+    out[1] = (vp > 0 && vm > 0) ? cov / (std::sqrt(vp) * std::sqrt(vm)) : 0.0;
+    double vmc = 0;
+    for (size_t i = 0; i < nc; ++i) vmc += ((double)nm[i] - mean_m) * ((double)nm[i] - mean_m);
+    vmc /= (double)nc;
+    out[2] = mean_m > 0 ? (vmc - mean_m) / mean_m : 0.0;
+}
+
+// the reference's binning (janus_mu19_production.rs:220-236) of f32 positions in [-L/2, L/2)
+void orc_segregation_f32_ref(int64_t n, const float* pos, const int8_t* sp, double L, int nc, double* out) {
+    const double cell = L / nc, half = L / 2.0;
+    std::vector<int32_t> np((size_t)nc * nc * nc, 0), nm((size_t)nc * nc * nc, 0);
+    auto bin = [&](float v) {
+        double w = std::fmod((double)v + half, L);
+        if (w < 0) w += L;  // rem_euclid
+        return (size_t)(w / cell) % (size_t)nc;
+    };
+    for (int64_t i = 0; i < n; ++i) {
+        const size_t idx = (bin(pos[3 * i]) * nc + bin(pos[3 * i + 1])) * nc + bin(pos[3 * i + 2]);
+        if (sp[i] > 0) ++np[idx];
+        else ++nm[idx];
+    }
+    seg_metrics(np, nm, out);
+}
+
+// fixed-point binning: cell = floor(u nc / 2^32)
+void orc_segregation(int64_t n, const uint32_t* pos, const int8_t* sp, int nc, double* out) {
+    std::vector<int32_t> np((size_t)nc * nc * nc, 0), nm((size_t)nc * nc * nc, 0);
+    auto bin = [&](uint32_t u) { return (size_t)(((uint64_t)u * (uint64_t)nc) >> 32); };
+    for (int64_t i = 0; i < n; ++i) {
+        const size_t idx = (bin(pos[3 * i]) * nc + bin(pos[3 * i + 1])) * nc + bin(pos[3 * i + 2]);
+        if (sp[i] > 0) ++np[idx];
+        else ++nm[idx];
+    }
+    seg_metrics(np, nm, out);
+}
 
 }  // extern "C"

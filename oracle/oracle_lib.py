@@ -40,6 +40,8 @@ class Oracle:
             "orc_refcpu_update": (None, [vp, i64, vp, vp, vp, i32, i32]),
             "orc_refcpu_forces": (None, [vp, i64, vp, i32, i32, vp]),
             "orc_num_threads": (C.c_int, []),
+            "orc_segregation": (None, [i64, vp, vp, i32, vp]),
+            "orc_segregation_f32_ref": (None, [i64, vp, vp, dbl, i32, vp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -93,6 +95,20 @@ class Oracle:
         self.lib.orc_node_sums(n, spos.ctypes.data, ssp.ctypes.data, nodes.ctypes.data, sums.ctypes.data,
                                boxes.ctypes.data)
         return sums, boxes
+
+    def segregation(self, pos_fixed, sp, n_cells):
+        pos = np.ascontiguousarray(pos_fixed, np.uint32)
+        sp = np.ascontiguousarray(sp, np.int8)
+        out = np.zeros(3)
+        self.lib.orc_segregation(len(sp), pos.ctypes.data, sp.ctypes.data, n_cells, out.ctypes.data)
+        return out
+
+    def segregation_f32_ref(self, pos_f32, sp, L, n_cells):
+        pos = np.ascontiguousarray(pos_f32, np.float32)
+        sp = np.ascontiguousarray(sp, np.int8)
+        out = np.zeros(3)
+        self.lib.orc_segregation_f32_ref(len(sp), pos.ctypes.data, sp.ctypes.data, L, n_cells, out.ctypes.data)
+        return out
 
     def deposit(self, pos_fixed, sp, mp, mm, G):
         p = np.ascontiguousarray(pos_fixed, np.uint32).reshape(-1, 3)
