@@ -241,6 +241,30 @@ int   jtp_device_synchronize(void);
 int   jtp_device_count(int* out);
 int   jtp_device_memory(size_t* free_bytes, size_t* total_bytes);  /* hipMemGetInfo of the current device */
 
+/* ---- initial conditions
+ * Zel'dovich lattice ICs on the GPU: n_grid^3 particles, one per cell centre, displaced by
+ * psi = Re IFFT[-i k delta(k) / k^2] / n^3 with P(k) = k^(n_s - 4) delta_rms^2 and D = 1/(1 + z_init),
+ * rescaled so that max |psi| = 0.3 lattice spacing; velocity = psi x H0 sqrt(1 + z_init) in Mpc/Gyr
+ * (H0 = 69.9 km/s/Mpc); round(n^3 / (1 + mu)) randomly chosen cells hold m+, the rest m-.
+ * Replaces generate_zeldovich_ics (src/ic_gen.rs:51-169, 179-309).  The context must have
+ * n_cells == n_grid and box_size == p->box_size; its FFT precision is the computation's.  Random
+ * numbers come from SplitMix64 streams keyed by the seed (not the reference's StdRng streams: the
+ * same seed gives a different realisation with the same statistics).  Particle i is cell i,
+ * i = (iz n + iy) n + ix as in the reference.  *n_plus receives the m+ count. */
+typedef struct jtp_ic_params {
+    double box_size;   /* Mpc */
+    int32_t n_grid;    /* particles per dimension (a power of two in [16, 512]) */
+    double z_init;
+    uint64_t seed;
+    double mu;         /* |m-| / m+: sets the m+ fraction 1 / (1 + mu) */
+    double n_s;        /* spectral index */
+    double delta_rms;  /* overdensity amplitude */
+} jtp_ic_params;
+/* IcParams::default (ic_gen.rs:24-36): L = 50, n = 128, z = 10, seed 42, mu 19, n_s 0.965, 0.10 */
+void  jtp_ic_default_params(jtp_ic_params* p);
+int   jtp_generate_ics(jtp_ctx* ctx, const jtp_ic_params* p, uint32_t* d_pos_fixed, void* d_vel, int32_t vel_f64,
+                       int8_t* d_species, int64_t* n_plus, void* stream);
+
 /* ---- snapshots
  * The reference's two on-disk formats (little-endian):
  *   JTP_SNAP_PRODUCTION  save_snapshot, src/bin/janus_mu19_production.rs:199-216: u32 N, f32 L, u32 step,

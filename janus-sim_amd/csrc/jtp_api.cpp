@@ -577,6 +577,38 @@ int jtp_diagnostics(jtp_ctx* h, int64_t n, const void* d_vel, const int8_t* d_sp
     return JTP_OK;
 }
 
+void jtp_ic_default_params(jtp_ic_params* p) {
+    if (!p) return;
+    p->box_size = 50.0;
+    p->n_grid = 128;
+    p->z_init = 10.0;
+    p->seed = 42;
+    p->mu = 19.0;
+    p->n_s = 0.965;
+    p->delta_rms = 0.10;
+}
+
+int jtp_generate_ics(jtp_ctx* h, const jtp_ic_params* p, uint32_t* d_pos, void* d_vel, int32_t vel_f64,
+                     int8_t* d_species, int64_t* n_plus_out, void* stream) {
+    if (!h || !p) return JTP_ERR_ARG;
+    Ctx& c = h->c;
+    if (p->n_grid != c.p.n_cells) return fail(&c, JTP_ERR_ARG, "n_grid must equal the context's n_cells");
+    if (p->box_size != c.p.box_size) return fail(&c, JTP_ERR_ARG, "box_size must equal the context's box_size");
+    if (!(p->mu >= 0) || !std::isfinite(p->n_s) || !(p->delta_rms > 0) || !(p->z_init > -1))
+        return fail(&c, JTP_ERR_ARG, "bad IC parameters");
+    if (!d_pos || !d_vel || !d_species) return fail(&c, JTP_ERR_ARG, "null device pointer");
+    const int64_t n = c.G3();
+    int rc = prepare(c, n);  // sort buffers for the sign keys
+    if (rc != JTP_OK) return rc;
+    const int64_t n_plus = (int64_t)std::llround((double)n / (1.0 + p->mu));  // ic_gen.rs:58-60
+    hipStream_t s = (hipStream_t)stream;
+    c.have_result = false;
+    JTP_HIP(&c, generate_ics(c, *p, d_pos, d_vel, vel_f64, d_species, n_plus, s));
+    JTP_HIP(&c, hipStreamSynchronize(s));
+    if (n_plus_out) *n_plus_out = n_plus;
+    return JTP_OK;
+}
+
 int jtp_segregation(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_t* d_species, int32_t n_cells,
                     double out[3], void* stream) {
     if (!h || !out) return JTP_ERR_ARG;

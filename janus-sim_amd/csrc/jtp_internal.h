@@ -201,6 +201,11 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s);           // grid <- su
 // phi <- Green * rho; matched = 1 multiplies by the gradient-stencil match (the force's Green
 // function when deconvolving; -1 = as the context's force), 0 leaves the potential unmatched
 hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched = -1);
+// out <- unnormalised inverse real 3-D FFT of the Hermitian half spectrum in c.spec (overwritten)
+hipError_t pm_inverse_fft(Ctx& c, void* out, hipStream_t s);
+// Zel'dovich lattice initial conditions of G^3 particles (G = the context's mesh)
+hipError_t generate_ics(Ctx& c, const jtp_ic_params& p, uint32_t* pos, void* vel, int vel_f64, int8_t* species,
+                        int64_t n_plus, hipStream_t s);
 hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s);            // pm_acc <- -sigma grad phi
 hipError_t pm_tables(Ctx& c);                                      // twiddles + Green tables (sync)
 

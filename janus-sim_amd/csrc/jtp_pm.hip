@@ -680,6 +680,50 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
     }
 }
 
+// Inverse z pass alone (the z pass of the solver without the forward transform and Green multiply).
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, int logG, const Cplx<R>* __restrict__ tw) {
+    __shared__ Cplx<R> buf[kFftPoints];
+    const int G = 1 << logG, H = G / 2 + 1;
+    const int KB = kFftPoints / G;
+    const int nkb = (H + KB - 1) / KB;
+    const int y = blockIdx.x / nkb, kx0 = (blockIdx.x % nkb) * KB;
+    const int64_t zs = (int64_t)G * H;
+    Cplx<R>* base = spec + (int64_t)y * H;
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int z = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        buf[z * KB + jb] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
+    }
+    __syncthreads();
+    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, true);
+    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+        const int z = e / KB, jb = e % KB;
+        const int kx = kx0 + jb;
+        if (kx < H) base[z * zs + kx] = buf[z * KB + jb];
+    }
+}
+
+template <typename R>
+static hipError_t inverse_fft_t(Ctx& c, R* out, hipStream_t s) {
+    const int G = (int)c.G(), H = G / 2 + 1, logG = c.logG;
+    const int npair = kFftPoints / G;
+    const int KB = kFftPoints / G;
+    const int nkb = (H + KB - 1) / KB;
+    const unsigned nx = (unsigned)((int64_t)G * G / (2 * npair));
+    const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
+    Cplx<R>* spec = (Cplx<R>*)c.spec;
+    k_fft_z_inv<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw);
+    k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 1);
+    k_fft_x_c2r<R><<<dim3(nx), dim3(256), 0, s>>>(spec, out, logG, tw);
+    return hipGetLastError();
+}
+
+hipError_t pm_inverse_fft(Ctx& c, void* out, hipStream_t s) {
+    if (c.p.precision == JTP_FP64) return inverse_fft_t<double>(c, (double*)out, s);
+    return inverse_fft_t<float>(c, (float*)out, s);
+}
+
 template <typename R>
 static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
     const int G = (int)c.G(), H = G / 2 + 1, logG = c.logG;

@@ -44,7 +44,7 @@ EXPORTED = [
     "jtp_ensemble_create", "jtp_ensemble_destroy", "jtp_ensemble_last_error", "jtp_ensemble_compute_forces",
     "jtp_ensemble_get_stats", "jtp_diagnostics", "jtp_potential_energy", "jtp_device_memory",
     "jtp_segregation", "jtp_snap_meta_default", "jtp_snapshot_write", "jtp_snapshot_read_meta", "jtp_snapshot_read",
-    "jtp_io_last_error",
+    "jtp_io_last_error", "jtp_ic_default_params", "jtp_generate_ics",
 ]
 
 
@@ -83,6 +83,12 @@ class SnapMeta(C.Structure):
         ("z_start_run", C.c_double), ("sfr", C.c_double), ("rho_max", C.c_double), ("run_label", C.c_char * 256),
         ("mass_plus", C.c_float), ("mass_minus", C.c_float), ("eps_plus", C.c_float), ("eps_minus", C.c_float),
     ]
+
+
+class IcParams(C.Structure):
+    """jtp_ic_params (include/janus_treepm.h)."""
+    _fields_ = [("box_size", C.c_double), ("n_grid", C.c_int32), ("z_init", C.c_double), ("seed", C.c_uint64),
+                ("mu", C.c_double), ("n_s", C.c_double), ("delta_rms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -148,6 +154,8 @@ def load_library(path: str = LIB_PATH):
         "jtp_snapshot_read_meta": (C.c_int, [C.c_char_p, i32, C.POINTER(SnapMeta)]),
         "jtp_snapshot_read": (C.c_int, [C.c_char_p, i32, i64, vp, vp, i32, vp, dbl, vp]),
         "jtp_io_last_error": (C.c_char_p, []),
+        "jtp_ic_default_params": (None, [C.POINTER(IcParams)]),
+        "jtp_generate_ics": (C.c_int, [vp, C.POINTER(IcParams), vp, vp, i32, vp, C.POINTER(i64), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -174,6 +182,15 @@ def snap_meta(fmt: int, box_size: float, **fields) -> SnapMeta:
     for k, v in fields.items():
         setattr(m, k, v.encode() if k == "run_label" and isinstance(v, str) else v)
     return m
+
+
+def ic_params(**fields) -> IcParams:
+    """IC parameters with IcParams::default (ic_gen.rs:24-36), then `fields`."""
+    p = IcParams()
+    load_library().jtp_ic_default_params(C.byref(p))
+    for k, v in fields.items():
+        setattr(p, k, v)
+    return p
 
 
 def _io_check(rc):
@@ -326,6 +343,15 @@ class ForceContext:
         self._check(self.lib.jtp_segregation(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), int(n_cells), out,
                                              stream))
         return tuple(out)
+
+    def generate_ics(self, params: "IcParams", d_pos_fixed, d_vel, d_species, vel_f64: bool = False,
+                     stream=None) -> int:
+        """Zel'dovich lattice ICs of n_grid^3 particles into device arrays; returns the m+ count."""
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        n_plus = C.c_int64(0)
+        self._check(self.lib.jtp_generate_ics(self.h, C.byref(params), ptr(d_pos_fixed), ptr(d_vel), int(vel_f64),
+                                              ptr(d_species), C.byref(n_plus), stream))
+        return n_plus.value
 
     def compute_forces(self, pos_fixed: np.ndarray, species: np.ndarray) -> np.ndarray:
         """Host convenience: upload, compute, download (float32 or float64 [n,3])."""

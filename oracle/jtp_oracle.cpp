@@ -18,6 +18,10 @@
 //   orc_ewald              FP64 Ewald direct summation with Plummer correction of the nearest image
 //   orc_refcpu_*           TreePMForce (treepm_force.rs:35-131) + TreePMTree (tree_short.rs:49-247) +
 //                          PmGrid two-grid solve, restated for the CPU baseline and compat parity
+//   orc_splitmix64         SplitMix64 (Steele, Lea, Flood 2014; Vigna's reference C), the IC random stream
+//   orc_ic                 generate_zeldovich_ics / generate_displacement_field (src/ic_gen.rs:51-169,
+//                          179-309) step by step with the SplitMix64 streams in place of StdRng: full
+//                          complex spectrum, complex inverse FFT per component, real part / n^3
 //   orc_segregation*       compute_segregation_metrics (src/bin/janus_mu19_production.rs:219-273): the
 //                          _f32_ref form bins f32 positions exactly as the reference does, the fixed form
 //                          bins 32-bit fixed-point coordinates by floor(u n / 2^32)
@@ -799,6 +803,96 @@ void orc_refcpu_forces(void* h, int64_t nt, const int64_t* tidx, int pm, int tre
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+// ------------------------------------------------------------------ initial conditions
+static uint64_t sm_next(uint64_t& state) {
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// output j (0-based) of the generator seeded with s: j + 1 advances of the state
+static uint64_t sm_at(uint64_t s, uint64_t j) {
+    uint64_t st = s + j * 0x9E3779B97F4A7C15ull;
+    return sm_next(st);
+}
+
+void orc_splitmix64(uint64_t seed, int64_t count, uint64_t* out) {
+    uint64_t st = seed;
+    for (int64_t i = 0; i < count; ++i) out[i] = sm_next(st);
+}
+
+// pos [n^3][3] in [-L/2, L/2], vel [n^3][3], sign [n^3]; returns n+ (ic_gen.rs:51-169)
+int64_t orc_ic(int n, double L, double z_init, uint64_t seed, double mu, double n_s, double delta_rms, double* pos,
+               double* vel, int8_t* sign) {
+    const int64_t n3 = (int64_t)n * n * n;
+    const int64_t n_plus = (int64_t)std::llround((double)n3 / (1.0 + mu));
+    const double dk = 2.0 * M_PI / L, spacing = L / n, half = L / 2.0, d_growth = 1.0 / (1.0 + z_init);
+    const int half_n = n / 2;
+    auto kof = [&](int i) { return (i <= half_n ? (double)i : (double)(i - n)) * dk; };
+    auto u01 = [](uint64_t r) { return (double)(r >> 11) / 9007199254740992.0; };
+    // delta(k) over the full grid (ic_gen.rs:196-221)
+    std::vector<cd> dk_field(n3, cd(0, 0));
+    for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+            for (int ix = 0; ix < n; ++ix) {
+                const double kx = kof(ix), ky = kof(iy), kz = kof(iz);
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (!(k2 > 0)) continue;
+                const int64_t idx = ((int64_t)iz * n + iy) * n + ix;
+                const double pk = std::pow(std::sqrt(k2), n_s - 4.0) * delta_rms * delta_rms;
+                const double amp = std::sqrt(pk / 2.0);
+                const double phase = u01(sm_at(seed, 3 * idx)) * 2.0 * M_PI;
+                const double u1 = u01(sm_at(seed, 3 * idx + 1)), u2 = u01(sm_at(seed, 3 * idx + 2));
+                const double r = std::sqrt(-2.0 * std::log(1.0 - u1));
+                const double re = r * std::cos(2.0 * M_PI * u2) * amp * d_growth;
+                const double im = r * std::sin(2.0 * M_PI * u2) * amp * d_growth;
+                dk_field[idx] = cd(re * std::cos(phase) - im * std::sin(phase), re * std::sin(phase) + im * std::cos(phase));
+            }
+    // psi_i = Re IFFT[-i k_i delta / k^2] / n^3 (ic_gen.rs:224-296)
+    std::vector<double> psi[3];
+    for (int dim = 0; dim < 3; ++dim) {
+        std::vector<cd> f(n3);
+        for (int iz = 0; iz < n; ++iz)
+            for (int iy = 0; iy < n; ++iy)
+                for (int ix = 0; ix < n; ++ix) {
+                    const int64_t idx = ((int64_t)iz * n + iy) * n + ix;
+                    const double kx = kof(ix), ky = kof(iy), kz = kof(iz);
+                    const double k2 = kx * kx + ky * ky + kz * kz;
+                    const double ki = dim == 0 ? kx : (dim == 1 ? ky : kz);
+                    f[idx] = k2 > 0 ? dk_field[idx] * cd(0.0, -ki / k2) : cd(0, 0);
+                }
+        fft3d(f, n, true);
+        psi[dim].resize(n3);
+        for (int64_t i = 0; i < n3; ++i) psi[dim][i] = f[i].real() / (double)n3;
+    }
+    // scale to max |psi| = 0.3 spacing; velocity scale H0 sqrt(1 + z) (ic_gen.rs:82-95)
+    double maxd = 0;
+    for (int64_t i = 0; i < n3; ++i)
+        maxd = std::max(maxd, std::sqrt(psi[0][i] * psi[0][i] + psi[1][i] * psi[1][i] + psi[2][i] * psi[2][i]));
+    const double scale = maxd > 1e-10 ? spacing * 0.3 / maxd : 1.0;
+    const double vel_scale = 69.9 / 977.8 * std::sqrt(1.0 + z_init);
+    // signs: the n+ cells of smallest (key, index) (ic_gen.rs:99-110 draws a random subset by shuffling)
+    std::vector<std::pair<uint64_t, int64_t>> keys(n3);
+    for (int64_t i = 0; i < n3; ++i) keys[i] = {sm_at(seed + 12345, (uint64_t)i), i};
+    std::sort(keys.begin(), keys.end());
+    for (int64_t r = 0; r < n3; ++r) sign[keys[r].second] = r < n_plus ? 1 : -1;
+    // lattice + displacement, periodic wrap (ic_gen.rs:117-149)
+    for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+            for (int ix = 0; ix < n; ++ix) {
+                const int64_t g = ((int64_t)iz * n + iy) * n + ix;
+                const int ii[3] = {ix, iy, iz};
+                for (int a = 0; a < 3; ++a) {
+                    double x = (ii[a] + 0.5) * spacing - half + psi[a][g] * scale;
+                    while (x > half) x -= L;
+                    while (x < -half) x += L;
+                    pos[3 * g + a] = x;
+                    vel[3 * g + a] = psi[a][g] * scale * vel_scale;
+                }
+            }
+    return n_plus;
+}
 
 // ------------------------------------------------------------------ segregation metrics
 // janus_mu19_production.rs:238-272, from per-cell counts

@@ -41,6 +41,8 @@ class Oracle:
             "orc_refcpu_forces": (None, [vp, i64, vp, i32, i32, vp]),
             "orc_num_threads": (C.c_int, []),
             "orc_segregation": (None, [i64, vp, vp, i32, vp]),
+            "orc_splitmix64": (None, [C.c_uint64, i64, vp]),
+            "orc_ic": (i64, [i32, dbl, dbl, C.c_uint64, dbl, dbl, dbl, vp, vp, vp]),
             "orc_segregation_f32_ref": (None, [i64, vp, vp, dbl, i32, vp]),
         }
         for k, (r, a) in sig.items():
@@ -95,6 +97,20 @@ class Oracle:
         self.lib.orc_node_sums(n, spos.ctypes.data, ssp.ctypes.data, nodes.ctypes.data, sums.ctypes.data,
                                boxes.ctypes.data)
         return sums, boxes
+
+    def splitmix64(self, seed, count):
+        out = np.zeros(count, np.uint64)
+        self.lib.orc_splitmix64(seed, count, out.ctypes.data)
+        return out
+
+    def ic(self, n, L, z_init=10.0, seed=42, mu=19.0, n_s=0.965, delta_rms=0.10):
+        """(pos [n^3, 3] in [-L/2, L/2], vel, sign, n_plus) of the Zel'dovich lattice ICs."""
+        pos = np.zeros((n ** 3, 3))
+        vel = np.zeros((n ** 3, 3))
+        sign = np.zeros(n ** 3, np.int8)
+        n_plus = self.lib.orc_ic(n, L, z_init, seed, mu, n_s, delta_rms, pos.ctypes.data, vel.ctypes.data,
+                                 sign.ctypes.data)
+        return pos, vel, sign, int(n_plus)
 
     def segregation(self, pos_fixed, sp, n_cells):
         pos = np.ascontiguousarray(pos_fixed, np.uint32)
