@@ -109,8 +109,8 @@ typedef struct jtp_stats {
  *      TreePMForce::new, treepm_force.rs:35-51) */
 typedef struct jtp_ctx jtp_ctx;
 /* Defaults: r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, softening = L/G/30,
- * G = 1, leaf 4, group 64, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on,
- * interlacing on, theta_cut = 0.7 theta. */
+ * G = 1, leaf 4, group 32, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on,
+ * interlacing on, theta_cut = 0.66 theta. */
 void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells);
 int  jtp_create(const jtp_params* p, jtp_ctx** out);
 int  jtp_destroy(jtp_ctx* ctx);

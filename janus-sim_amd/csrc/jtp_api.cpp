@@ -240,7 +240,7 @@ void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
     p->softening = h / 30.0;
     p->g_constant = 1.0;
     p->leaf_size = 4;
-    p->group_size = 64;
+    p->group_size = 32;
     p->precision = JTP_FP32;
     p->split_mode = JTP_SPLIT_ERFC_GAUSS;
     p->gradient = JTP_GRAD_4PT;
@@ -248,7 +248,7 @@ void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
     p->pm_enabled = 1;
     p->tree_enabled = 1;
     p->interlace = 1;
-    p->theta_cut = 0.7 * p->theta;
+    p->theta_cut = 0.66 * p->theta;
 }
 
 int jtp_create(const jtp_params* p, jtp_ctx** out) {

@@ -600,6 +600,16 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ i
     }
 }
 
+// Block -> (plane, kx block) for the y and z passes.  A workgroup reads KB consecutive kx of every
+// row, i.e. KB * sizeof(complex) bytes per row (64 B at G = 512 in FP32: half a 128-byte line),
+// and the other half of each line belongs to the next kx block.  Blocks are dispatched round-robin
+// over the 8 XCDs, so consecutive logical blocks are dealt to one XCD in sequence: the kx block
+// that reads the other half of the lines runs next on the same XCD and finds them in its L2.
+__device__ inline int fft_logical_block(int nblocks) {
+    const int b = (int)blockIdx.x;
+    return (nblocks & 7) ? b : (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+
 // Y pass (complex, forward or inverse): block = (z, block of KB consecutive kx).
 template <typename R>
 __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, int logG, const Cplx<R>* __restrict__ tw,
@@ -608,7 +618,8 @@ __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, int l
     const int G = 1 << logG, H = G / 2 + 1;
     const int KB = kFftPoints / G;
     const int nkb = (H + KB - 1) / KB;
-    const int z = blockIdx.x / nkb, kx0 = (blockIdx.x % nkb) * KB;
+    const int lb = fft_logical_block(G * nkb);
+    const int z = lb / nkb, kx0 = (lb % nkb) * KB;
     Cplx<R>* base = spec + (int64_t)z * G * H;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
         const int y = e / KB, jb = e % KB;
@@ -635,7 +646,8 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
     const int G = 1 << logG, H = G / 2 + 1;
     const int KB = kFftPoints / G;
     const int nkb = (H + KB - 1) / KB;
-    const int y = blockIdx.x / nkb, kx0 = (blockIdx.x % nkb) * KB;
+    const int lb = fft_logical_block(G * nkb);
+    const int y = lb / nkb, kx0 = (lb % nkb) * KB;
     const int64_t zs = (int64_t)G * H;
     Cplx<R>* base = spec + (int64_t)y * H;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
@@ -687,7 +699,8 @@ __global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, i
     const int G = 1 << logG, H = G / 2 + 1;
     const int KB = kFftPoints / G;
     const int nkb = (H + KB - 1) / KB;
-    const int y = blockIdx.x / nkb, kx0 = (blockIdx.x % nkb) * KB;
+    const int lb = fft_logical_block(G * nkb);
+    const int y = lb / nkb, kx0 = (lb % nkb) * KB;
     const int64_t zs = (int64_t)G * H;
     Cplx<R>* base = spec + (int64_t)y * H;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {

@@ -32,8 +32,8 @@ def test_default_params_follow_reference_derivation():
     p = j.default_params(250.0, 64)
     cell = 250.0 / 64
     assert p.r_s == pytest.approx(1.25 * cell) and p.r_cut == pytest.approx(4.5 * 1.25 * cell)
-    assert p.theta == 0.5 and p.leaf_size == 4 and p.group_size == 64 and p.precision == j.FP32
-    assert p.interlace == 1 and p.theta_cut == pytest.approx(0.35) and p.deconvolve == 1
+    assert p.theta == 0.5 and p.leaf_size == 4 and p.group_size == 32 and p.precision == j.FP32
+    assert p.interlace == 1 and p.theta_cut == pytest.approx(0.33) and p.deconvolve == 1
 
 
 @pytest.mark.parametrize("field,value", [
