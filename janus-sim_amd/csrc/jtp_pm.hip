@@ -478,76 +478,118 @@ __global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __rest
 }
 
 // ------------------------------------------------------------------ FFT core
-// In-place radix-2 Stockham over `nlines` lines of length G held in LDS with
-// element (line l, point p) at buf[l * SL + p * SP].  Each stage reads its
-// butterflies into registers, synchronises, and writes them back in autosorted
-// order (Stockham): out[2j - k] = a + w b, out[2j - k + Ns] = a - w b with
-// k = j mod Ns and w = exp(-+2 pi i k / (2 Ns)) from the table tw[m] = e^{-2 pi i m / G}.
-template <typename R, int PER>
-__device__ inline void lds_fft(Cplx<R>* buf, int G, int logG, int nlines, int SL, int SP, bool line_fast,
-                               const Cplx<R>* __restrict__ tw, bool inv) {
-    const int halfG = G >> 1;
+// In-place mixed-radix Stockham over NL = kFftPoints / G lines of length G = 2^LG held in LDS
+// (element (line l, point p) at l * G + p, or at l + p * NL when LINE_FAST).  Every size is a
+// compile-time constant (the kernels are instantiated per mesh size), so the index arithmetic is
+// shifts and masks and the stage loop unrolls.  Stages: radix 4 while 4 Ns <= G, then one radix-2
+// stage if LG is odd.  A stage of radix R with Ns = product of the earlier radices takes
+// v_r = x[j + r G/R], multiplies by w^(r k) (k = j mod Ns, w = exp(-+2 pi i / (R Ns)), from the
+// table tw[m] = e^{-2 pi i m / G}), does the R-point DFT and writes y_r to
+// (j / Ns) R Ns + k + r Ns (autosorted order); reads go to registers, a barrier, then the writes.
+template <int LG, bool LINE_FAST>
+__device__ inline int fft_at(int l, int p) {
+    constexpr int LNL = 12 - LG;  // log2 of the line count (kFftPoints = 2^12)
+    return LINE_FAST ? l + (p << LNL) : (l << LG) + p;
+}
+
+template <typename R>
+__device__ inline Cplx<R> cmul(Cplx<R> a, Cplx<R> w) {
+    return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
+}
+
+template <typename R, int LG, bool LINE_FAST>
+__device__ inline void lds_fft(Cplx<R>* buf, const Cplx<R>* __restrict__ tw, bool inv) {
+    static_assert(kFftPoints == 4096, "index shifts assume 2^12 points per workgroup");
+    constexpr int G = 1 << LG, LNL = 12 - LG;
     const int tid = threadIdx.x;
-    for (int ls = 0; ls < logG; ++ls) {
-        const int Ns = 1 << ls;
-        Cplx<R> va[PER], vb[PER];
-        int ln[PER], jj[PER];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
+    for (int ls = 0; ls + 2 <= LG; ls += 2) {
+        const int Ns = 1 << ls;
+        constexpr int Q = G / 4, LQ = LG - 2;
+        Cplx<R> y[4][4];
+        int ll[4], oo[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {  // 1024 radix-4 butterflies per stage, 4 per thread
             const int q = tid + u * 256;
-            int l, j;
-            if (line_fast) {
-                l = q % nlines;
-                j = q / nlines;
-            } else {
-                l = q / halfG;
-                j = q % halfG;
-            }
-            ln[u] = l;
-            jj[u] = j;
-            Cplx<R> a = buf[l * SL + j * SP];
-            Cplx<R> b = buf[l * SL + (j + halfG) * SP];
+            const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LQ);
+            const int j = LINE_FAST ? (q >> LNL) : (q & (Q - 1));
             const int k = j & (Ns - 1);
-            Cplx<R> w = tw[(k << (logG - 1 - ls))];
-            if (inv) w.y = -w.y;
-            const Cplx<R> wb = {w.x * b.x - w.y * b.y, w.x * b.y + w.y * b.x};
-            va[u] = {a.x + wb.x, a.y + wb.y};
-            vb[u] = {a.x - wb.x, a.y - wb.y};
+            Cplx<R> v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = buf[fft_at<LG, LINE_FAST>(l, j + r * Q)];
+#pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                Cplx<R> w = tw[(r * k) << (LG - 2 - ls)];
+                if (inv) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+            const Cplx<R> a0 = {v[0].x + v[2].x, v[0].y + v[2].y}, a1 = {v[0].x - v[2].x, v[0].y - v[2].y};
+            const Cplx<R> a2 = {v[1].x + v[3].x, v[1].y + v[3].y}, a3 = {v[1].x - v[3].x, v[1].y - v[3].y};
+            // forward: y1 = a1 - i a3, y3 = a1 + i a3 (inverse: the opposite signs)
+            const Cplx<R> ia3 = inv ? Cplx<R>{-a3.y, a3.x} : Cplx<R>{a3.y, -a3.x};  // (-+i) a3
+            y[u][0] = {a0.x + a2.x, a0.y + a2.y};
+            y[u][2] = {a0.x - a2.x, a0.y - a2.y};
+            y[u][1] = {a1.x + ia3.x, a1.y + ia3.y};
+            y[u][3] = {a1.x - ia3.x, a1.y - ia3.y};
+            ll[u] = l;
+            oo[u] = ((j >> ls) << (ls + 2)) + k;
         }
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int j = jj[u];
-            const int k = j & (Ns - 1);
-            const int o = 2 * j - k;
-            buf[ln[u] * SL + o * SP] = va[u];
-            buf[ln[u] * SL + (o + Ns) * SP] = vb[u];
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) buf[fft_at<LG, LINE_FAST>(ll[u], oo[u] + r * Ns)] = y[u][r];
+        __syncthreads();
+    }
+    if (LG & 1) {  // last stage radix 2, Ns = G / 2: outputs j and j + G/2, twiddle tw[j]
+        constexpr int H2 = G / 2, LH = LG - 1;
+        Cplx<R> va[8], vb[8];
+        int ll[8], jj[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = tid + u * 256;
+            const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LH);
+            const int j = LINE_FAST ? (q >> LNL) : (q & (H2 - 1));
+            const Cplx<R> a = buf[fft_at<LG, LINE_FAST>(l, j)];
+            Cplx<R> w = tw[j];
+            if (inv) w.y = -w.y;
+            const Cplx<R> wb = cmul(buf[fft_at<LG, LINE_FAST>(l, j + H2)], w);
+            va[u] = {a.x + wb.x, a.y + wb.y};
+            vb[u] = {a.x - wb.x, a.y - wb.y};
+            ll[u] = l;
+            jj[u] = j;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            buf[fft_at<LG, LINE_FAST>(ll[u], jj[u])] = va[u];
+            buf[fft_at<LG, LINE_FAST>(ll[u], jj[u] + H2)] = vb[u];
         }
         __syncthreads();
     }
 }
 
-constexpr int kFftPer = kFftPoints / 2 / 256;  // butterflies per thread per stage
-
 // X forward (R2C): pairs of real lines (y even, y odd) packed as one complex line.
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cplx<R>* __restrict__ out, int logG,
+template <typename R, int LG>
+__global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cplx<R>* __restrict__ out,
                                                    const Cplx<R>* __restrict__ tw) {
     __shared__ Cplx<R> buf[kFftPoints];
-    const int G = 1 << logG, H = G / 2 + 1;
-    const int npair = kFftPoints / G;              // complex lines per block
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int npair = kFftPoints / G;                   // complex lines per block
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;  // first real line
     const R* src = in + line0 * G;
+    // G = 16: the grid's G^2 = 256 real rows fill half of one block; the rest is zero and not stored
+    constexpr int rows = 2 * npair < G * G ? 2 * npair : G * G;
     for (int e = threadIdx.x; e < 2 * kFftPoints; e += 256) {
-        const int line = e / G, x = e % G;
-        const R v = src[e];
+        const int line = e >> LG, x = e & (G - 1);
+        const R v = line < rows ? src[e] : (R)0;
         if (line & 1) buf[(line >> 1) * G + x].y = v; else buf[(line >> 1) * G + x].x = v;
     }
     __syncthreads();
-    lds_fft<R, kFftPer>(buf, G, logG, npair, G, 1, false, tw, false);
+    lds_fft<R, LG, false>(buf, tw, false);
     Cplx<R>* dst = out + line0 * H;
-    for (int e = threadIdx.x; e < 2 * npair * H; e += 256) {
-        const int row = e / H, k = e % H;
+    for (int e = threadIdx.x; e < rows * H; e += 256) {
+        const int row = e / H, k = e - row * H;
         const int p = row >> 1;
         const Cplx<R> z = buf[p * G + k];
         const Cplx<R> zc = buf[p * G + ((G - k) & (G - 1))];
@@ -561,21 +603,24 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cpl
     }
 }
 
-// X inverse (C2R): two half-spectrum rows A, B -> Z = A_full + i B_full -> inverse FFT -> (a, b).
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ in, R* __restrict__ out, int logG,
+// X inverse (C2R): two half-spectrum rows A, B -> Z = A + i B, inverse FFT, re / im = the two real rows.
+template <typename R, int LG>
+__global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ in, R* __restrict__ out,
                                                    const Cplx<R>* __restrict__ tw) {
     __shared__ Cplx<R> buf[kFftPoints];
-    const int G = 1 << logG, H = G / 2 + 1;
-    const int npair = kFftPoints / G;
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int npair = kFftPoints / G;
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;
     const Cplx<R>* src = in + line0 * H;
+    constexpr int rows = 2 * npair < G * G ? 2 * npair : G * G;  // G = 16: half a block
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int p = e / G, k = e % G;
+        const int p = e >> LG, k = e & (G - 1);
         const Cplx<R>* ra = src + (int64_t)(2 * p) * H;
         const Cplx<R>* rb = ra + H;
         Cplx<R> a, b;
-        if (k <= G / 2) {
+        if (2 * p >= rows) {
+            a = b = Cplx<R>{0, 0};
+        } else if (k <= G / 2) {
             a = ra[k];
             b = rb[k];
             if (k == 0 || k == G / 2) {
@@ -591,10 +636,10 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ i
         buf[p * G + k] = {a.x - b.y, a.y + b.x};
     }
     __syncthreads();
-    lds_fft<R, kFftPer>(buf, G, logG, npair, G, 1, false, tw, true);
+    lds_fft<R, LG, false>(buf, tw, true);
     R* dst = out + line0 * G;
-    for (int e = threadIdx.x; e < 2 * kFftPoints; e += 256) {
-        const int line = e / G, x = e % G;
+    for (int e = threadIdx.x; e < rows * G; e += 256) {
+        const int line = e >> LG, x = e & (G - 1);
         const Cplx<R> z = buf[(line >> 1) * G + x];
         dst[e] = (line & 1) ? z.y : z.x;
     }
@@ -611,52 +656,50 @@ __device__ inline int fft_logical_block(int nblocks) {
 }
 
 // Y pass (complex, forward or inverse): block = (z, block of KB consecutive kx).
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, int logG, const Cplx<R>* __restrict__ tw,
-                                               int inv) {
+template <typename R, int LG>
+__global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw, int inv) {
     __shared__ Cplx<R> buf[kFftPoints];
-    const int G = 1 << logG, H = G / 2 + 1;
-    const int KB = kFftPoints / G;
-    const int nkb = (H + KB - 1) / KB;
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int KB = kFftPoints / G, LKB = 12 - LG;
+    constexpr int nkb = (H + KB - 1) / KB;
     const int lb = fft_logical_block(G * nkb);
     const int z = lb / nkb, kx0 = (lb % nkb) * KB;
     Cplx<R>* base = spec + (int64_t)z * G * H;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int y = e / KB, jb = e % KB;
+        const int y = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
-        buf[y * KB + jb] = (kx < H) ? base[(int64_t)y * H + kx] : Cplx<R>{0, 0};
+        buf[e] = (kx < H) ? base[(int64_t)y * H + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, inv != 0);
+    lds_fft<R, LG, true>(buf, tw, inv != 0);
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int y = e / KB, jb = e % KB;
+        const int y = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
-        if (kx < H) base[(int64_t)y * H + kx] = buf[y * KB + jb];
+        if (kx < H) base[(int64_t)y * H + kx] = buf[e];
     }
 }
 
 // Z pass with the Green function: forward z FFT, multiply, inverse z FFT.
 // green(kx, ky, kz) = norm * T[kx] T[ky] T[kz] / (K2[kx] + K2[ky] + K2[kz]); zero for DC and
 // for max(|kx|,|ky|,|kz|) < k_min (apply_green_kernel, cufft_wrapper.cu:160-207).
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec, int logG,
-                                                     const Cplx<R>* __restrict__ tw, const R* __restrict__ gtab,
-                                                     R norm, int k_min, int matched) {
+template <typename R, int LG>
+__global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw,
+                                                     const R* __restrict__ gtab, R norm, int k_min, int matched) {
     __shared__ Cplx<R> buf[kFftPoints];
-    const int G = 1 << logG, H = G / 2 + 1;
-    const int KB = kFftPoints / G;
-    const int nkb = (H + KB - 1) / KB;
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int KB = kFftPoints / G, LKB = 12 - LG;
+    constexpr int nkb = (H + KB - 1) / KB;
     const int lb = fft_logical_block(G * nkb);
     const int y = lb / nkb, kx0 = (lb % nkb) * KB;
-    const int64_t zs = (int64_t)G * H;
+    constexpr int64_t zs = (int64_t)G * H;
     Cplx<R>* base = spec + (int64_t)y * H;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int z = e / KB, jb = e % KB;
+        const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
-        buf[z * KB + jb] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
+        buf[e] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, false);
+    lds_fft<R, LG, true>(buf, tw, false);
     const R* T = gtab;
     const R* K2 = gtab + G;
     const R* Ks = gtab + 2 * G;  // signed k_i
@@ -664,7 +707,7 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
     const int ky = y;
     const int aky = ky <= G / 2 ? ky : G - ky;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int kz = e / KB, jb = e % KB;
+        const int kz = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         const int akz = kz <= G / 2 ? kz : G - kz;
         const int akx = kx;
@@ -680,56 +723,89 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
                 f *= (Ks[kx] * D[kx] + Ks[ky] * D[ky] + Ks[kz] * D[kz]) / (dd > fl ? dd : fl);
             }
         }
-        Cplx<R> v = buf[kz * KB + jb];
-        buf[kz * KB + jb] = {v.x * f, v.y * f};
+        const Cplx<R> v = buf[e];
+        buf[e] = {v.x * f, v.y * f};
     }
     __syncthreads();
-    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, true);
+    lds_fft<R, LG, true>(buf, tw, true);
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int z = e / KB, jb = e % KB;
+        const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
-        if (kx < H) base[z * zs + kx] = buf[z * KB + jb];
+        if (kx < H) base[z * zs + kx] = buf[e];
     }
 }
 
 // Inverse z pass alone (the z pass of the solver without the forward transform and Green multiply).
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, int logG, const Cplx<R>* __restrict__ tw) {
+template <typename R, int LG>
+__global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw) {
     __shared__ Cplx<R> buf[kFftPoints];
-    const int G = 1 << logG, H = G / 2 + 1;
-    const int KB = kFftPoints / G;
-    const int nkb = (H + KB - 1) / KB;
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int KB = kFftPoints / G, LKB = 12 - LG;
+    constexpr int nkb = (H + KB - 1) / KB;
     const int lb = fft_logical_block(G * nkb);
     const int y = lb / nkb, kx0 = (lb % nkb) * KB;
-    const int64_t zs = (int64_t)G * H;
+    constexpr int64_t zs = (int64_t)G * H;
     Cplx<R>* base = spec + (int64_t)y * H;
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int z = e / KB, jb = e % KB;
+        const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
-        buf[z * KB + jb] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
+        buf[e] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, kFftPer>(buf, G, logG, KB, 1, KB, true, tw, true);
+    lds_fft<R, LG, true>(buf, tw, true);
     for (int e = threadIdx.x; e < kFftPoints; e += 256) {
-        const int z = e / KB, jb = e % KB;
+        const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
-        if (kx < H) base[z * zs + kx] = buf[z * KB + jb];
+        if (kx < H) base[z * zs + kx] = buf[e];
     }
 }
 
-template <typename R>
-static hipError_t inverse_fft_t(Ctx& c, R* out, hipStream_t s) {
-    const int G = (int)c.G(), H = G / 2 + 1, logG = c.logG;
-    const int npair = kFftPoints / G;
-    const int KB = kFftPoints / G;
-    const int nkb = (H + KB - 1) / KB;
-    const unsigned nx = (unsigned)((int64_t)G * G / (2 * npair));
+template <typename R, int LG>
+static hipError_t inverse_fft_lg(Ctx& c, R* out, hipStream_t s) {
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int KB = kFftPoints / G, nkb = (H + KB - 1) / KB;
+    constexpr unsigned nx = (unsigned)(G * G > 2 * (kFftPoints / G) ? (int64_t)G * G / (2 * (kFftPoints / G)) : 1);
     const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
     Cplx<R>* spec = (Cplx<R>*)c.spec;
-    k_fft_z_inv<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw);
-    k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 1);
-    k_fft_x_c2r<R><<<dim3(nx), dim3(256), 0, s>>>(spec, out, logG, tw);
+    k_fft_z_inv<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw);
+    k_fft_y<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
+    k_fft_x_c2r<R, LG><<<dim3(nx), dim3(256), 0, s>>>(spec, out, tw);
     return hipGetLastError();
+}
+
+template <typename R, int LG>
+static hipError_t fft_solve_lg(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
+    constexpr int G = 1 << LG, H = G / 2 + 1;
+    constexpr int KB = kFftPoints / G, nkb = (H + KB - 1) / KB;
+    constexpr unsigned nx = (unsigned)(G * G > 2 * (kFftPoints / G) ? (int64_t)G * G / (2 * (kFftPoints / G)) : 1);
+    const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
+    Cplx<R>* spec = (Cplx<R>*)c.spec;
+    k_fft_x_r2c<R, LG><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, tw);
+    k_fft_y<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 0);
+    k_fft_z_green<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, (const R*)c.gtab,
+                                                                         (R)c.green_norm, c.p.k_min, matched);
+    k_fft_y<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
+    k_fft_x_c2r<R, LG><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, tw);
+    return hipGetLastError();
+}
+
+// the mesh sizes the context accepts: G = 16 .. 512
+#define JTP_FFT_DISPATCH(CALL)              \
+    switch (c.logG) {                       \
+        case 4: return CALL(4);             \
+        case 5: return CALL(5);             \
+        case 6: return CALL(6);             \
+        case 7: return CALL(7);             \
+        case 8: return CALL(8);             \
+        case 9: return CALL(9);             \
+        default: return hipErrorInvalidValue; \
+    }
+
+template <typename R>
+static hipError_t inverse_fft_t(Ctx& c, R* out, hipStream_t s) {
+#define JTP_INV(LG) inverse_fft_lg<R, LG>(c, out, s)
+    JTP_FFT_DISPATCH(JTP_INV)
+#undef JTP_INV
 }
 
 hipError_t pm_inverse_fft(Ctx& c, void* out, hipStream_t s) {
@@ -739,20 +815,9 @@ hipError_t pm_inverse_fft(Ctx& c, void* out, hipStream_t s) {
 
 template <typename R>
 static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
-    const int G = (int)c.G(), H = G / 2 + 1, logG = c.logG;
-    const int npair = kFftPoints / G;
-    const int KB = kFftPoints / G;
-    const int nkb = (H + KB - 1) / KB;
-    const unsigned nx = (unsigned)((int64_t)G * G / (2 * npair));
-    const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
-    Cplx<R>* spec = (Cplx<R>*)c.spec;
-    k_fft_x_r2c<R><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, logG, tw);
-    k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 0);
-    k_fft_z_green<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, (const R*)c.gtab,
-                                                                     (R)c.green_norm, c.p.k_min, matched);
-    k_fft_y<R><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, logG, tw, 1);
-    k_fft_x_c2r<R><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, logG, tw);
-    return hipGetLastError();
+#define JTP_SOLVE(LG) fft_solve_lg<R, LG>(c, rho, phi, s, matched)
+    JTP_FFT_DISPATCH(JTP_SOLVE)
+#undef JTP_SOLVE
 }
 
 hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched) {

@@ -87,11 +87,12 @@ def test_deposit_deterministic(jtp):
     assert np.array_equal(a1, a2)
 
 
-@pytest.mark.parametrize("precision", [1, 0])
-def test_pm_solve_numpy(jtp, precision):
-    """jtp_pm_solve against numpy's FFT (an independent library) with the closed-form Green function."""
-    print(f"case: pm_solve vs numpy precision={precision}", flush=True)
-    G, L, rs = 32, 100.0, 3.0
+@pytest.mark.parametrize("precision,G", [(1, 32), (0, 32), (1, 16), (0, 64), (1, 128), (0, 256)])
+def test_pm_solve_numpy(jtp, precision, G):
+    """jtp_pm_solve against numpy's FFT (an independent library) with the closed-form Green function,
+    at mesh sizes with an even and an odd number of radix-2 factors."""
+    print(f"case: pm_solve vs numpy precision={precision} G={G}", flush=True)
+    L, rs = 100.0, 96.0 / G  # ~1 cell
     rng = np.random.default_rng(1)
     rho = rng.standard_normal((G, G, G))
     ps = jtp.PoissonSolver(G, L, g_constant=1.0, r_s=rs, precision=precision, deconvolve=1)
