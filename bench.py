@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.join(ROOT, "janus-sim_amd"))
 sys.path.insert(0, ROOT)
 
 import janus_treepm as j  # noqa: E402
-from janus_treepm import synth  # noqa: E402
+from janus_treepm import autotune, synth  # noqa: E402
 
 METRIC = "TreePM particle-force-updates/s at N=2.2e7 on 1 GPU, FP32 vs FP64 at matched error"
 PEAK_FP32_TFLOPS = 157.3
@@ -226,12 +226,23 @@ def run_multistep(args):
     pos0 = j.to_fixed(w.pos, w.box_size)
     steps = args.multisteps
 
+    tuned = {}
+
     def params(prec):
         p = j.default_params(w.box_size, w.n_cells)
         p.r_s, p.r_cut, p.theta, p.softening, p.leaf_size = w.r_s, w.r_cut, w.theta, w.softening, w.leaf_size
+        for k, v in tuned.items():
+            setattr(p, k, v)
         p.precision = prec
         return p
 
+    if args.autotune:
+        # leaf and group size from the library's tuner, timed on the initial particles themselves
+        t0 = time.perf_counter()
+        res = autotune.autotune(w.pos, w.species, params(j.FP32), w.m_plus, w.m_minus_abs, k=1)
+        tuned.update(leaf_size=res["leaf_size"], group_size=res["group_size"])
+        log(f"autotune: leaf {res['leaf_size']}, group {res['group_size']} in {time.perf_counter() - t0:.1f} s "
+            f"(sample {res['sample']}; leaf, group, ms, tree error: {res['table']})")
     energy_ctx = j.ForceContext(params(j.FP64), w.m_plus, w.m_minus_abs)
 
     energy_ms = []
@@ -245,7 +256,9 @@ def run_multistep(args):
         return wpot + d["kinetic"].sum(), mom, wpot
 
     out = {"metric": "TreePM KDK steps at N=4.08e7, PM 512^3, 1 GPU", "n_particles": n, "steps": steps,
-           "pm_grid": 512, "box": 300.0, "n_halos": 8000, "forms": {}}
+           "pm_grid": 512, "box": 300.0, "n_halos": 8000, "forms": {},
+           "leaf_size": params(j.FP32).leaf_size, "group_size": params(j.FP32).group_size,
+           "autotuned": bool(args.autotune)}
     dt = None
     for prec, label in ((j.FP32, "fp32"), (j.FP64, "fp64")):
         rt = np.float64 if prec == j.FP64 else np.float32
@@ -302,6 +315,8 @@ def main():
     ap.add_argument("--workload", default="clustered_22M",
                     choices=sorted(WORKLOADS) + ["ensemble_c3", "multistep_41M"])
     ap.add_argument("--multisteps", type=int, default=100, help="KDK steps of the multistep_41M workload")
+    ap.add_argument("--autotune", action="store_true",
+                    help="multistep_41M: pick leaf and group size with janus_treepm.autotune first")
     ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's streaming copy kernel on MIB MiB, 5 times (HBM counter calibration)")
