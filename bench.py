@@ -240,9 +240,10 @@ def run_multistep(args):
         # leaf and group size from the library's tuner, timed on the initial particles themselves
         t0 = time.perf_counter()
         res = autotune.autotune(w.pos, w.species, params(j.FP32), w.m_plus, w.m_minus_abs, k=1)
-        tuned.update(leaf_size=res["leaf_size"], group_size=res["group_size"])
-        log(f"autotune: leaf {res['leaf_size']}, group {res['group_size']} in {time.perf_counter() - t0:.1f} s "
-            f"(sample {res['sample']}; leaf, group, ms, tree error: {res['table']})")
+        tuned.update(leaf_size=res["leaf_size"], group_size=res["group_size"], fft_tile=res["fft_tile"])
+        log(f"autotune: leaf {res['leaf_size']}, group {res['group_size']}, fft tile {res['fft_tile']} in "
+            f"{time.perf_counter() - t0:.1f} s (sample {res['sample']}; leaf, group, ms, tree error: {res['table']}; "
+            f"fft tile, ms: {res['fft_table']})")
     energy_ctx = j.ForceContext(params(j.FP64), w.m_plus, w.m_minus_abs)
 
     energy_ms = []
@@ -258,6 +259,7 @@ def run_multistep(args):
     out = {"metric": "TreePM KDK steps at N=4.08e7, PM 512^3, 1 GPU", "n_particles": n, "steps": steps,
            "pm_grid": 512, "box": 300.0, "n_halos": 8000, "forms": {},
            "leaf_size": params(j.FP32).leaf_size, "group_size": params(j.FP32).group_size,
+           "fft_tile": params(j.FP32).fft_tile,
            "autotuned": bool(args.autotune)}
     dt = None
     for prec, label in ((j.FP32, "fp32"), (j.FP64, "fp64")):

@@ -70,6 +70,10 @@ typedef struct jtp_params {
                             (cancels the leading aliasing error; one extra deposit/solve/interp) */
     double  theta_cut;   /* opening angle for nodes that straddle r_cut, >= 0 (0 = always open
                             them); the walk uses min(theta, theta_cut)                  */
+    int32_t fft_tile;    /* log2 of the complex points one FFT workgroup holds in LDS: 11 (16 KB
+                            FP32 / 32 KB FP64: 8 workgroups per CU), 12 (32 / 64 KB) or 13 (FP32
+                            only, 64 KB: 128-byte rows in the y / z passes at G = 512); 0 = 11.
+                            Every tile gives bit-identical results.                   */
 } jtp_params;
 /* Walk acceptance: a node's two monopoles are used when extent < theta * gap, gap being the
  * periodic group-box to node-box distance; if the node also reaches beyond r_cut from the

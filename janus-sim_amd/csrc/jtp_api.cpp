@@ -212,6 +212,10 @@ static int check_params(const jtp_params* p, std::string& msg) {
     if (p->k_min < 0) { msg = "k_min must be >= 0"; return JTP_ERR_ARG; }
     if (p->interlace != 0 && p->interlace != 1) { msg = "interlace must be 0 or 1"; return JTP_ERR_ARG; }
     if (!(p->theta_cut >= 0) || !std::isfinite(p->theta_cut)) { msg = "theta_cut must be >= 0"; return JTP_ERR_ARG; }
+    if (p->fft_tile != 0 && p->fft_tile != 11 && p->fft_tile != 12 && !(p->fft_tile == 13 && p->precision == JTP_FP32)) {
+        msg = "fft_tile must be 0, 11, 12 or (FP32) 13";
+        return JTP_ERR_ARG;
+    }
     return JTP_OK;
 }
 
@@ -249,6 +253,7 @@ void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
     p->tree_enabled = 1;
     p->interlace = 1;
     p->theta_cut = 0.66 * p->theta;
+    p->fft_tile = 0;
 }
 
 int jtp_create(const jtp_params* p, jtp_ctx** out) {

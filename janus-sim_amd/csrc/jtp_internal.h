@@ -19,7 +19,6 @@ constexpr int kSortIPT = 8;          // keys per thread per radix-sort tile
 constexpr int kSortTile = kBlock * kSortIPT;
 constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Morton cell (L/16)
 constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
-constexpr int kFftPoints = 4096;     // complex points held in LDS per FFT workgroup
 constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by wavefronts,
 constexpr int kDenseChunk = 1024;    // one per chunk of this many particles
 constexpr int kMaxRounds = 128;      // size of the bottom-up control block (frontier count ... root flag)

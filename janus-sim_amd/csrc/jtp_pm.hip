@@ -478,7 +478,7 @@ __global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __rest
 }
 
 // ------------------------------------------------------------------ FFT core
-// In-place mixed-radix Stockham over NL = kFftPoints / G lines of length G = 2^LG held in LDS
+// In-place mixed-radix Stockham over NL = 2^LP / G lines of length G = 2^LG held in LDS
 // (element (line l, point p) at l * G + p, or at l + p * NL when LINE_FAST).  Every size is a
 // compile-time constant (the kernels are instantiated per mesh size), so the index arithmetic is
 // shifts and masks and the stage loop unrolls.  Stages: radix 4 while 4 Ns <= G, then one radix-2
@@ -486,37 +486,41 @@ __global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __rest
 // v_r = x[j + r G/R], multiplies by w^(r k) (k = j mod Ns, w = exp(-+2 pi i / (R Ns)), from the
 // table tw[m] = e^{-2 pi i m / G}), does the R-point DFT and writes y_r to
 // (j / Ns) R Ns + k + r Ns (autosorted order); reads go to registers, a barrier, then the writes.
-template <int LG, bool LINE_FAST>
+template <int LG, int LP, bool LINE_FAST>
 __device__ inline int fft_at(int l, int p) {
-    constexpr int LNL = 12 - LG;  // log2 of the line count (kFftPoints = 2^12)
+    constexpr int LNL = LP - LG;  // log2 of the line count (2^LP points per workgroup)
     return LINE_FAST ? l + (p << LNL) : (l << LG) + p;
 }
 
 template <typename R>
 __device__ inline Cplx<R> cmul(Cplx<R> a, Cplx<R> w) {
+#pragma clang fp contract(off)
     return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
 }
 
-template <typename R, int LG, bool LINE_FAST>
+template <typename R, int LG, int LP, bool LINE_FAST>
 __device__ inline void lds_fft(Cplx<R>* buf, const Cplx<R>* __restrict__ tw, bool inv) {
-    static_assert(kFftPoints == 4096, "index shifts assume 2^12 points per workgroup");
-    constexpr int G = 1 << LG, LNL = 12 - LG;
+    // no FMA contraction: every tile size (and unroll shape) then rounds each butterfly identically
+#pragma clang fp contract(off)
+    static_assert(LP >= 10 && LP >= LG, "at least one radix-4 butterfly per thread and one line per block");
+    constexpr int G = 1 << LG, LNL = LP - LG;
+    constexpr int P4 = 1 << (LP - 10), P2 = 1 << (LP - 9);  // butterflies per thread per stage
     const int tid = threadIdx.x;
 #pragma unroll
     for (int ls = 0; ls + 2 <= LG; ls += 2) {
         const int Ns = 1 << ls;
         constexpr int Q = G / 4, LQ = LG - 2;
-        Cplx<R> y[4][4];
-        int ll[4], oo[4];
+        Cplx<R> y[P4][4];
+        int ll[P4], oo[P4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {  // 1024 radix-4 butterflies per stage, 4 per thread
+        for (int u = 0; u < P4; ++u) {  // 2^LP / 4 radix-4 butterflies per stage
             const int q = tid + u * 256;
             const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LQ);
             const int j = LINE_FAST ? (q >> LNL) : (q & (Q - 1));
             const int k = j & (Ns - 1);
             Cplx<R> v[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) v[r] = buf[fft_at<LG, LINE_FAST>(l, j + r * Q)];
+            for (int r = 0; r < 4; ++r) v[r] = buf[fft_at<LG, LP, LINE_FAST>(l, j + r * Q)];
 #pragma unroll
             for (int r = 1; r < 4; ++r) {
                 Cplx<R> w = tw[(r * k) << (LG - 2 - ls)];
@@ -536,24 +540,24 @@ __device__ inline void lds_fft(Cplx<R>* buf, const Cplx<R>* __restrict__ tw, boo
         }
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < P4; ++u)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) buf[fft_at<LG, LINE_FAST>(ll[u], oo[u] + r * Ns)] = y[u][r];
+            for (int r = 0; r < 4; ++r) buf[fft_at<LG, LP, LINE_FAST>(ll[u], oo[u] + r * Ns)] = y[u][r];
         __syncthreads();
     }
     if (LG & 1) {  // last stage radix 2, Ns = G / 2: outputs j and j + G/2, twiddle tw[j]
         constexpr int H2 = G / 2, LH = LG - 1;
-        Cplx<R> va[8], vb[8];
-        int ll[8], jj[8];
+        Cplx<R> va[P2], vb[P2];
+        int ll[P2], jj[P2];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < P2; ++u) {
             const int q = tid + u * 256;
             const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LH);
             const int j = LINE_FAST ? (q >> LNL) : (q & (H2 - 1));
-            const Cplx<R> a = buf[fft_at<LG, LINE_FAST>(l, j)];
+            const Cplx<R> a = buf[fft_at<LG, LP, LINE_FAST>(l, j)];
             Cplx<R> w = tw[j];
             if (inv) w.y = -w.y;
-            const Cplx<R> wb = cmul(buf[fft_at<LG, LINE_FAST>(l, j + H2)], w);
+            const Cplx<R> wb = cmul(buf[fft_at<LG, LP, LINE_FAST>(l, j + H2)], w);
             va[u] = {a.x + wb.x, a.y + wb.y};
             vb[u] = {a.x - wb.x, a.y - wb.y};
             ll[u] = l;
@@ -561,32 +565,33 @@ __device__ inline void lds_fft(Cplx<R>* buf, const Cplx<R>* __restrict__ tw, boo
         }
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            buf[fft_at<LG, LINE_FAST>(ll[u], jj[u])] = va[u];
-            buf[fft_at<LG, LINE_FAST>(ll[u], jj[u] + H2)] = vb[u];
+        for (int u = 0; u < P2; ++u) {
+            buf[fft_at<LG, LP, LINE_FAST>(ll[u], jj[u])] = va[u];
+            buf[fft_at<LG, LP, LINE_FAST>(ll[u], jj[u] + H2)] = vb[u];
         }
         __syncthreads();
     }
 }
 
 // X forward (R2C): pairs of real lines (y even, y odd) packed as one complex line.
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cplx<R>* __restrict__ out,
                                                    const Cplx<R>* __restrict__ tw) {
-    __shared__ Cplx<R> buf[kFftPoints];
+#pragma clang fp contract(off)
+    __shared__ Cplx<R> buf[1 << LP];
     constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int npair = kFftPoints / G;                   // complex lines per block
+    constexpr int npair = (1 << LP) / G;                    // complex lines per block
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;  // first real line
     const R* src = in + line0 * G;
     // G = 16: the grid's G^2 = 256 real rows fill half of one block; the rest is zero and not stored
     constexpr int rows = 2 * npair < G * G ? 2 * npair : G * G;
-    for (int e = threadIdx.x; e < 2 * kFftPoints; e += 256) {
+    for (int e = threadIdx.x; e < (2 << LP); e += 256) {
         const int line = e >> LG, x = e & (G - 1);
         const R v = line < rows ? src[e] : (R)0;
         if (line & 1) buf[(line >> 1) * G + x].y = v; else buf[(line >> 1) * G + x].x = v;
     }
     __syncthreads();
-    lds_fft<R, LG, false>(buf, tw, false);
+    lds_fft<R, LG, LP, false>(buf, tw, false);
     Cplx<R>* dst = out + line0 * H;
     for (int e = threadIdx.x; e < rows * H; e += 256) {
         const int row = e / H, k = e - row * H;
@@ -604,16 +609,16 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cpl
 }
 
 // X inverse (C2R): two half-spectrum rows A, B -> Z = A + i B, inverse FFT, re / im = the two real rows.
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ in, R* __restrict__ out,
                                                    const Cplx<R>* __restrict__ tw) {
-    __shared__ Cplx<R> buf[kFftPoints];
+    __shared__ Cplx<R> buf[1 << LP];
     constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int npair = kFftPoints / G;
+    constexpr int npair = (1 << LP) / G;
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;
     const Cplx<R>* src = in + line0 * H;
     constexpr int rows = 2 * npair < G * G ? 2 * npair : G * G;  // G = 16: half a block
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int p = e >> LG, k = e & (G - 1);
         const Cplx<R>* ra = src + (int64_t)(2 * p) * H;
         const Cplx<R>* rb = ra + H;
@@ -636,7 +641,7 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ i
         buf[p * G + k] = {a.x - b.y, a.y + b.x};
     }
     __syncthreads();
-    lds_fft<R, LG, false>(buf, tw, true);
+    lds_fft<R, LG, LP, false>(buf, tw, true);
     R* dst = out + line0 * G;
     for (int e = threadIdx.x; e < rows * G; e += 256) {
         const int line = e >> LG, x = e & (G - 1);
@@ -656,23 +661,23 @@ __device__ inline int fft_logical_block(int nblocks) {
 }
 
 // Y pass (complex, forward or inverse): block = (z, block of KB consecutive kx).
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw, int inv) {
-    __shared__ Cplx<R> buf[kFftPoints];
+    __shared__ Cplx<R> buf[1 << LP];
     constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int KB = kFftPoints / G, LKB = 12 - LG;
+    constexpr int KB = (1 << LP) / G, LKB = LP - LG;
     constexpr int nkb = (H + KB - 1) / KB;
     const int lb = fft_logical_block(G * nkb);
     const int z = lb / nkb, kx0 = (lb % nkb) * KB;
     Cplx<R>* base = spec + (int64_t)z * G * H;
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int y = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         buf[e] = (kx < H) ? base[(int64_t)y * H + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, LG, true>(buf, tw, inv != 0);
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    lds_fft<R, LG, LP, true>(buf, tw, inv != 0);
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int y = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         if (kx < H) base[(int64_t)y * H + kx] = buf[e];
@@ -682,31 +687,32 @@ __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, const
 // Z pass with the Green function: forward z FFT, multiply, inverse z FFT.
 // green(kx, ky, kz) = norm * T[kx] T[ky] T[kz] / (K2[kx] + K2[ky] + K2[kz]); zero for DC and
 // for max(|kx|,|ky|,|kz|) < k_min (apply_green_kernel, cufft_wrapper.cu:160-207).
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw,
                                                      const R* __restrict__ gtab, R norm, int k_min, int matched) {
-    __shared__ Cplx<R> buf[kFftPoints];
+#pragma clang fp contract(off)
+    __shared__ Cplx<R> buf[1 << LP];
     constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int KB = kFftPoints / G, LKB = 12 - LG;
+    constexpr int KB = (1 << LP) / G, LKB = LP - LG;
     constexpr int nkb = (H + KB - 1) / KB;
     const int lb = fft_logical_block(G * nkb);
     const int y = lb / nkb, kx0 = (lb % nkb) * KB;
     constexpr int64_t zs = (int64_t)G * H;
     Cplx<R>* base = spec + (int64_t)y * H;
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         buf[e] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, LG, true>(buf, tw, false);
+    lds_fft<R, LG, LP, true>(buf, tw, false);
     const R* T = gtab;
     const R* K2 = gtab + G;
     const R* Ks = gtab + 2 * G;  // signed k_i
     const R* D = gtab + 3 * G;   // transfer function of the gradient stencil along one axis
     const int ky = y;
     const int aky = ky <= G / 2 ? ky : G - ky;
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int kz = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         const int akz = kz <= G / 2 ? kz : G - kz;
@@ -727,8 +733,8 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
         buf[e] = {v.x * f, v.y * f};
     }
     __syncthreads();
-    lds_fft<R, LG, true>(buf, tw, true);
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    lds_fft<R, LG, LP, true>(buf, tw, true);
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         if (kx < H) base[z * zs + kx] = buf[e];
@@ -736,75 +742,79 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
 }
 
 // Inverse z pass alone (the z pass of the solver without the forward transform and Green multiply).
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw) {
-    __shared__ Cplx<R> buf[kFftPoints];
+    __shared__ Cplx<R> buf[1 << LP];
     constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int KB = kFftPoints / G, LKB = 12 - LG;
+    constexpr int KB = (1 << LP) / G, LKB = LP - LG;
     constexpr int nkb = (H + KB - 1) / KB;
     const int lb = fft_logical_block(G * nkb);
     const int y = lb / nkb, kx0 = (lb % nkb) * KB;
     constexpr int64_t zs = (int64_t)G * H;
     Cplx<R>* base = spec + (int64_t)y * H;
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         buf[e] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, LG, true>(buf, tw, true);
-    for (int e = threadIdx.x; e < kFftPoints; e += 256) {
+    lds_fft<R, LG, LP, true>(buf, tw, true);
+    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
         if (kx < H) base[z * zs + kx] = buf[e];
     }
 }
 
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 static hipError_t inverse_fft_lg(Ctx& c, R* out, hipStream_t s) {
-    constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int KB = kFftPoints / G, nkb = (H + KB - 1) / KB;
-    constexpr unsigned nx = (unsigned)(G * G > 2 * (kFftPoints / G) ? (int64_t)G * G / (2 * (kFftPoints / G)) : 1);
+    constexpr int G = 1 << LG, H = G / 2 + 1, NPT = 1 << LP;
+    constexpr int KB = NPT / G, nkb = (H + KB - 1) / KB;
+    constexpr unsigned nx = (unsigned)(G * G > 2 * (NPT / G) ? (int64_t)G * G / (2 * (NPT / G)) : 1);
     const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
     Cplx<R>* spec = (Cplx<R>*)c.spec;
-    k_fft_z_inv<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw);
-    k_fft_y<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
-    k_fft_x_c2r<R, LG><<<dim3(nx), dim3(256), 0, s>>>(spec, out, tw);
+    k_fft_z_inv<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw);
+    k_fft_y<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
+    k_fft_x_c2r<R, LG, LP><<<dim3(nx), dim3(256), 0, s>>>(spec, out, tw);
     return hipGetLastError();
 }
 
-template <typename R, int LG>
+template <typename R, int LG, int LP>
 static hipError_t fft_solve_lg(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
-    constexpr int G = 1 << LG, H = G / 2 + 1;
-    constexpr int KB = kFftPoints / G, nkb = (H + KB - 1) / KB;
-    constexpr unsigned nx = (unsigned)(G * G > 2 * (kFftPoints / G) ? (int64_t)G * G / (2 * (kFftPoints / G)) : 1);
+    constexpr int G = 1 << LG, H = G / 2 + 1, NPT = 1 << LP;
+    constexpr int KB = NPT / G, nkb = (H + KB - 1) / KB;
+    constexpr unsigned nx = (unsigned)(G * G > 2 * (NPT / G) ? (int64_t)G * G / (2 * (NPT / G)) : 1);
     const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
     Cplx<R>* spec = (Cplx<R>*)c.spec;
-    k_fft_x_r2c<R, LG><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, tw);
-    k_fft_y<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 0);
-    k_fft_z_green<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, (const R*)c.gtab,
+    k_fft_x_r2c<R, LG, LP><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, tw);
+    k_fft_y<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 0);
+    k_fft_z_green<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, (const R*)c.gtab,
                                                                          (R)c.green_norm, c.p.k_min, matched);
-    k_fft_y<R, LG><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
-    k_fft_x_c2r<R, LG><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, tw);
+    k_fft_y<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
+    k_fft_x_c2r<R, LG, LP><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, tw);
     return hipGetLastError();
 }
 
-// the mesh sizes the context accepts: G = 16 .. 512
-#define JTP_FFT_DISPATCH(CALL)              \
-    switch (c.logG) {                       \
-        case 4: return CALL(4);             \
-        case 5: return CALL(5);             \
-        case 6: return CALL(6);             \
-        case 7: return CALL(7);             \
-        case 8: return CALL(8);             \
-        case 9: return CALL(9);             \
-        default: return hipErrorInvalidValue; \
+// the mesh sizes the context accepts (G = 16 .. 512) and FFT tiles (2^11, 2^12 points; 2^13 for FP32)
+#define JTP_FFT_DISPATCH_LG(CALL, LP)          \
+    switch (c.logG) {                          \
+        case 4: return CALL(4, LP);            \
+        case 5: return CALL(5, LP);            \
+        case 6: return CALL(6, LP);            \
+        case 7: return CALL(7, LP);            \
+        case 8: return CALL(8, LP);            \
+        case 9: return CALL(9, LP);            \
+        default: return hipErrorInvalidValue;  \
     }
 
 template <typename R>
 static hipError_t inverse_fft_t(Ctx& c, R* out, hipStream_t s) {
-#define JTP_INV(LG) inverse_fft_lg<R, LG>(c, out, s)
-    JTP_FFT_DISPATCH(JTP_INV)
+#define JTP_INV(LG, LP) inverse_fft_lg<R, LG, LP>(c, out, s)
+    if constexpr (sizeof(R) == 4) {
+        if (c.p.fft_tile == 13) JTP_FFT_DISPATCH_LG(JTP_INV, 13)
+    }
+    if (c.p.fft_tile == 12) JTP_FFT_DISPATCH_LG(JTP_INV, 12)
+    JTP_FFT_DISPATCH_LG(JTP_INV, 11)
 #undef JTP_INV
 }
 
@@ -815,8 +825,12 @@ hipError_t pm_inverse_fft(Ctx& c, void* out, hipStream_t s) {
 
 template <typename R>
 static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
-#define JTP_SOLVE(LG) fft_solve_lg<R, LG>(c, rho, phi, s, matched)
-    JTP_FFT_DISPATCH(JTP_SOLVE)
+#define JTP_SOLVE(LG, LP) fft_solve_lg<R, LG, LP>(c, rho, phi, s, matched)
+    if constexpr (sizeof(R) == 4) {
+        if (c.p.fft_tile == 13) JTP_FFT_DISPATCH_LG(JTP_SOLVE, 13)
+    }
+    if (c.p.fft_tile == 12) JTP_FFT_DISPATCH_LG(JTP_SOLVE, 12)
+    JTP_FFT_DISPATCH_LG(JTP_SOLVE, 11)
 #undef JTP_SOLVE
 }
 

@@ -61,7 +61,7 @@ class Params(C.Structure):
         ("g_constant", C.c_double), ("leaf_size", C.c_int32), ("group_size", C.c_int32),
         ("precision", C.c_int32), ("split_mode", C.c_int32), ("gradient", C.c_int32),
         ("deconvolve", C.c_int32), ("pm_enabled", C.c_int32), ("tree_enabled", C.c_int32),
-        ("interlace", C.c_int32), ("theta_cut", C.c_double),
+        ("interlace", C.c_int32), ("theta_cut", C.c_double), ("fft_tile", C.c_int32),
     ]
 
 

@@ -1,4 +1,4 @@
-"""Autotuner for the walk's leaf size and group size.
+"""Autotuner for the walk's leaf size and group size and the FFT tile.
 
 The tuner times candidate (leaf_size, group_size) pairs on a sub-box of the caller's particles and
 keeps the fastest whose tree error matches the default's.  The sub-box is a cube of edge L / k cut
@@ -24,6 +24,7 @@ import numpy as np
 from . import FP32, ForceContext, DeviceArray, default_params, to_fixed
 
 DEFAULT_CANDIDATES = ((4, 64), (2, 64), (8, 64), (4, 32), (8, 32), (2, 32), (4, 16), (8, 16))
+FFT_TILES = (11, 12, 13)  # log2 points per FFT workgroup (jtp_params.fft_tile); all give the same bits
 
 
 def sub_box(pos: np.ndarray, species: np.ndarray, box_size: float, n_cells: int, k: int = 1, seed: int = 0):
@@ -44,7 +45,8 @@ def autotune(pos: np.ndarray, species: np.ndarray, params, m_plus: float, m_minu
              candidates=DEFAULT_CANDIDATES, tolerance: float = 1.05, repeats: int = 3, seed: int = 0) -> dict:
     """Pick (leaf_size, group_size) for `params` (FP32 form) on a 1/k^3 sub-box of the particles.
 
-    Returns {"leaf_size", "group_size", "table": [(leaf, group, ms, tree_error), ...], "sample": n}.
+    Returns {"leaf_size", "group_size", "fft_tile", "table": [(leaf, group, ms, tree_error), ...],
+    "fft_table": [(tile, ms of a PM-only evaluation), ...], "sample": n}.
     A candidate qualifies if its tree error is <= tolerance x the error of params' own (leaf, group).
     """
     x, sp, L, G = sub_box(pos, species, params.box_size, params.n_cells, k, seed)
@@ -52,7 +54,7 @@ def autotune(pos: np.ndarray, species: np.ndarray, params, m_plus: float, m_minu
         raise ValueError("sub-box holds too few particles to tune on")
     scale = L / params.box_size
 
-    def make(leaf, group, theta=None):
+    def make(leaf, group, theta=None, fft_tile=None, tree=None):
         p = default_params(L, G)
         for name in ("r_s", "r_cut", "softening"):
             setattr(p, name, getattr(params, name))
@@ -62,6 +64,9 @@ def autotune(pos: np.ndarray, species: np.ndarray, params, m_plus: float, m_minu
         p.split_mode, p.gradient, p.deconvolve = params.split_mode, params.gradient, params.deconvolve
         p.interlace, p.pm_enabled, p.tree_enabled = params.interlace, params.pm_enabled, params.tree_enabled
         p.leaf_size, p.group_size, p.precision = leaf, group, FP32
+        p.fft_tile = params.fft_tile if fft_tile is None else fft_tile
+        if tree is not None:
+            p.tree_enabled = tree
         if p.r_cut > L / 5:
             raise ValueError(f"sub-box edge {L:g} is too small for r_cut {p.r_cut:g}; use a smaller k")
         return ForceContext(p, m_plus, m_minus_abs)
@@ -91,5 +96,18 @@ def autotune(pos: np.ndarray, species: np.ndarray, params, m_plus: float, m_minu
     base = [row for row in table if row[0] == params.leaf_size and row[1] == params.group_size][0]
     ok = [row for row in table if row[3] <= tolerance * base[3]]
     pick = min(ok, key=lambda row: row[2])
-    return {"leaf_size": pick[0], "group_size": pick[1], "table": table, "sample": n, "sub_box": L,
-            "volume_fraction": scale ** 3}
+    # FFT tile: PM-only evaluations (tree off) of the same particles, fastest wins (results identical)
+    fft_table = []
+    for tile in FFT_TILES:
+        ctx = make(pick[0], pick[1], fft_tile=tile, tree=0)
+        ctx.compute_forces_device(n, dpos, dsp, dacc)
+        best = np.inf
+        for _ in range(repeats):
+            t = time.perf_counter()
+            ctx.compute_forces_device(n, dpos, dsp, dacc)
+            best = min(best, time.perf_counter() - t)
+        fft_table.append((tile, best * 1e3))
+        ctx.close()
+    tile = min(fft_table, key=lambda row: row[1])[0]
+    return {"leaf_size": pick[0], "group_size": pick[1], "fft_tile": tile, "table": table, "fft_table": fft_table,
+            "sample": n, "sub_box": L, "volume_fraction": scale ** 3}

@@ -34,12 +34,14 @@ def test_default_params_follow_reference_derivation():
     assert p.r_s == pytest.approx(1.25 * cell) and p.r_cut == pytest.approx(4.5 * 1.25 * cell)
     assert p.theta == 0.5 and p.leaf_size == 4 and p.group_size == 32 and p.precision == j.FP32
     assert p.interlace == 1 and p.theta_cut == pytest.approx(0.33) and p.deconvolve == 1
+    assert p.fft_tile == 0
 
 
 @pytest.mark.parametrize("field,value", [
     ("n_cells", 48), ("n_cells", 8), ("n_cells", 1024), ("box_size", 0.0), ("r_cut", 60.0), ("r_cut", 0.0),
     ("theta", 1.5), ("softening", 0.0), ("leaf_size", 0), ("leaf_size", 17), ("group_size", 65),
     ("precision", 2), ("split_mode", 3), ("gradient", 2), ("k_min", -1), ("r_s", 0.0), ("theta_cut", -0.1), ("interlace", 2),
+    ("fft_tile", 10), ("fft_tile", 14),
 ])
 def test_create_rejects_bad_params(field, value):
     lib = j.load_library()
