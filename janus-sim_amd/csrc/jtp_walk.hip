@@ -23,13 +23,15 @@
 
 namespace jtp {
 
-// LDS per wave (FP32): 384 x 16 B list + 576 x 4 B stack = 8.3 KB, so four 4-wave workgroups fit
-// a CU (16 waves, the 124-VGPR limit); pops shrink when the stack nears kStackCap, so a smaller
-// stack costs parallelism in deep traversals, never correctness.
+// LDS per wave (FP32): 448 x 16 B list + 576 x 4 B stack = 9.4 KB, so four 4-wave workgroups fit
+// a CU (150 of its 160 KB, 16 waves); a flush every 224 entries.  Measured at N = 2.2e7: a 256 /
+// 128 list (six workgroups per CU) and a 512 / 256 list (three) are both ~10 % slower; 384 / 192
+// is 3 % slower.  Pops shrink when the stack nears kStackCap, so a smaller stack costs
+// parallelism in deep traversals, never correctness.
 constexpr int kStackCap = 448;          // logical stack bound per wave
 constexpr int kStackPhys = kStackCap + 128;
-constexpr int kListCap = 384;
-constexpr int kFlushAt = 192;
+constexpr int kListCap = 448;
+constexpr int kFlushAt = 224;
 constexpr int kPad = 4;  // list slots kept free for the lane-sharing padding
 constexpr int32_t kNoNodeW = (int32_t)0x80000000;  // empty start cell
 
