@@ -135,7 +135,7 @@ void make_split_poly(Ctx& c) {
 static void free_particle_buffers(Ctx& c) {
     void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
                     c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
-                    c.frontier0, c.frontier1, c.dense_list, c.dense_corner};
+                    c.frontier0, c.frontier1, c.dense_list, c.dense_corner, c.interp_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c.keys0 = c.keys1 = nullptr;
@@ -147,6 +147,7 @@ static void free_particle_buffers(Ctx& c) {
     c.parent_int = c.parent_leaf = c.flags = nullptr;
     c.frontier0 = c.frontier1 = c.dense_list = nullptr;
     c.dense_corner = nullptr;
+    c.interp_list = nullptr;
     c.node_sum = c.node_mono = nullptr;
     c.groups = nullptr;
     c.pm_acc = nullptr;
@@ -179,6 +180,7 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.frontier0, cap));
     chk(dalloc(c.frontier1, cap));
     chk(dalloc(c.dense_list, cap / kDenseCell + 1));
+    chk(dalloc(c.interp_list, (size_t)interp_list_size(c.G3(), cap)));
     chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * (8 + 27) * rs));
     chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));
     chk(dalloc_bytes(c.node_mono, (size_t)cap * (rs == 8 ? 64 : 32)));
@@ -283,8 +285,10 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc_bytes(c.phi, G3 * rs);
     if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.rho2, G3 * rs);
     if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.phi2, G3 * rs);
-    if (e == hipSuccess) e = dalloc_bytes(c.grad, G3 * 4 * rs);
-    if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.grad2, G3 * 4 * rs);
+    if (p->precision == JTP_FP64) {  // gradient grids (the FP32 interpolation keeps them in LDS tiles)
+        if (e == hipSuccess) e = dalloc_bytes(c.grad, G3 * 4 * rs);
+        if (e == hipSuccess && p->interlace) e = dalloc_bytes(c.grad2, G3 * 4 * rs);
+    }
     if (e == hipSuccess) e = dalloc_bytes(c.spec, (size_t)G * G * c.Hc() * 2 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.twiddle, (size_t)2 * G * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)4 * G * rs);
@@ -605,6 +609,7 @@ int jtp_generate_ics(jtp_ctx* h, const jtp_ic_params* p, uint32_t* d_pos, void* 
     const int64_t n = c.G3();
     int rc = prepare(c, n);  // sort buffers for the sign keys
     if (rc != JTP_OK) return rc;
+    if (!c.grad) JTP_HIP(&c, dalloc_bytes(c.grad, (size_t)n * c.real_size()));  // third displacement grid
     const int64_t n_plus = (int64_t)std::llround((double)n / (1.0 + p->mu));  // ic_gen.rs:58-60
     hipStream_t s = (hipStream_t)stream;
     c.have_result = false;

@@ -21,10 +21,17 @@ constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Mor
 constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
 constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by wavefronts,
 constexpr int kDenseChunk = 1024;    // one per chunk of this many particles
+constexpr int kInterpChunk = 1024;   // FP32 interpolation: particles per workgroup of a heavy tile,
+constexpr int kLightMax = 256;       // tiles with at most this many are light (one particle per thread)
+// int2 entries of Ctx::interp_list for G^3 cells and up to n particles: the light-tile list, then
+// sum over tiles of ceil(np / kInterpChunk) <= n / kInterpChunk + n / (kLightMax + 1) + 1 chunks
+inline int64_t interp_list_size(int64_t G3, int64_t n) {
+    return (G3 / 512 + 1) / 2 + n / kInterpChunk + n / (kLightMax + 1) + 1;
+}
 constexpr int kMaxRounds = 128;      // size of the bottom-up control block (frontier count ... root flag)
 constexpr int kXcds = 8;             // MI355X accelerator complex dies; workgroup i runs on XCD i % 8
 // device counter block (int32): offsets
-constexpr int kCtrNGroups = 0, kCtrErr = 1, kCtrDense = 3;
+constexpr int kCtrNGroups = 0, kCtrErr = 1, kCtrDense = 3, kCtrInterp = 4;  // kCtrInterp: 2 words
 constexpr int kCtrRounds = 8;                       // [0] global frontier size, [kMaxRounds+1] root flag
 constexpr int kCtrRoot = kCtrRounds + kMaxRounds + 1;
 constexpr int kCtrXcd = kCtrRounds + kMaxRounds + 4;  // kXcds walk-queue heads
@@ -121,6 +128,8 @@ struct Ctx {
     int32_t *frontier0 = nullptr;    // bottom-up: nodes whose range crosses a chunk edge, ready to climb
     int32_t *frontier1 = nullptr;    // bottom-up: nodes whose range crosses a superchunk edge, ready to climb
     int32_t *dense_list = nullptr;   // deposit: the PM cells holding more than kDenseCell particles
+    int2 *interp_list = nullptr;     // FP32 interpolation: the light tiles (G^3 / 512 int32), then the
+                                     // (tile, first particle) of the heavy tiles' chunks; interp_list_size
     void *node_sum = nullptr;        // NodeSum per internal node
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
     int2 *groups = nullptr;
@@ -136,7 +145,8 @@ struct Ctx {
     void *rho = nullptr;             // real G^3 deposited mass per cell
     void *phi = nullptr;             // real G^3 potential
     void *rho2 = nullptr, *phi2 = nullptr;  // interlaced grid (nodes at the cell centres of the first)
-    void *grad = nullptr, *grad2 = nullptr;  // real4 [G^3] stencil gradient of phi / phi2 (w unused)
+    void *grad = nullptr, *grad2 = nullptr;  // FP64: real4 [G^3] stencil gradient of phi / phi2 (w unused); FP32: grad
+                                             // is allocated only by jtp_generate_ics (one real grid)
     void *spec = nullptr;            // complex G^2 (G/2+1)
     void *twiddle = nullptr;         // complex G  e^{-2 pi i m/G}
     void *gtab = nullptr;            // real 2G: per-axis window/Gaussian factor, k_i^2

@@ -10,7 +10,8 @@
 //     PmGrid::solve_poisson_filtered, pm_grid.rs:132-271 -> five LDS Stockham passes
 //     (R2C x, y, z with the Green multiply fused between forward and inverse z, y, C2R x).
 //   cic_gather (nbody_gpu_twopass.rs:162-257) and PmGrid::interpolate_force
-//     (pm_grid.rs:275-359) -> k_grad + k_interp on the single net potential (Phi- = -Phi+).
+//     (pm_grid.rs:275-359) -> on the single net potential (Phi- = -Phi+): FP32 k_interp_light /
+//     k_interp_heavy per 8^3-cell tile from LDS; FP64 k_grad + k_interp through gradient grids.
 //
 // Grid layout: real[z][y][x] (x fastest; index i + G(j + G k) as in pm_grid.rs:97),
 // half spectrum complex[z][y][kx], kx in [0, G/2].
@@ -989,12 +990,294 @@ __global__ void k_interp(const uint4* __restrict__ parts, int64_t n, const typen
     acc[3 * i + 2] = sg * a[2];
 }
 
+// FP32: one workgroup per 8^3-cell tile, whose particles are one contiguous Morton run.  The
+// potential around the tile is staged in LDS with the stencil's 2-cell halo, the gradient of every
+// node the tile's particles reach is computed there once (as k_grad does per node), and each
+// particle reads its 8 corner gradients per grid from LDS.  Grid 1: particles of cells 0..7 reach
+// nodes 0..8, whose stencils read phi at -2..10; grid 2 (its node j at the centre of cell j - 1):
+// nodes 0..9, phi2 at -2..11.  The arithmetic is k_grad's and interp_grid's, term for term.
+constexpr int kIp1 = kTile + 5;  // phi nodes -2 .. 10 per axis
+constexpr int kIp2 = kTile + 6;  // phi2 nodes -2 .. 11
+constexpr int kIg1 = kTile + 1;  // gradient nodes 0 .. 8
+constexpr int kIg2 = kTile + 2;  // gradient nodes 0 .. 9
+
+template <bool FOUR>
+__device__ inline float lds_stencil(const float* p, int b, int st, float inv) {
+    const float p1 = p[b + st], m1 = p[b - st];
+    if (FOUR) {
+        const float p2 = p[b + 2 * st], m2 = p[b - 2 * st];
+        return (8.0f * (p1 - m1) - (p2 - m2)) * inv;
+    }
+    return (p1 - m1) * inv;
+}
+
+// phi at nodes o - 2 .. o + NP - 3 per axis (periodic), staged row by row: each x-row is read as the
+// four aligned float4 groups at x = o - 4 .. o + 11 (o is a multiple of 8 and G of 4, so no group
+// straddles the periodic seam), four lanes per row, and the NP wanted values of a row are written to
+// LDS (x stride 1, y stride NP, z stride NP^2).  Item j = threadIdx.x + 256 k is group j & 3 of row
+// j >> 2; every load is issued before the first is used, so a thread waits for one round trip.
+template <int NP>
+constexpr int stage_per_thread() { return (NP * NP * 4 + 255) / 256; }
+
+template <int NP>
+__device__ inline void stage_load(const float* __restrict__ phi, int32_t ox, int32_t oy, int32_t oz, int logG,
+                                  float4 (&v)[stage_per_thread<NP>()]) {
+    const int32_t msk = (1 << logG) - 1;
+#pragma unroll
+    for (int k = 0; k < stage_per_thread<NP>(); ++k) {
+        const int j = threadIdx.x + 256 * k;
+        if (j < NP * NP * 4) {
+            const int row = j >> 2, ly = row % NP - 2, lz = row / NP - 2;
+            const int64_t rb = (((int64_t)((oz + lz) & msk)) << logG | ((oy + ly) & msk)) << logG;
+            v[k] = *(const float4*)(phi + rb + ((ox - 4 + 4 * (j & 3)) & msk));
+        }
+    }
+}
+
+template <int NP>
+__device__ inline void stage_store(const float4 (&v)[stage_per_thread<NP>()], float* p) {
+#pragma unroll
+    for (int k = 0; k < stage_per_thread<NP>(); ++k) {
+        const int j = threadIdx.x + 256 * k;
+        if (j < NP * NP * 4) {
+            float* row = p + (j >> 2) * NP + 2;  // x = 0 of the row
+            const int x0 = 4 * (j & 3) - 4;        // x of v[k].x
+            const float e[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (x0 + q >= -2 && x0 + q <= NP - 3) row[x0 + q] = e[q];
+        }
+    }
+}
+
+// the stencil gradient at nodes o .. o + NG - 1 per axis from the staged potential p -> g
+template <bool FOUR, int NP, int NG>
+__device__ inline void stage_grad(const float* p, float inv, float4* g) {
+    for (int i = threadIdx.x; i < NG * NG * NG; i += 256) {
+        const int nx = i % NG, ny = (i / NG) % NG, nz = i / (NG * NG);
+        const int b = ((nz + 2) * NP + ny + 2) * NP + nx + 2;
+        float4 v;
+        v.x = lds_stencil<FOUR>(p, b, 1, inv);
+        v.y = lds_stencil<FOUR>(p, b, NP, inv);
+        v.z = lds_stencil<FOUR>(p, b, NP * NP, inv);
+        v.w = 0.0f;
+        g[i] = v;
+    }
+}
+
+// CIC sum of the LDS gradients around local node (ix, iy, iz), in interp_grid's order
+template <int NG>
+__device__ inline void tile_cic(const float4* g, int ix, int iy, int iz, float fx, float fy, float fz, float a[3]) {
+    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    const float4* b = g + (iz * NG + iy) * NG + ix;
+    float ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const float w = wx[dx] * wy[dy] * wz[dz];
+                const float4 v = b[(dz * NG + dy) * NG + dx];
+                ax += w * v.x;
+                ay += w * v.y;
+                az += w * v.z;
+            }
+    a[0] = ax;
+    a[1] = ay;
+    a[2] = az;
+}
+
+// CIC sum over the 8 corners around local node (ix, iy, iz) with each corner's gradient taken from
+// the staged potential p (NP nodes per axis, node 0 at local -2) by the stencil: light tiles, where
+// computing every node's gradient first would cost more than it saves.
+template <bool FOUR, int NP>
+__device__ inline void tile_cic_phi(const float* p, int ix, int iy, int iz, float fx, float fy, float fz, float inv,
+                                    float a[3]) {
+    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    const int b0 = ((iz + 2) * NP + iy + 2) * NP + ix + 2;
+    float ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const float w = wx[dx] * wy[dy] * wz[dz];
+                const int b = b0 + (dz * NP + dy) * NP + dx;
+                ax += w * lds_stencil<FOUR>(p, b, 1, inv);
+                ay += w * lds_stencil<FOUR>(p, b, NP, inv);
+                az += w * lds_stencil<FOUR>(p, b, NP * NP, inv);
+            }
+    a[0] = ax;
+    a[1] = ay;
+    a[2] = az;
+}
+
+// Tiles are classed by particle count: light tiles (1 .. kLightMax particles, one per thread) are
+// listed for k_interp_light; every chunk of kInterpChunk particles of a heavier tile (a halo core
+// holds 1e4-1e5) is listed, as (tile, first particle), for a k_interp_heavy workgroup.  The list
+// order varies between runs; each particle is still computed by one thread with the same
+// arithmetic, so the results do not.
+__global__ __launch_bounds__(256) void k_interp_mark(const int32_t* __restrict__ cs, int32_t ntile,
+                                                     int32_t* __restrict__ light, int2* __restrict__ heavy,
+                                                     int32_t* __restrict__ count) {
+    const int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    int32_t beg = 0, np = 0;
+    if (t < ntile) {
+        beg = cs[(int64_t)t << 9];
+        np = cs[((int64_t)t + 1) << 9] - beg;
+    }
+    const bool lt = np > 0 && np <= kLightMax;
+    const int32_t nh = np > kLightMax ? (np + kInterpChunk - 1) / kInterpChunk : 0;
+    const int at_l = wave_reserve(lt ? 1 : 0, count);
+    if (lt) light[at_l] = t;
+    const int at_h = wave_reserve(nh, count + 1);
+    for (int32_t k = 0; k < nh; ++k) heavy[at_h + k] = make_int2(t, beg + k * kInterpChunk);
+}
+
+// List entry of block b, or -1: the entries are dealt to the 8 XCDs in contiguous ranges (block b
+// runs on XCD b % 8), so neighbouring tiles share that XCD's L2.  Needs gridDim.x >= cnt + 7.
+__device__ inline int32_t xcd_entry(uint32_t b, int32_t cnt) {
+    const int32_t per = (cnt + 7) >> 3, k = (int32_t)(b >> 3);
+    const int32_t e = (int32_t)(b & 7u) * per + k;
+    return (k < per && e < cnt) ? e : -1;
+}
+
+// local cell of a particle of the tile with origin o, for grid 1 (0..7) and the half-cell-shifted
+// grid 2 (0..8); the particle's cell is in the tile (its key is its position's), and the masks keep
+// every LDS index in range whatever the input
+__device__ inline int cell1(uint32_t u, int sh, int32_t o) { return (int)(((u >> sh) - (uint32_t)o) & (kTile - 1)); }
+__device__ inline int cell2(uint32_t u, int sh, int32_t o, uint32_t msk) {
+    return (int)min(((u >> sh) - (uint32_t)o) & msk, (uint32_t)kTile);
+}
+
+template <bool FOUR, bool BOTH>
+__global__ __launch_bounds__(256) void k_interp_heavy(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                                                      const int2* __restrict__ list, const int32_t* __restrict__ count,
+                                                      const float* __restrict__ phi, const float* __restrict__ phi2,
+                                                      int logG, float inv, float* __restrict__ acc) {
+    __shared__ float p1[kIp1 * kIp1 * kIp1];
+    __shared__ float p2[BOTH ? kIp2 * kIp2 * kIp2 : 1];
+    __shared__ float4 g1[kIg1 * kIg1 * kIg1];
+    __shared__ float4 g2[BOTH ? kIg2 * kIg2 * kIg2 : 1];
+    const int32_t e = xcd_entry(blockIdx.x, *count);
+    if (e < 0) return;  // the same for every thread of the block
+    const int2 w = list[e];
+    const uint64_t base = (uint64_t)w.x << 9;
+    const int32_t pbeg = w.y, pend = min(cs[base + 512], pbeg + kInterpChunk);
+    const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
+                  oz = (int32_t)compact3_21(base >> 2);
+    {
+        float4 v1[stage_per_thread<kIp1>()], v2[stage_per_thread<kIp2>()];
+        stage_load<kIp1>(phi, ox, oy, oz, logG, v1);
+        if (BOTH) stage_load<kIp2>(phi2, ox, oy, oz, logG, v2);
+        stage_store<kIp1>(v1, p1);
+        if (BOTH) stage_store<kIp2>(v2, p2);
+    }
+    __syncthreads();
+    stage_grad<FOUR, kIp1, kIg1>(p1, inv, g1);
+    if (BOTH) stage_grad<FOUR, kIp2, kIg2>(p2, inv, g2);
+    __syncthreads();
+    const int sh = 32 - logG;
+    const uint32_t msk = (1u << logG) - 1u, half = 1u << (31 - logG);
+    for (int32_t i = pbeg + (int32_t)threadIdx.x; i < pend; i += 256) {
+        const uint4 r = parts[i];
+        float a[3];
+        tile_cic<kIg1>(g1, cell1(r.x, sh, ox), cell1(r.y, sh, oy), cell1(r.z, sh, oz), frac_of<float>(r.x, logG),
+                       frac_of<float>(r.y, logG), frac_of<float>(r.z, logG), a);
+        float sg = (int32_t)r.w > 0 ? -1.0f : 1.0f;
+        if (BOTH) {
+            const uint32_t ux = r.x + half, uy = r.y + half, uz = r.z + half;
+            float b[3];
+            tile_cic<kIg2>(g2, cell2(ux, sh, ox, msk), cell2(uy, sh, oy, msk), cell2(uz, sh, oz, msk),
+                           frac_of<float>(ux, logG), frac_of<float>(uy, logG), frac_of<float>(uz, logG), b);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) a[ax] += b[ax];
+            sg *= 0.5f;
+        }
+        acc[3 * (int64_t)i] = sg * a[0];
+        acc[3 * (int64_t)i + 1] = sg * a[1];
+        acc[3 * (int64_t)i + 2] = sg * a[2];
+    }
+}
+
+// Light tiles: only the potential is staged (~20 KB of LDS, so up to 8 workgroups per CU hide the
+// loads' latency), the particle is loaded together with the halo, and each corner's gradient is
+// taken from LDS directly.
+template <bool FOUR, bool BOTH>
+__global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+                                                      const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+                                                      const float* __restrict__ phi, const float* __restrict__ phi2,
+                                                      int logG, float inv, float* __restrict__ acc) {
+    __shared__ float p1[kIp1 * kIp1 * kIp1];
+    __shared__ float p2[BOTH ? kIp2 * kIp2 * kIp2 : 1];
+    const int32_t e = xcd_entry(blockIdx.x, *count);
+    if (e < 0) return;  // the same for every thread of the block
+    const uint64_t base = (uint64_t)list[e] << 9;
+    const int32_t pbeg = cs[base], np = cs[base + 512] - pbeg;  // 1 .. kLightMax
+    const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
+                  oz = (int32_t)compact3_21(base >> 2);
+    const int32_t i = pbeg + (int32_t)threadIdx.x;
+    const bool have = (int32_t)threadIdx.x < np;
+    uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    if (have) r = parts[i];
+    {
+        float4 v1[stage_per_thread<kIp1>()], v2[stage_per_thread<kIp2>()];
+        stage_load<kIp1>(phi, ox, oy, oz, logG, v1);
+        if (BOTH) stage_load<kIp2>(phi2, ox, oy, oz, logG, v2);
+        stage_store<kIp1>(v1, p1);
+        if (BOTH) stage_store<kIp2>(v2, p2);
+    }
+    __syncthreads();
+    if (!have) return;  // no barrier follows
+    const int sh = 32 - logG;
+    const uint32_t msk = (1u << logG) - 1u, half = 1u << (31 - logG);
+    float a[3];
+    tile_cic_phi<FOUR, kIp1>(p1, cell1(r.x, sh, ox), cell1(r.y, sh, oy), cell1(r.z, sh, oz),
+                             frac_of<float>(r.x, logG), frac_of<float>(r.y, logG), frac_of<float>(r.z, logG), inv, a);
+    float sg = (int32_t)r.w > 0 ? -1.0f : 1.0f;
+    if (BOTH) {
+        const uint32_t ux = r.x + half, uy = r.y + half, uz = r.z + half;
+        float b[3];
+        tile_cic_phi<FOUR, kIp2>(p2, cell2(ux, sh, ox, msk), cell2(uy, sh, oy, msk), cell2(uz, sh, oz, msk),
+                                 frac_of<float>(ux, logG), frac_of<float>(uy, logG), frac_of<float>(uz, logG), inv, b);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) a[ax] += b[ax];
+        sg *= 0.5f;
+    }
+    acc[3 * (int64_t)i] = sg * a[0];
+    acc[3 * (int64_t)i + 1] = sg * a[1];
+    acc[3 * (int64_t)i + 2] = sg * a[2];
+}
+
 template <typename R>
 static hipError_t interp_t(Ctx& c, int64_t n, hipStream_t s) {
     using V = typename Vec4T<R>::T;
     const double h = c.p.box_size / (double)c.G();
     const bool four = c.p.gradient == JTP_GRAD_4PT;
     const R inv = (R)(four ? 1.0 / (12.0 * h) : 1.0 / (2.0 * h));
+    if constexpr (sizeof(R) == 4) {
+        const unsigned ntile = (unsigned)(c.G3() >> 9);  // 8^3-cell tiles (G >= 16)
+        int32_t* count = c.dev_counters + kCtrInterp;    // light tiles, heavy chunks
+        hipError_t e = launch_fill_i32(count, 2, 0, s);
+        if (e != hipSuccess) return e;
+        int32_t* light = (int32_t*)c.interp_list;  // ntile entries, then the heavy chunks
+        int2* heavy = c.interp_list + (ntile + 1) / 2;
+        k_interp_mark<<<dim3((ntile + 255) / 256), dim3(256), 0, s>>>(c.cell_start, (int32_t)ntile, light, heavy,
+                                                                       count);
+        const unsigned nheavy = (unsigned)(interp_list_size(c.G3(), n) - (ntile + 1) / 2);
+        const float* p2 = c.p.interlace ? (const float*)c.phi2 : nullptr;
+        auto kh = four ? (c.p.interlace ? k_interp_heavy<true, true> : k_interp_heavy<true, false>)
+                       : (c.p.interlace ? k_interp_heavy<false, true> : k_interp_heavy<false, false>);
+        auto kl = four ? (c.p.interlace ? k_interp_light<true, true> : k_interp_light<true, false>)
+                       : (c.p.interlace ? k_interp_light<false, true> : k_interp_light<false, false>);
+        kh<<<dim3(nheavy + 8), dim3(256), 0, s>>>(c.parts, c.cell_start, heavy, count + 1, (const float*)c.phi, p2,
+                                                  c.logG, (float)inv, (float*)c.pm_acc);
+        kl<<<dim3(ntile + 8), dim3(256), 0, s>>>(c.parts, c.cell_start, light, count, (const float*)c.phi, p2,
+                                                 c.logG, (float)inv, (float*)c.pm_acc);
+        return hipGetLastError();
+    }
     const unsigned ng = (unsigned)((c.G3() + kBlock - 1) / kBlock);
     auto kg = four ? k_grad<R, true> : k_grad<R, false>;
     kg<<<dim3(ng), dim3(kBlock), 0, s>>>((const R*)c.phi, c.logG, inv, (V*)c.grad);

@@ -58,6 +58,34 @@ def test_deposit_solve_interp(jtp, orc, precision, interlace):
     assert np.array_equal(ctx.stage(jtp.STAGE_ACC_PM), acc)
 
 
+@pytest.mark.parametrize("four,interlace", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_interp_tile_classes(jtp, orc, four, interlace):
+    """The FP32 interpolation works per 8^3-cell tile: empty tiles, light tiles (<= 256 particles),
+    heavy tiles of one chunk and heavy tiles cut into several 1024-particle chunks must all agree
+    with the oracle's interpolation (pm_grid.rs:275-359) of the same potential, the GPU's own, so
+    that only the interpolation's FP32 rounding (~1e-7 of max |a| here) is left."""
+    print(f"case: interpolation tile classes four={four} interlace={interlace}", flush=True)
+    w = synth.clustered(n_total=150_000, box_size=100.0, n_cells=64, n_halos=30, seed=21)
+    keep = ~np.all((w.pos >= 0) & (w.pos < 25.0), axis=1)  # an empty 16^3-cell corner: 8 empty tiles
+    pos = jtp.to_fixed(w.pos[keep], w.box_size)
+    sp = w.species[keep]
+    G = w.n_cells
+    c = np.bincount((orc.morton(pos) >> np.uint64(63 - 3 * 6 + 9)).astype(np.int64), minlength=G ** 3 // 512)
+    assert (c == 0).sum() >= 8 and ((c > 0) & (c <= 256)).any() and ((c > 256) & (c <= 1024)).any()
+    assert (c > 3 * 1024).any()
+    ctx = _ctx(jtp, w, precision=0, tree_enabled=0, interlace=interlace,
+               gradient=jtp.GRAD_4PT if four else jtp.GRAD_2PT)
+    acc = ctx.compute_forces(pos, sp)
+    grids = [(pos, jtp.STAGE_PHI)] + ([(_half_cell(pos, G), jtp.STAGE_PHI2)] if interlace else [])
+    acc_ref = np.zeros((len(sp), 3))
+    for p_in, st_phi in grids:
+        phi = ctx.stage(st_phi).astype(np.float64)
+        acc_ref += orc.interp(p_in, sp, G, w.box_size, phi, four=bool(four)) / len(grids)
+    assert np.max(np.abs(acc - acc_ref)) <= 2e-6 * np.max(np.abs(acc_ref))
+    # a second evaluation gives the same bits (the chunk lists are built in a varying order)
+    assert np.array_equal(ctx.compute_forces(pos, sp), acc)
+
+
 @pytest.mark.parametrize("precision", [1, 0])
 def test_species_mass_conservation(jtp, orc, precision):
     """Per-species deposited mass is conserved (pattern of pm_grid.rs:426-439)."""
