@@ -274,31 +274,33 @@ __global__ void k_deposit_both(const uint4* __restrict__ parts, const int32_t* _
 // shell around the tile (cells -1 .. 8 per axis), so the workgroup reads the particles of all
 // 10^3 cells and keeps only the contributions to its own nodes: every node is written by one
 // workgroup with a plain store, no global atomics and no zeroing pass.  Within the workgroup the
-// contributions are summed in LDS as 64-bit fixed point: integer addition is associative, so the
-// sums do not depend on the order in which the LDS atomics land and the grid is bit-reproducible.
-// Charges arrive pre-scaled by 2^k with |q| 2^k <= 2^30, so a contribution q W 2^k rounds to an
-// int32 and a node sum of n < 2^31 particles fits in 61 bits; the unit 2^-k is 2^-30 of the
-// largest |q| or finer (FP32 resolves 2^-24 of it).  Dense cells (> kDenseCell particles) enter through their
-// corner sums from k_dense_reduce*, computed with the same pre-scaled charges.  Each thread takes a
-// contiguous run of the tile's sparse particles, so the lanes of a wavefront sit in different
-// cells and seldom hit one LDS address together.  Tiles are dealt to the 8 XCDs in contiguous
-// Morton ranges (blocks are dispatched round-robin over the XCDs), so the shell cells a tile reads
-// were mostly read by its predecessor on the same XCD and sit in that XCD's L2.
+// sparse cells' contributions are summed in LDS as 32-bit fixed point: integer addition is
+// associative, so the sums do not depend on the order in which the LDS atomics land and the grid
+// is bit-reproducible.  Charges arrive pre-scaled by 2^k with |q| 2^k <= 2^22, and a node takes at
+// most 2 x 2 x 2 sparse cells' worth of unit weight per axis (node grid: cells n-1, n with weights
+// <= 1; interlaced grid: cells m-1, m, m+1 with weights <= 1/2, 1, 1/2) of <= kDenseCell = 32
+// particles each, so |sum| <= 8 * 32 * 2^22 = 2^30 fits an int32; the unit 2^-k is 2^-22 of the
+// largest |q| or finer, and rounding each contribution to it adds a relative error of ~2^-23 per
+// particle, as FP32 itself does.  Dense cells (> kDenseCell particles) enter through their corner
+// sums from k_dense_reduce*, computed with the same pre-scaled charges and added per owned node in
+// a fixed cell order when the node is stored.  Each thread takes a contiguous run of the tile's
+// sparse particles, so the lanes of a wavefront sit in different cells and seldom hit one LDS
+// address together.  Tiles are dealt to the 8 XCDs in contiguous Morton ranges (blocks are
+// dispatched round-robin over the XCDs), so the shell cells a tile reads were mostly read by its
+// predecessor on the same XCD and sit in that XCD's L2.  LDS: 20 KB, so 7 workgroups per CU hide
+// the latency of the cell-table and particle reads.
 constexpr int kTile = 8;
 constexpr int kShell = kTile + 2;                      // cells -1 .. 8 per axis
 constexpr int kShellCells = kShell * kShell * kShell;  // 1000
 constexpr int kN1 = kTile + 3;                         // node-grid nodes -1 .. 9 per axis
 constexpr int kN2 = kTile + 4;                         // centres of cells -2 .. 9 per axis
+constexpr int kFixBits = 22;                           // |q| 2^k <= 2^kFixBits
+static_assert(kFixBits + 8 <= 30, "a node sum must fit an int32");
+static_assert(kDenseCell <= 32, "the int32 bound assumes <= 32 particles per sparse cell");
 // the largest LDS index a shell particle reaches: cell 8 (index 9) plus corner offset 1 per axis in
 // a1; cell 8 above its centre (index 10) plus offset 1 per axis in a2
 static_assert(((kTile + 2) * kN1 + kTile + 2) * kN1 + kTile + 2 < kN1 * kN1 * kN1, "a1 too small");
 static_assert(((kTile + 3) * kN2 + kTile + 3) * kN2 + kTile + 3 < kN2 * kN2 * kN2, "a2 too small");
-
-// |v| <= 2^30: one hardware float -> int32 conversion, then sign extension (a float -> int64
-// conversion has no instruction and costs ~20)
-__device__ inline long long fix_round(float v) { return (long long)__float2int_rn(v); }
-
-__device__ inline void lds_add(unsigned long long* a, long long v) { atomicAdd(a, (unsigned long long)v); }
 
 template <bool BOTH>
 __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
@@ -310,11 +312,12 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
     // every node the shell's particles reach, so that the corner offsets are compile-time constants
     // (LDS instruction offsets) and no contribution needs a bounds test; only the tile's own nodes
     // are stored at the end
-    __shared__ unsigned long long a1[kN1 * kN1 * kN1];           // node-grid nodes -1 .. 9, index l + 1
-    __shared__ unsigned long long a2[BOTH ? kN2 * kN2 * kN2 : 1];  // centres of cells -2 .. 9, index l + 2
-    __shared__ int32_t cst[kShellCells];      // particle start of each shell cell
+    __shared__ int32_t a1[kN1 * kN1 * kN1];           // node-grid nodes -1 .. 9, index l + 1
+    __shared__ int32_t a2[BOTH ? kN2 * kN2 * kN2 : 1];  // centres of cells -2 .. 9, index l + 2
+    __shared__ int32_t cst[kShellCells];      // particle start of a sparse shell cell; -1 - slot of a dense one
     __shared__ int32_t pre[kShellCells + 1];  // prefix of the sparse cells' counts
     __shared__ int32_t wsum[4];
+    __shared__ int32_t any_dense;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t G = 1ll << logG;
     const int32_t msk = (int32_t)G - 1;
@@ -326,9 +329,10 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
     for (int i = tid; i < kN1 * kN1 * kN1; i += 256) a1[i] = 0;
     if (BOTH)
         for (int i = tid; i < kN2 * kN2 * kN2; i += 256) a2[i] = 0;
-    __syncthreads();
+    if (tid == 0) any_dense = 0;
     // shell cells 4 tid .. 4 tid + 3 (s = (cz + 1) 100 + (cy + 1) 10 + cx + 1)
     int cnt[4] = {0, 0, 0, 0};
+    bool dense = false;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int sc = 4 * tid + u;
@@ -337,23 +341,12 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
         const uint64_t cm = spread3_21((uint32_t)((ox + cx) & msk)) | (spread3_21((uint32_t)((oy + cy) & msk)) << 1) |
                             (spread3_21((uint32_t)((oz + cz) & msk)) << 2);
         const int32_t st = cs[cm], k = cs[cm + 1] - st;
-        cst[sc] = st;
         if (k <= kDenseCell) {
+            cst[sc] = st;
             cnt[u] = k;
-            continue;
-        }
-        const int64_t sl = slot[cm];
-        const int b1 = ((cz + 1) * kN1 + cy + 1) * kN1 + cx + 1;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)  // node-grid node = cell + (q & 1, q >> 1 & 1, q >> 2)
-            lds_add(&a1[b1 + ((q >> 2) * kN1 + ((q >> 1) & 1)) * kN1 + (q & 1)],
-                    __double2ll_rn((double)corner8[8 * sl + q]));
-        if (BOTH) {
-            const int b2 = ((cz + 1) * kN2 + cy + 1) * kN2 + cx + 1;  // centre of cell c - 1
-#pragma unroll
-            for (int q = 0; q < 27; ++q)  // centre of cell c + e, e = (q % 3, q / 3 % 3, q / 9) - 1
-                lds_add(&a2[b2 + ((q / 9) * kN2 + (q / 3) % 3) * kN2 + q % 3],
-                        __double2ll_rn((double)corner27[27 * sl + q]));
+        } else {
+            cst[sc] = -1 - slot[cm];
+            dense = true;
         }
     }
     // block exclusive scan of the sparse counts in shell-cell order
@@ -365,7 +358,8 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
         if (lane >= off) incl += v;
     }
     if (lane == 63) wsum[wv] = incl;
-    __syncthreads();
+    __syncthreads();  // also orders the zeroing before the atomics below
+    if (dense) any_dense = 1;
     int woff = 0;
     for (int w = 0; w < wv; ++w) woff += wsum[w];
     int run = woff + incl - mine;
@@ -397,11 +391,11 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
             const float q = (int32_t)r.w > 0 ? qp : qm;
             {
                 const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {q * (1.0f - fz), q * fz};
-                unsigned long long* a = &a1[((cz + 1) * kN1 + cy + 1) * kN1 + cx + 1];
+                int32_t* a = &a1[((cz + 1) * kN1 + cy + 1) * kN1 + cx + 1];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                    lds_add(a + (dz * kN1 + dy) * kN1 + dx, fix_round(wz[dz] * wy[dy] * wx[dx]));
+                    atomicAdd(a + (dz * kN1 + dy) * kN1 + dx, __float2int_rn(wz[dz] * wy[dy] * wx[dx]));
                 }
             }
             if (BOTH) {
@@ -412,25 +406,44 @@ __global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ 
                 const float vx[2] = {lox ? 0.5f - fx : 1.5f - fx, lox ? 0.5f + fx : fx - 0.5f};
                 const float vy[2] = {loy ? 0.5f - fy : 1.5f - fy, loy ? 0.5f + fy : fy - 0.5f};
                 const float vz[2] = {q * (loz ? 0.5f - fz : 1.5f - fz), q * (loz ? 0.5f + fz : fz - 0.5f)};
-                unsigned long long* a = &a2[(bz * kN2 + by) * kN2 + bx];
+                int32_t* a = &a2[(bz * kN2 + by) * kN2 + bx];
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                    lds_add(a + (dz * kN2 + dy) * kN2 + dx, fix_round(vz[dz] * vy[dy] * vx[dx]));
+                    atomicAdd(a + (dz * kN2 + dy) * kN2 + dx, __float2int_rn(vz[dz] * vy[dy] * vx[dx]));
                 }
             }
         }
     }
     __syncthreads();
     // node (x, y, z) of the tile: the node grid's node at cell corner o + l, the interlaced grid's
-    // node at the centre of cell o + l, stored at index o + l + 1
+    // node at the centre of cell o + l; the dense cells around it add their corner sums in a fixed
+    // order (node grid: cells l - e, e in {0,1}^3; interlaced: cells l - e, e in {-1,0,1}^3)
+    const bool dn = any_dense != 0;
     for (int i = tid; i < kTile * kTile * kTile; i += 256) {
         const int lx = i % kTile, ly = (i / kTile) % kTile, lz = i / (kTile * kTile);
-        rho[((int64_t)(oz + lz) * G + (oy + ly)) * G + (ox + lx)] =
-            (float)((double)(long long)a1[((lz + 1) * kN1 + ly + 1) * kN1 + lx + 1] * unit);
-        if (BOTH)
+        double v1 = (double)a1[((lz + 1) * kN1 + ly + 1) * kN1 + lx + 1];
+        if (dn) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {  // cell l - e (shell index l - e + 1), its corner e
+                const int ex = q & 1, ey = (q >> 1) & 1, ez = q >> 2;
+                const int32_t c = cst[((lz - ez + 1) * kShell + ly - ey + 1) * kShell + lx - ex + 1];
+                if (c < 0) v1 += (double)corner8[8 * (int64_t)(-1 - c) + q];
+            }
+        }
+        rho[((int64_t)(oz + lz) * G + (oy + ly)) * G + (ox + lx)] = (float)(v1 * unit);
+        if (BOTH) {
+            double v2 = (double)a2[((lz + 2) * kN2 + ly + 2) * kN2 + lx + 2];
+            if (dn) {
+                for (int q = 0; q < 27; ++q) {  // cell l - e, e = (q % 3, q / 3 % 3, q / 9) - 1: its entry q
+                    const int ex = q % 3 - 1, ey = (q / 3) % 3 - 1, ez = q / 9 - 1;
+                    const int32_t c = cst[((lz - ez + 1) * kShell + ly - ey + 1) * kShell + lx - ex + 1];
+                    if (c < 0) v2 += (double)corner27[27 * (int64_t)(-1 - c) + q];
+                }
+            }
             rho2[((int64_t)((oz + lz + 1) & msk) * G + ((oy + ly + 1) & msk)) * G + ((ox + lx + 1) & msk)] =
-                (float)((double)(long long)a2[((lz + 2) * kN2 + ly + 2) * kN2 + lx + 2] * unit);
+                (float)(v2 * unit);
+        }
     }
 }
 
@@ -876,10 +889,10 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     } else {
         float* c8 = (float*)c.dense_corner;
         float* c27 = c8 + 8 * nd_cap;
-        // charges scaled by 2^k, |q| 2^k <= 2^30 (exact: a power of two); the grids' unit is 2^-k
+        // charges scaled by 2^k, |q| 2^k <= 2^kFixBits (exact: a power of two); the grids' unit is 2^-k
         int ex = 0;
         (void)std::frexp(c.m_plus > c.m_minus ? c.m_plus : c.m_minus, &ex);  // max |q| <= 2^ex
-        const int k = 30 - ex;
+        const int k = kFixBits - ex;
         const float qp = (float)std::ldexp(c.m_plus, k), qm = (float)-std::ldexp(c.m_minus, k);
         const double unit = std::ldexp(1.0, -k);
         const unsigned ntile = (unsigned)(G3 >> 9);  // 8^3-cell tiles (G >= 16: a multiple of 8)
