@@ -134,7 +134,7 @@ void make_split_poly(Ctx& c) {
 
 static void free_particle_buffers(Ctx& c) {
     void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
-                    c.parent_int, c.parent_leaf, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
+                    c.up, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
                     c.frontier0, c.frontier1, c.dense_list, c.dense_corner, c.interp_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -144,7 +144,8 @@ static void free_particle_buffers(Ctx& c) {
     c.parts = nullptr;
     c.node_geo = nullptr;
     c.node_kids = nullptr;
-    c.parent_int = c.parent_leaf = c.flags = nullptr;
+    c.up = nullptr;
+    c.flags = nullptr;
     c.frontier0 = c.frontier1 = c.dense_list = nullptr;
     c.dense_corner = nullptr;
     c.interp_list = nullptr;
@@ -174,8 +175,7 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.parts, cap));
     chk(dalloc(c.node_geo, 2 * cap));
     chk(dalloc(c.node_kids, cap));
-    chk(dalloc(c.parent_int, cap));
-    chk(dalloc(c.parent_leaf, cap));
+    chk(dalloc(c.up, cap));
     chk(dalloc(c.flags, cap));
     chk(dalloc(c.frontier0, cap));
     chk(dalloc(c.frontier1, cap));

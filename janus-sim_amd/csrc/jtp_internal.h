@@ -122,8 +122,7 @@ struct Ctx {
     // tree
     uint4 *node_geo = nullptr;       // 2 per internal node
     int2 *node_kids = nullptr;
-    int32_t *parent_int = nullptr;   // parent of internal node
-    int32_t *parent_leaf = nullptr;  // parent of leaf
+    int2 *up = nullptr;              // internal node: (parent, sibling); the root's parent is -1
     int32_t *flags = nullptr;        // bottom-up arrival counters
     int32_t *frontier0 = nullptr;    // bottom-up: nodes whose range crosses a chunk edge, ready to climb
     int32_t *frontier1 = nullptr;    // bottom-up: nodes whose range crosses a superchunk edge, ready to climb

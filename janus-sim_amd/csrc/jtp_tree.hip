@@ -34,8 +34,7 @@ __device__ inline int kdelta(const uint64_t* __restrict__ k, int64_t n, int64_t 
 }
 
 __global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, uint4* __restrict__ geo,
-                         int2* __restrict__ kids, int32_t* __restrict__ parent_int,
-                         int32_t* __restrict__ parent_leaf, int32_t* __restrict__ flags) {
+                         int2* __restrict__ kids, int2* __restrict__ up, int32_t* __restrict__ flags) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int64_t first, last;
@@ -69,8 +68,10 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, uint4* __
     kids[i] = make_int2(lc, rc);
     geo[2 * i].w = (uint32_t)first;
     geo[2 * i + 1].w = (uint32_t)last;
-    if (lc >= 0) parent_int[lc] = (int32_t)i; else parent_leaf[~lc] = (int32_t)i;
-    if (rc >= 0) parent_int[rc] = (int32_t)i; else parent_leaf[~rc] = (int32_t)i;
+    // an internal child's parent and sibling, so that the bottom-up climb reads one record per level
+    if (lc >= 0) up[lc] = make_int2((int32_t)i, rc);
+    if (rc >= 0) up[rc] = make_int2((int32_t)i, lc);
+    if (i == 0) up[0] = make_int2(-1, 0);  // the root
     flags[i] = (lc < 0 ? 1 : 0) + (rc < 0 ? 1 : 0);  // arrivals still missing: 2 - this
 }
 
@@ -205,17 +206,17 @@ __device__ inline ChildData finish_start(int32_t p, const uint4* __restrict__ pa
 }
 
 // The second arrival at node `par` climbs with the data of its child `p` in registers and loads
-// only the sibling.
+// only the sibling `sib` (up[p].y).  The left child's index is the split gamma and the right one's
+// gamma + 1, leaves and internal nodes alike, so p is the left child iff p < the sibling's index.
 template <int PREC>
-__device__ inline ChildData finish_up(int32_t par, int32_t p, const ChildData& cur, const uint4* __restrict__ parts,
-                                      uint4* __restrict__ geo, const int2* __restrict__ kids,
+__device__ inline ChildData finish_up(int32_t par, int32_t p, int32_t sib, const ChildData& cur,
+                                      const uint4* __restrict__ parts, uint4* __restrict__ geo,
                                       NodeSum* __restrict__ sums, void* mono, double qp, double qm) {
-    const int2 k = kids[par];
-    const bool left = k.x == p;
-    ChildData sib;
-    load_child(left ? k.y : k.x, parts, geo, sums, sib);
-    return left ? combine_write<PREC>(par, cur, sib, geo, sums, mono, qp, qm)
-                : combine_write<PREC>(par, sib, cur, geo, sums, mono, qp, qm);
+    const bool left = p < (sib < 0 ? ~sib : sib);
+    ChildData o;
+    load_child(sib, parts, geo, sums, o);
+    return left ? combine_write<PREC>(par, cur, o, geo, sums, mono, qp, qm)
+                : combine_write<PREC>(par, o, cur, geo, sums, mono, qp, qm);
 }
 
 // Bottom-up pass in two launches, with no device-scope fence (the reference climbs with a
@@ -237,18 +238,22 @@ __device__ inline ChildData finish_up(int32_t par, int32_t p, const ChildData& c
 //    range crosses a superchunk edge).
 // Each node is finished exactly once, with its children combined in the fixed order (left, right).
 constexpr int kBuChunk = 4096;
+static_assert(kBuChunk <= 65536, "start offsets are 16-bit");
 
 __device__ inline void wg_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
 __device__ inline void wg_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 
+constexpr int kBuLocalThreads = 512;  // 8 waves per workgroup, 4 workgroups (32 waves) per CU
+
 template <int PREC>
-__global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
-                                                  const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+__global__ __launch_bounds__(kBuLocalThreads) void k_bu_local(const uint4* __restrict__ parts, int64_t nI,
+                                                              uint4* __restrict__ geo,
+                                                  const int2* __restrict__ kids, const int2* __restrict__ up,
                                                   int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                   double qp, double qm, int32_t* __restrict__ gfront,
                                                   int32_t* __restrict__ ctrl) {
     __shared__ int32_t cnt[kBuChunk];    // arrival count of a local node, -1 if the node is not local
-    __shared__ int32_t start[kBuChunk];  // local nodes with two leaf children
+    __shared__ uint16_t start[kBuChunk];  // local nodes with two leaf children (offset from base)
     __shared__ int32_t nstart;
     const int tid = threadIdx.x;
     const int64_t base = (int64_t)blockIdx.x * kBuChunk;
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
             const int64_t first = geo[2 * i].w, last = geo[2 * i + 1].w;
             if (first >= base && last <= lastLeaf) {
                 c = leaves;
-                if (leaves == 2) start[atomicAdd(&nstart, 1)] = (int32_t)i;
+                if (leaves == 2) start[atomicAdd(&nstart, 1)] = (uint16_t)k;
             } else if (leaves == 2) {
                 gfront[atomicAdd(&ctrl[0], 1)] = (int32_t)i;
             }
@@ -275,21 +280,22 @@ __global__ __launch_bounds__(256) void k_bu_local(const uint4* __restrict__ part
     __syncthreads();
     const int ns = nstart;
     for (int t = tid; t < ns; t += blockDim.x) {
-        int32_t p = start[t];
+        int32_t p = (int32_t)(base + start[t]);
         ChildData cur = finish_start<PREC>(p, parts, geo, kids, sums, mono, qp, qm);
         while (true) {
             if (p == 0) {
                 ctrl[kMaxRounds + 1] = 1;  // root complete
                 break;
             }
-            const int32_t par = parent_int[p];
+            const int2 u = up[p];
+            const int32_t par = u.x;
             if (par < 0 || par >= nI) break;
             const int64_t lp = (int64_t)par - base;
             if (lp >= 0 && lp < kBuChunk && cnt[lp] >= 0) {
                 wg_release();
                 if (atomicAdd(&cnt[lp], 1) != 1) break;
                 wg_acquire();
-                cur = finish_up<PREC>(par, p, cur, parts, geo, kids, sums, mono, qp, qm);
+                cur = finish_up<PREC>(par, p, u.y, cur, parts, geo, sums, mono, qp, qm);
                 p = par;
             } else {
                 if (atomicAdd(&flags[par], 1) == 1) gfront[atomicAdd(&ctrl[0], 1)] = par;
@@ -308,7 +314,7 @@ constexpr int64_t kBuSuper = (int64_t)kBuChunk * 64;
 
 template <int PREC>
 __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
-                                                 const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+                                                 const int2* __restrict__ kids, const int2* __restrict__ up,
                                                  int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                  double qp, double qm, const int32_t* __restrict__ starts,
                                                  int32_t* __restrict__ next, int32_t* __restrict__ ctrl) {
@@ -327,7 +333,8 @@ __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts
                 ctrl[kMaxRounds + 1] = 1;
                 break;
             }
-            const int32_t par = parent_int[p];
+            const int2 u = up[p];
+            const int32_t par = u.x;
             if (par < 0 || par >= nI) break;
             const bool inside = par >= lo && par <= hi && (int64_t)geo[2 * par].w >= lo &&
                                 (int64_t)geo[2 * par + 1].w <= hi;
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts
                 wg_release();
                 if (atomicAdd(&flags[par], 1) != 1) break;
                 wg_acquire();
-                cur = finish_up<PREC>(par, p, cur, parts, geo, kids, sums, mono, qp, qm);
+                cur = finish_up<PREC>(par, p, u.y, cur, parts, geo, sums, mono, qp, qm);
                 p = par;
             } else {
                 if (atomicAdd(&flags[par], 1) == 1) next[atomicAdd(&ctrl[1], 1)] = par;
@@ -347,7 +354,7 @@ __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts
 
 template <int PREC>
 __global__ __launch_bounds__(1024) void k_bu_top(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
-                                                 const int2* __restrict__ kids, const int32_t* __restrict__ parent_int,
+                                                 const int2* __restrict__ kids, const int2* __restrict__ up,
                                                  int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                  double qp, double qm, const int32_t* __restrict__ starts,
                                                  int32_t* __restrict__ ctrl) {
@@ -361,12 +368,13 @@ __global__ __launch_bounds__(1024) void k_bu_top(const uint4* __restrict__ parts
                 ctrl[kMaxRounds + 1] = 1;
                 break;
             }
-            const int32_t par = parent_int[p];
+            const int2 u = up[p];
+            const int32_t par = u.x;
             if (par < 0 || par >= nI) break;
             wg_release();
             if (atomicAdd(&flags[par], 1) != 1) break;
             wg_acquire();
-            cur = finish_up<PREC>(par, p, cur, parts, geo, kids, sums, mono, qp, qm);
+            cur = finish_up<PREC>(par, p, u.y, cur, parts, geo, sums, mono, qp, qm);
             p = par;
         }
     }
@@ -486,33 +494,29 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
     if (n >= 2) {
         const int64_t nI = n - 1;
         const unsigned nbI = (unsigned)((nI + kBlock - 1) / kBlock);
-        hipError_t e = launch_fill_i32(c.parent_int, nI, -1, s);
-        if (e == hipSuccess) e = launch_fill_i32(c.parent_leaf, n, -1, s);
-        if (e != hipSuccess) return e;
-        k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, c.node_geo, c.node_kids, c.parent_int,
-                                                    c.parent_leaf, c.flags);
+        k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, c.node_geo, c.node_kids, c.up, c.flags);
         int32_t* ctrl = c.dev_counters + kCtrRounds;
-        e = launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
+        hipError_t e = launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
         if (e != hipSuccess) return e;
         const unsigned nbLocal = (unsigned)((n + kBuChunk - 1) / kBuChunk);
         const unsigned nbSuper = (unsigned)((n + kBuSuper - 1) / kBuSuper);
         const double qp = c.p.g_constant * c.m_plus, qm = c.p.g_constant * c.m_minus;
         NodeSum* sums = (NodeSum*)c.node_sum;
         if (c.p.precision == JTP_FP64) {
-            k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+            k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(kBuLocalThreads), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
-            k_bu_mid<JTP_FP64><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+            k_bu_mid<JTP_FP64><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
                                                                     c.flags, sums, c.node_mono, qp, qm, c.frontier0,
                                                                     c.frontier1, ctrl);
-            k_bu_top<JTP_FP64><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
+            k_bu_top<JTP_FP64><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up, c.flags,
                                                               sums, c.node_mono, qp, qm, c.frontier1, ctrl);
         } else {
-            k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(256), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+            k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(kBuLocalThreads), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
-            k_bu_mid<JTP_FP32><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int,
+            k_bu_mid<JTP_FP32><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
                                                                     c.flags, sums, c.node_mono, qp, qm, c.frontier0,
                                                                     c.frontier1, ctrl);
-            k_bu_top<JTP_FP32><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.parent_int, c.flags,
+            k_bu_top<JTP_FP32><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up, c.flags,
                                                               sums, c.node_mono, qp, qm, c.frontier1, ctrl);
         }
     }
