@@ -198,6 +198,50 @@ def test_walk_deterministic(jtp, gs):
     assert np.array_equal(a1, a2)
 
 
+def _worst_case_cells(G):
+    """32 particles (the most a cell holds before it counts as dense) of the heavier species in
+    every cell around a node-grid node and around an interlaced centre, each at the position of
+    largest weight on that node: the largest node sums the sparse path can meet (8 cells x 32
+    particles x weight 1 at a node; 27 cells x 32 particles x weights 1/2, 1, 1/2 per axis at a
+    centre).  Nodes at a tile corner (16), inside a tile (50) and centres at a tile edge (24, 40)."""
+    h = 1 << (32 - int(np.log2(G)))
+    pts = []
+    for n in (16, 50):  # node n: cell n - 1 at its top corner, cell n at its bottom corner
+        ax = [n * h - 1, n * h]
+        pts += [(x, y, z) for x in ax for y in ax for z in ax]
+    for m in (24, 40):  # centre of cell m: cell m - 1 at its top, m at its centre, m + 1 at its bottom
+        ax = [m * h - 1, m * h + h // 2, (m + 1) * h]
+        pts += [(x, y, z) for x in ax for y in ax for z in ax]
+    pos = np.repeat(np.array(pts, np.uint64), 32, axis=0).astype(np.uint32)
+    return pos, np.full(len(pos), -1, np.int8)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_deposit_worst_case_sparse_sums(jtp, orc, precision):
+    """The FP32 tile deposit sums sparse cells in 32-bit fixed point with |q| 2^k <= 2^22; its
+    bound |node sum| <= 2^30 is reached here and must not overflow."""
+    print(f"case: deposit worst-case sparse node sums precision={precision}", flush=True)
+    G, L = 64, 100.0
+    pos, sp = _worst_case_cells(G)
+    counts = np.bincount(orc.morton(pos) >> np.uint64(63 - 18), minlength=G ** 3)
+    assert counts.max() == 32  # every loaded cell is sparse, none dense
+    p = jtp.default_params(L, G)
+    p.precision, p.tree_enabled = precision, 0
+    ctx = jtp.ForceContext(p, 1.0, 19.0)
+    ctx.compute_forces(pos, sp)
+    tol = 1e-12 if precision == 1 else 1e-6
+    for st, p_in in ((jtp.STAGE_RHO, pos), (jtp.STAGE_RHO2, _half_cell(pos, G))):
+        rho = ctx.stage(st).astype(np.float64)
+        rho_ref = orc.deposit(p_in, sp, 1.0, 19.0, G)
+        assert np.max(np.abs(rho - rho_ref)) <= tol * np.max(np.abs(rho_ref))
+    # the nodes at the bound: 256 particle masses at the node and at the centres
+    rho_ref = orc.deposit(pos, sp, 1.0, 19.0, G)
+    rho2_ref = orc.deposit(_half_cell(pos, G), sp, 1.0, 19.0, G)
+    unit = np.abs(orc.deposit(pos[:1], sp[:1], 1.0, 19.0, G)).sum()  # one particle's mass in grid units
+    assert np.max(np.abs(rho_ref)) / unit == pytest.approx(256.0, rel=1e-6)
+    assert np.max(np.abs(rho2_ref)) / unit == pytest.approx(256.0, rel=1e-6)
+
+
 @pytest.mark.parametrize("precision", [1, 0])
 def test_deposit_dense_cells(jtp, orc, precision):
     """Halo cores put > 32 particles in a cell: the wavefront pre-reduction path must agree with the oracle."""
