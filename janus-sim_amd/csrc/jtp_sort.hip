@@ -147,13 +147,19 @@ __global__ __launch_bounds__(256) void k_scan_part(uint32_t* __restrict__ part, 
     }
 }
 
+// Wavefront w of the block owns the contiguous slice [512 w, 512 w + 512) of the tile (lane l
+// holds the keys at 512 w + 64 r + l, r = 0..7), so the ranks follow the input order with no
+// block barrier between rounds: a round's rank is the wave's running count of the digit (its own
+// LDS counters, read and then bumped by the digit's first lane, in program order within the wave)
+// plus the number of earlier lanes holding the digit.  One block step then turns the per-wave
+// counts into each wave's start within the tile's run of the digit.
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restrict__ kin,
                                                        const int32_t* __restrict__ vin,
                                                        uint64_t* __restrict__ kout, int32_t* __restrict__ vout,
                                                        int64_t n, int shift, const uint32_t* __restrict__ hist,
                                                        const uint32_t* __restrict__ part, int nblk) {
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t run[256];
+    static_assert(kSortTile == 4 * 64 * kSortIPT, "four wavefronts of kSortIPT keys per lane");
+    __shared__ uint32_t wcnt[4][256];  // per-wave digit counts, then each wave's start in the tile
     __shared__ int64_t gbase[256];
     __shared__ uint32_t scan_lds[4];
     __shared__ uint64_t skeys[kSortTile];
@@ -162,6 +168,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restric
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int b = blockIdx.x;
     const int64_t base = (int64_t)b * kSortTile;
+    const int64_t wbase = base + (int64_t)w * 64 * kSortIPT + lane;
     const uint64_t lt = lanemask_lt();
 
     uint64_t key[kSortIPT];
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restric
     uint32_t rk[kSortIPT];
 #pragma unroll
     for (int r = 0; r < kSortIPT; ++r) {
-        int64_t idx = base + r * 256 + tid;
+        const int64_t idx = wbase + r * 64;
         if (idx < n) {
             key[r] = kin[idx];
             val[r] = vin[idx];
@@ -180,12 +187,12 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restric
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) wcnt[i][tid] = 0;
-    run[tid] = 0;
     __syncthreads();
 
+    uint32_t* mine = wcnt[w];
 #pragma unroll
     for (int r = 0; r < kSortIPT; ++r) {
-        const bool valid = base + r * 256 + tid < n;
+        const bool valid = wbase + r * 64 < n;
         const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -194,36 +201,30 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restric
             const uint64_t bb = __ballot(bs);
             peers &= bs ? bb : ~bb;
         }
-        const uint32_t lr = (uint32_t)__popcll(peers & lt);
-        if (valid && lr == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + lr;
-            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
-            rk[r] = pos;
-        }
-        __syncthreads();
-        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        wcnt[0][tid] = 0;
-        wcnt[1][tid] = 0;
-        wcnt[2][tid] = 0;
-        wcnt[3][tid] = 0;
-        __syncthreads();
+        const uint32_t before = mine[d];
+        rk[r] = before + (uint32_t)__popcll(peers & lt);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // every lane's read before the bump
+        if (valid && (peers & lt) == 0) mine[d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // tile offsets per digit and global bases
-    uint32_t toff = block256_excl_scan(run[tid], scan_lds, nullptr);
+    __syncthreads();
+    // digit tid: the tile's run starts at toff; wave ww's keys of the digit follow those of waves < ww
     {
+        const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+        const uint32_t toff = block256_excl_scan(c0 + c1 + c2 + c3, scan_lds, nullptr);
+        wcnt[0][tid] = toff;
+        wcnt[1][tid] = toff + c0;
+        wcnt[2][tid] = toff + c0 + c1;
+        wcnt[3][tid] = toff + c0 + c1 + c2;
         const int64_t hidx = (int64_t)tid * nblk + b;
-        const int64_t g = (int64_t)hist[hidx] + (int64_t)part[hidx / kScanChunk];
-        gbase[tid] = g - (int64_t)toff;
+        gbase[tid] = (int64_t)hist[hidx] + (int64_t)part[hidx / kScanChunk] - (int64_t)toff;
     }
-    run[tid] = toff;  // reuse as tile offset table
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortIPT; ++r) {
-        if (base + r * 256 + tid < n) {
+        if (wbase + r * 64 < n) {
             const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
-            const uint32_t l = run[d] + rk[r];
+            const uint32_t l = mine[d] + rk[r];
             skeys[l] = key[r];
             svals[l] = val[r];
         }
@@ -237,7 +238,6 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restric
         kout[o] = k;
         vout[o] = svals[l];
     }
-    (void)lane;
 }
 
 hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s) {
