@@ -79,7 +79,7 @@ def zeldovich_displacement(n: int, box_size: float, seed: int, index: float = -2
     kmag = np.sqrt(k2)
     psi = []
     for K in (KX, KY, KZ):
-        comp = np.fft.irfftn(1j * (K / kmag) * amp * wk, s=(n, n, n))
+        comp = np.fft.irfftn(1j * (K / kmag) * amp * wk, s=(n, n, n), axes=(0, 1, 2))
         psi.append(comp)
     psi = np.stack(psi, axis=-1)
     rms = np.sqrt(np.mean(np.sum(psi ** 2, axis=-1)))
@@ -175,7 +175,7 @@ def clustered(n_total: int = 22_000_000, box_size: float = 250.0, n_cells: int =
     gz = gk[: G // 2 + 1].copy()
     gz[G // 2] = gk[G // 2]
     filt = gk[:, None, None] * gk[None, :, None] * gz[None, None, :]
-    sm = np.fft.irfftn(np.fft.rfftn(rho) * filt, s=(G, G, G))
+    sm = np.fft.irfftn(np.fft.rfftn(rho) * filt, s=(G, G, G), axes=(0, 1, 2))
     med = np.median(sm)
     low = sm < med
     out = []

@@ -58,6 +58,38 @@ def test_deposit_solve_interp(jtp, orc, precision, interlace):
     assert np.array_equal(ctx.stage(jtp.STAGE_ACC_PM), acc)
 
 
+@pytest.mark.parametrize("precision,interlace", [(0, 1), (1, 1), (0, 0)])
+def test_pm_chain_smallest_mesh(jtp, orc, precision, interlace):
+    """G = 16, the smallest mesh: 8 tiles, so every tile's shell and stencil halo wraps the
+    periodic box.  Deposit, potential and interpolation against the oracle's chain."""
+    print(f"case: PM chain at G=16 precision={precision} interlace={interlace}", flush=True)
+    G, L = 16, 100.0
+    rng = np.random.default_rng(16)
+    pos = rng.integers(0, 2 ** 32, (1200, 3), dtype=np.uint64).astype(np.uint32)
+    pos[:400] = pos[0] + rng.integers(0, 2 ** 22, (400, 3), dtype=np.uint64).astype(np.uint32)  # a dense cell
+    tiles = np.bincount((orc.morton(pos) >> np.uint64(63 - 12 + 9)).astype(np.int64), minlength=8)
+    assert tiles.max() > 256 and ((tiles > 0) & (tiles <= 256)).any()  # heavy and light tiles
+    sp = rng.choice(np.array([1, -1], np.int8), len(pos))
+    p = jtp.default_params(L, G)
+    p.r_s = 0.5 * L / G
+    p.r_cut = 4.5 * p.r_s
+    p.precision, p.tree_enabled, p.interlace = precision, 0, interlace
+    ctx = jtp.ForceContext(p, 1.0, 19.0)
+    acc = ctx.compute_forces(pos, sp)
+    grids = [(pos, jtp.STAGE_RHO, jtp.STAGE_PHI)]
+    if interlace:
+        grids.append((_half_cell(pos, G), jtp.STAGE_RHO2, jtp.STAGE_PHI2))
+    acc_ref = np.zeros((len(sp), 3))
+    for p_in, st_rho, st_phi in grids:
+        rho_ref = orc.deposit(p_in, sp, 1.0, 19.0, G)
+        tol = 1e-12 if precision == 1 else 2e-6
+        assert np.max(np.abs(ctx.stage(st_rho).astype(np.float64) - rho_ref)) <= tol * np.max(np.abs(rho_ref))
+        phi_ref = orc.poisson(rho_ref, G, L, 1.0, p.r_s, 0, 1, match=0)
+        acc_ref += orc.interp(p_in, sp, G, L, phi_ref, four=True) / len(grids)
+    tol = 1e-9 if precision == 1 else 1e-4
+    assert np.max(np.abs(acc - acc_ref)) <= tol * np.max(np.abs(acc_ref))
+
+
 @pytest.mark.parametrize("four,interlace", [(1, 1), (1, 0), (0, 1), (0, 0)])
 def test_interp_tile_classes(jtp, orc, four, interlace):
     """The FP32 interpolation works per 8^3-cell tile: empty tiles, light tiles (<= 256 particles),
