@@ -593,6 +593,8 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cpl
                                                    const Cplx<R>* __restrict__ tw) {
 #pragma clang fp contract(off)
     __shared__ Cplx<R> buf[1 << LP];
+    __shared__ Cplx<R> stw[1 << LG];  // the twiddles, staged once (ordered by the barrier after the load)
+    for (int i = threadIdx.x; i < (1 << LG); i += 256) stw[i] = tw[i];
     constexpr int G = 1 << LG, H = G / 2 + 1;
     constexpr int npair = (1 << LP) / G;                    // complex lines per block
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;  // first real line
@@ -605,7 +607,7 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cpl
         if (line & 1) buf[(line >> 1) * G + x].y = v; else buf[(line >> 1) * G + x].x = v;
     }
     __syncthreads();
-    lds_fft<R, LG, LP, false>(buf, tw, false);
+    lds_fft<R, LG, LP, false>(buf, stw, false);
     Cplx<R>* dst = out + line0 * H;
     for (int e = threadIdx.x; e < rows * H; e += 256) {
         const int row = e / H, k = e - row * H;
@@ -627,6 +629,8 @@ template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ in, R* __restrict__ out,
                                                    const Cplx<R>* __restrict__ tw) {
     __shared__ Cplx<R> buf[1 << LP];
+    __shared__ Cplx<R> stw[1 << LG];  // the twiddles, staged once (ordered by the barrier after the load)
+    for (int i = threadIdx.x; i < (1 << LG); i += 256) stw[i] = tw[i];
     constexpr int G = 1 << LG, H = G / 2 + 1;
     constexpr int npair = (1 << LP) / G;
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;
@@ -655,7 +659,7 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ i
         buf[p * G + k] = {a.x - b.y, a.y + b.x};
     }
     __syncthreads();
-    lds_fft<R, LG, LP, false>(buf, tw, true);
+    lds_fft<R, LG, LP, false>(buf, stw, true);
     R* dst = out + line0 * G;
     for (int e = threadIdx.x; e < rows * G; e += 256) {
         const int line = e >> LG, x = e & (G - 1);
@@ -678,6 +682,8 @@ __device__ inline int fft_logical_block(int nblocks) {
 template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw, int inv) {
     __shared__ Cplx<R> buf[1 << LP];
+    __shared__ Cplx<R> stw[1 << LG];  // the twiddles, staged once (ordered by the barrier after the load)
+    for (int i = threadIdx.x; i < (1 << LG); i += 256) stw[i] = tw[i];
     constexpr int G = 1 << LG, H = G / 2 + 1;
     constexpr int KB = (1 << LP) / G, LKB = LP - LG;
     constexpr int nkb = (H + KB - 1) / KB;
@@ -690,7 +696,7 @@ __global__ __launch_bounds__(256) void k_fft_y(Cplx<R>* __restrict__ spec, const
         buf[e] = (kx < H) ? base[(int64_t)y * H + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, LG, LP, true>(buf, tw, inv != 0);
+    lds_fft<R, LG, LP, true>(buf, stw, inv != 0);
     for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int y = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
@@ -706,6 +712,8 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
                                                      const R* __restrict__ gtab, R norm, int k_min, int matched) {
 #pragma clang fp contract(off)
     __shared__ Cplx<R> buf[1 << LP];
+    __shared__ Cplx<R> stw[1 << LG];  // the twiddles, staged once (ordered by the barrier after the load)
+    for (int i = threadIdx.x; i < (1 << LG); i += 256) stw[i] = tw[i];
     constexpr int G = 1 << LG, H = G / 2 + 1;
     constexpr int KB = (1 << LP) / G, LKB = LP - LG;
     constexpr int nkb = (H + KB - 1) / KB;
@@ -719,7 +727,7 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
         buf[e] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, LG, LP, true>(buf, tw, false);
+    lds_fft<R, LG, LP, true>(buf, stw, false);
     const R* T = gtab;
     const R* K2 = gtab + G;
     const R* Ks = gtab + 2 * G;  // signed k_i
@@ -747,7 +755,7 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
         buf[e] = {v.x * f, v.y * f};
     }
     __syncthreads();
-    lds_fft<R, LG, LP, true>(buf, tw, true);
+    lds_fft<R, LG, LP, true>(buf, stw, true);
     for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
@@ -759,6 +767,8 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
 template <typename R, int LG, int LP>
 __global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, const Cplx<R>* __restrict__ tw) {
     __shared__ Cplx<R> buf[1 << LP];
+    __shared__ Cplx<R> stw[1 << LG];  // the twiddles, staged once (ordered by the barrier after the load)
+    for (int i = threadIdx.x; i < (1 << LG); i += 256) stw[i] = tw[i];
     constexpr int G = 1 << LG, H = G / 2 + 1;
     constexpr int KB = (1 << LP) / G, LKB = LP - LG;
     constexpr int nkb = (H + KB - 1) / KB;
@@ -772,7 +782,7 @@ __global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, c
         buf[e] = (kx < H) ? base[z * zs + kx] : Cplx<R>{0, 0};
     }
     __syncthreads();
-    lds_fft<R, LG, LP, true>(buf, tw, true);
+    lds_fft<R, LG, LP, true>(buf, stw, true);
     for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int z = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
