@@ -495,8 +495,9 @@ __global__ void k_deposit(const uint4* __restrict__ parts, const int32_t* __rest
 // In-place mixed-radix Stockham over NL = 2^LP / G lines of length G = 2^LG held in LDS
 // (element (line l, point p) at l * G + p, or at l + p * NL when LINE_FAST).  Every size is a
 // compile-time constant (the kernels are instantiated per mesh size), so the index arithmetic is
-// shifts and masks and the stage loop unrolls.  Stages: radix 4 while 4 Ns <= G, then one radix-2
-// stage if LG is odd.  A stage of radix R with Ns = product of the earlier radices takes
+// shifts and masks and the stage loop unrolls.  Stages: radix 8 while 8 Ns <= G, then one radix-4
+// or radix-2 stage (3 LDS round trips at G = 256 and 512).  A stage of radix R with Ns = product
+// of the earlier radices takes
 // v_r = x[j + r G/R], multiplies by w^(r k) (k = j mod Ns, w = exp(-+2 pi i / (R Ns)), from the
 // table tw[m] = e^{-2 pi i m / G}), does the R-point DFT and writes y_r to
 // (j / Ns) R Ns + k + r Ns (autosorted order); reads go to registers, a barrier, then the writes.
@@ -512,79 +513,120 @@ __device__ inline Cplx<R> cmul(Cplx<R> a, Cplx<R> w) {
     return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x};
 }
 
-template <typename R, int LG, int LP, bool LINE_FAST>
-__device__ inline void lds_fft(Cplx<R>* buf, const Cplx<R>* __restrict__ tw, bool inv) {
-    // no FMA contraction: every tile size (and unroll shape) then rounds each butterfly identically
+// In-place DFT of RAD = 2, 4 or 8 points (forward e^{-2 pi i nk/RAD}; inverse the conjugate).  The
+// 8-point form splits n = n1 + 4 n2, k = 2 k1 + k2: X[2 k1] = DFT4(v[n1] + v[n1 + 4]),
+// X[2 k1 + 1] = DFT4((v[n1] - v[n1 + 4]) W8^n1).
+template <typename R>
+__device__ inline void dft4(const Cplx<R>* e, Cplx<R>* y, bool inv) {
 #pragma clang fp contract(off)
-    static_assert(LP >= 10 && LP >= LG, "at least one radix-4 butterfly per thread and one line per block");
-    constexpr int G = 1 << LG, LNL = LP - LG;
-    constexpr int P4 = 1 << (LP - 10), P2 = 1 << (LP - 9);  // butterflies per thread per stage
+    const Cplx<R> a0 = {e[0].x + e[2].x, e[0].y + e[2].y}, a1 = {e[0].x - e[2].x, e[0].y - e[2].y};
+    const Cplx<R> a2 = {e[1].x + e[3].x, e[1].y + e[3].y}, a3 = {e[1].x - e[3].x, e[1].y - e[3].y};
+    // forward: y1 = a1 - i a3, y3 = a1 + i a3 (inverse: the opposite signs)
+    const Cplx<R> ia3 = inv ? Cplx<R>{-a3.y, a3.x} : Cplx<R>{a3.y, -a3.x};  // (-+i) a3
+    y[0] = {a0.x + a2.x, a0.y + a2.y};
+    y[2] = {a0.x - a2.x, a0.y - a2.y};
+    y[1] = {a1.x + ia3.x, a1.y + ia3.y};
+    y[3] = {a1.x - ia3.x, a1.y - ia3.y};
+}
+
+template <typename R, int RAD>
+__device__ inline void dft_small(const Cplx<R>* v, Cplx<R>* y, bool inv) {
+#pragma clang fp contract(off)
+    if constexpr (RAD == 2) {
+        y[0] = {v[0].x + v[1].x, v[0].y + v[1].y};
+        y[1] = {v[0].x - v[1].x, v[0].y - v[1].y};
+    } else if constexpr (RAD == 4) {
+        dft4(v, y, inv);
+    } else {
+        const R s = (R)0.70710678118654752440;  // sqrt(1/2)
+        Cplx<R> c[4], d[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            c[i] = {v[i].x + v[i + 4].x, v[i].y + v[i + 4].y};
+            d[i] = {v[i].x - v[i + 4].x, v[i].y - v[i + 4].y};
+        }
+        const R x1 = d[1].x, y1 = d[1].y, x2 = d[2].x, y2 = d[2].y, x3 = d[3].x, y3 = d[3].y;
+        if (inv) {  // W8^-1 = (s, s), W8^-2 = i, W8^-3 = (-s, s)
+            d[1] = {s * (x1 - y1), s * (x1 + y1)};
+            d[2] = {-y2, x2};
+            d[3] = {-s * (x3 + y3), s * (x3 - y3)};
+        } else {    // W8 = (s, -s), W8^2 = -i, W8^3 = (-s, -s)
+            d[1] = {s * (x1 + y1), s * (y1 - x1)};
+            d[2] = {y2, -x2};
+            d[3] = {s * (y3 - x3), -s * (x3 + y3)};
+        }
+        Cplx<R> ce[4], de[4];
+        dft4(c, ce, inv);
+        dft4(d, de, inv);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            y[2 * i] = ce[i];
+            y[2 * i + 1] = de[i];
+        }
+    }
+}
+
+// One Stockham stage of radix 2^LR after LS bits (Ns = 2^LS): butterfly j reads points
+// j + r G / RAD, multiplies point r by W_{RAD Ns}^{r k} (k = j mod Ns, tw[m] = e^{-2 pi i m / G}),
+// and writes output r to ((j / Ns) RAD Ns + k) + r Ns; all reads, a barrier, all writes.
+template <typename R, int LG, int LP, bool LINE_FAST, int LR, int LS>
+__device__ inline void fft_stage(Cplx<R>* buf, const Cplx<R>* tw, bool inv) {
+#pragma clang fp contract(off)
+    constexpr int RAD = 1 << LR, G = 1 << LG, LNL = LP - LG, Q = G / RAD, LQ = LG - LR, Ns = 1 << LS;
+    constexpr int NB = (1 << LP) / RAD / 256;  // butterflies per thread
+    static_assert(NB >= 1, "a workgroup holds at least 256 butterflies");
     const int tid = threadIdx.x;
+    Cplx<R> y[NB][RAD];
+    int ll[NB], oo[NB];
 #pragma unroll
-    for (int ls = 0; ls + 2 <= LG; ls += 2) {
-        const int Ns = 1 << ls;
-        constexpr int Q = G / 4, LQ = LG - 2;
-        Cplx<R> y[P4][4];
-        int ll[P4], oo[P4];
+    for (int u = 0; u < NB; ++u) {
+        const int q = tid + u * 256;
+        const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LQ);
+        const int j = LINE_FAST ? (q >> LNL) : (q & (Q - 1));
+        const int k = j & (Ns - 1);
+        Cplx<R> v[RAD];
 #pragma unroll
-        for (int u = 0; u < P4; ++u) {  // 2^LP / 4 radix-4 butterflies per stage
-            const int q = tid + u * 256;
-            const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LQ);
-            const int j = LINE_FAST ? (q >> LNL) : (q & (Q - 1));
-            const int k = j & (Ns - 1);
-            Cplx<R> v[4];
+        for (int r = 0; r < RAD; ++r) v[r] = buf[fft_at<LG, LP, LINE_FAST>(l, j + r * Q)];
+        if constexpr (LS > 0) {  // the first stage's twiddles are all 1
 #pragma unroll
-            for (int r = 0; r < 4; ++r) v[r] = buf[fft_at<LG, LP, LINE_FAST>(l, j + r * Q)];
-#pragma unroll
-            for (int r = 1; r < 4; ++r) {
-                Cplx<R> w = tw[(r * k) << (LG - 2 - ls)];
+            for (int r = 1; r < RAD; ++r) {
+                Cplx<R> w = tw[(r * k) << (LG - LS - LR)];
                 if (inv) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
-            const Cplx<R> a0 = {v[0].x + v[2].x, v[0].y + v[2].y}, a1 = {v[0].x - v[2].x, v[0].y - v[2].y};
-            const Cplx<R> a2 = {v[1].x + v[3].x, v[1].y + v[3].y}, a3 = {v[1].x - v[3].x, v[1].y - v[3].y};
-            // forward: y1 = a1 - i a3, y3 = a1 + i a3 (inverse: the opposite signs)
-            const Cplx<R> ia3 = inv ? Cplx<R>{-a3.y, a3.x} : Cplx<R>{a3.y, -a3.x};  // (-+i) a3
-            y[u][0] = {a0.x + a2.x, a0.y + a2.y};
-            y[u][2] = {a0.x - a2.x, a0.y - a2.y};
-            y[u][1] = {a1.x + ia3.x, a1.y + ia3.y};
-            y[u][3] = {a1.x - ia3.x, a1.y - ia3.y};
-            ll[u] = l;
-            oo[u] = ((j >> ls) << (ls + 2)) + k;
         }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < P4; ++u)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) buf[fft_at<LG, LP, LINE_FAST>(ll[u], oo[u] + r * Ns)] = y[u][r];
-        __syncthreads();
+        dft_small<R, RAD>(v, y[u], inv);
+        ll[u] = l;
+        oo[u] = ((j >> LS) << (LS + LR)) + k;
     }
-    if (LG & 1) {  // last stage radix 2, Ns = G / 2: outputs j and j + G/2, twiddle tw[j]
-        constexpr int H2 = G / 2, LH = LG - 1;
-        Cplx<R> va[P2], vb[P2];
-        int ll[P2], jj[P2];
+    __syncthreads();
 #pragma unroll
-        for (int u = 0; u < P2; ++u) {
-            const int q = tid + u * 256;
-            const int l = LINE_FAST ? (q & ((1 << LNL) - 1)) : (q >> LH);
-            const int j = LINE_FAST ? (q >> LNL) : (q & (H2 - 1));
-            const Cplx<R> a = buf[fft_at<LG, LP, LINE_FAST>(l, j)];
-            Cplx<R> w = tw[j];
-            if (inv) w.y = -w.y;
-            const Cplx<R> wb = cmul(buf[fft_at<LG, LP, LINE_FAST>(l, j + H2)], w);
-            va[u] = {a.x + wb.x, a.y + wb.y};
-            vb[u] = {a.x - wb.x, a.y - wb.y};
-            ll[u] = l;
-            jj[u] = j;
-        }
-        __syncthreads();
+    for (int u = 0; u < NB; ++u)
 #pragma unroll
-        for (int u = 0; u < P2; ++u) {
-            buf[fft_at<LG, LP, LINE_FAST>(ll[u], jj[u])] = va[u];
-            buf[fft_at<LG, LP, LINE_FAST>(ll[u], jj[u] + H2)] = vb[u];
-        }
-        __syncthreads();
+        for (int r = 0; r < RAD; ++r) buf[fft_at<LG, LP, LINE_FAST>(ll[u], oo[u] + r * Ns)] = y[u][r];
+    __syncthreads();
+}
+
+// Radix-8 stages while three or more bits remain, then one radix-4 or radix-2 stage
+// (G = 512: 8 8 8; 256: 8 8 4; 128: 8 8 2; 64: 8 8; 32: 8 4; 16: 8 2).
+template <typename R, int LG, int LP, bool LINE_FAST, int LS>
+__device__ inline void fft_stages(Cplx<R>* buf, const Cplx<R>* tw, bool inv) {
+    if constexpr (LG - LS >= 3) {
+        fft_stage<R, LG, LP, LINE_FAST, 3, LS>(buf, tw, inv);
+        fft_stages<R, LG, LP, LINE_FAST, LS + 3>(buf, tw, inv);
+    } else if constexpr (LG - LS == 2) {
+        fft_stage<R, LG, LP, LINE_FAST, 2, LS>(buf, tw, inv);
+    } else if constexpr (LG - LS == 1) {
+        fft_stage<R, LG, LP, LINE_FAST, 1, LS>(buf, tw, inv);
     }
+}
+
+// 2^LP / G lines of G points in LDS (LINE_FAST: point p of line l at l + p 2^(LP-LG), else l G + p),
+// transformed in place; no FMA contraction, so every tile size rounds each butterfly identically.
+template <typename R, int LG, int LP, bool LINE_FAST>
+__device__ inline void lds_fft(Cplx<R>* buf, const Cplx<R>* __restrict__ tw, bool inv) {
+    static_assert(LP >= 11 && LP >= LG, "one radix-8 butterfly per thread and one line per block");
+    fft_stages<R, LG, LP, LINE_FAST, 0>(buf, tw, inv);
 }
 
 // X forward (R2C): pairs of real lines (y even, y odd) packed as one complex line.
