@@ -368,9 +368,11 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
                           hipStream_t s, bool timed, double* pot = nullptr) {
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[0], s));
     JTP_HIP(&c, launch_fill_i32((int32_t*)c.dev_inter, 4, 0, s));
-    JTP_HIP(&c, launch_morton_keys(d_pos, c.keys0, c.vals0, n, s));
+    // the packed records live in the node-sum buffer, which the tree build (after the gather) overwrites
+    uint4* rec = (uint4*)c.node_sum;
+    JTP_HIP(&c, launch_morton_keys(d_pos, d_species, c.keys0, c.vals0, rec, n, s));
     JTP_HIP(&c, radix_sort_pairs(c, n, s));
-    JTP_HIP(&c, launch_gather_parts(d_pos, d_species, c.vals0, c.parts, n, s));
+    JTP_HIP(&c, launch_gather_parts(rec, c.vals0, c.parts, n, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[1], s));
     // The tree build and the PM phases only share read-only inputs (sorted keys and records), so the
     // build runs on the side stream while the PM phases run on `s`; the walk waits for both.

@@ -192,13 +192,14 @@ struct Ctx {
 // All return hipError_t from hipGetLastError after the launch(es).
 hipError_t launch_fixed_from_f64(const double* pos, uint32_t* out, int64_t n3, double L, hipStream_t s);
 hipError_t launch_fixed_from_f32(const float* pos, uint32_t* out, int64_t n3, double L, hipStream_t s);
-hipError_t launch_morton_keys(const uint32_t* pos, uint64_t* keys, int32_t* vals, int64_t n, hipStream_t s);
+// keys, identity values and the packed (x, y, z, species) records in input order
+hipError_t launch_morton_keys(const uint32_t* pos, const int8_t* species, uint64_t* keys, int32_t* vals,
+                              uint4* rec, int64_t n, hipStream_t s);
 hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, float box_half,
                                   float inv_cell, hipStream_t s);
 hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s);   // keys0/vals0 -> keys0/vals0 (sorted)
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
-hipError_t launch_gather_parts(const uint32_t* pos, const int8_t* species, const int32_t* perm,
-                               uint4* parts, int64_t n, hipStream_t s);
+hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, hipStream_t s);
 
 hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s);            // Karras + bottom-up + mono + groups
 hipError_t launch_fill_i32(int32_t* out, int64_t m, int32_t v, hipStream_t s);

@@ -44,13 +44,19 @@ hipError_t launch_fixed_from_f32(const float* pos, uint32_t* out, int64_t n3, do
 }
 
 // ------------------------------------------------------------------ Morton keys
-// Native keys: the top 21 bits of each fixed-point axis, interleaved (x in bit 0).
-__global__ void k_morton_keys(const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys,
-                              int32_t* __restrict__ vals, int64_t n) {
+// Native keys: the top 21 bits of each fixed-point axis, interleaved (x in bit 0).  Each particle
+// is also packed into one 16-byte record (x, y, z, species) in input order, so that the gather
+// after the sort reads one aligned 16-byte word per particle instead of 12 bytes that may straddle
+// two segments plus a byte in a third.
+__global__ void k_morton_keys(const uint32_t* __restrict__ pos, const int8_t* __restrict__ species,
+                              uint64_t* __restrict__ keys, int32_t* __restrict__ vals, uint4* __restrict__ rec,
+                              int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = morton_fixed(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    const uint32_t x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    keys[i] = morton_fixed(x, y, z);
     vals[i] = (int32_t)i;
+    rec[i] = make_uint4(x, y, z, (uint32_t)(int32_t)species[i]);
 }
 
 // Reference-compatible keys from f32 positions (compute_morton_all, nbody_gpu_twopass.rs:1458-1484):
@@ -68,8 +74,11 @@ __global__ void k_morton_keys_f32(const float* __restrict__ pos, uint64_t* __res
     keys[i] = spread3_21(c[0]) | (spread3_21(c[1]) << 1) | (spread3_21(c[2]) << 2);
 }
 
-hipError_t launch_morton_keys(const uint32_t* pos, uint64_t* keys, int32_t* vals, int64_t n, hipStream_t s) {
-    if (n > 0) k_morton_keys<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, keys, vals, n);
+hipError_t launch_morton_keys(const uint32_t* pos, const int8_t* species, uint64_t* keys, int32_t* vals,
+                              uint4* rec, int64_t n, hipStream_t s) {
+    if (n > 0)
+        k_morton_keys<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, species, keys, vals,
+                                                                                         rec, n);
     return hipGetLastError();
 }
 hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, float box_half, float inv_cell,
@@ -284,19 +293,16 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
 // ------------------------------------------------------------------ particle gather
 // parts[i] = (x, y, z, species) of the i-th particle in key order (reorder_by_idx_*,
 // nbody_gpu_twopass.rs:1490-1514, fused into one 16-byte record).
-__global__ void k_gather_parts(const uint32_t* __restrict__ pos, const int8_t* __restrict__ species,
-                               const int32_t* __restrict__ perm, uint4* __restrict__ parts, int64_t n) {
+__global__ void k_gather_parts(const uint4* __restrict__ rec, const int32_t* __restrict__ perm,
+                               uint4* __restrict__ parts, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t p = perm[i];
-    parts[i] = make_uint4(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], (uint32_t)(int32_t)species[p]);
+    parts[i] = rec[perm[i]];
 }
 
-hipError_t launch_gather_parts(const uint32_t* pos, const int8_t* species, const int32_t* perm, uint4* parts,
-                               int64_t n, hipStream_t s) {
+hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, hipStream_t s) {
     if (n > 0)
-        k_gather_parts<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, species, perm,
-                                                                                           parts, n);
+        k_gather_parts<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(rec, perm, parts, n);
     return hipGetLastError();
 }
 
