@@ -114,7 +114,7 @@ struct Ctx {
     // capacities (elements)
     int64_t cap_n = 0;
     // particle-side buffers
-    uint64_t *keys0 = nullptr, *keys1 = nullptr;
+    uint64_t *keys0 = nullptr, *keys1 = nullptr;  // keys1: the sort's second buffer, then the build's int2 leaf ranges
     int32_t *vals0 = nullptr, *vals1 = nullptr;
     uint32_t *hist = nullptr;        // 256 * nblocks
     uint32_t *hist_blk = nullptr;    // scan partials

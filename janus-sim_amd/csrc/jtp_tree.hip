@@ -33,7 +33,7 @@ __device__ inline int kdelta(const uint64_t* __restrict__ k, int64_t n, int64_t 
     return clz64_dev(a ^ b);
 }
 
-__global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, uint4* __restrict__ geo,
+__global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, int2* __restrict__ rng,
                          int2* __restrict__ kids, int2* __restrict__ up, int32_t* __restrict__ flags) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -66,8 +66,7 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, int64_t n, uint4* __
     const int32_t lc = (first == gamma) ? ~(int32_t)first : (int32_t)gamma;
     const int32_t rc = (gamma + 1 == last) ? ~(int32_t)last : (int32_t)(gamma + 1);
     kids[i] = make_int2(lc, rc);
-    geo[2 * i].w = (uint32_t)first;
-    geo[2 * i + 1].w = (uint32_t)last;
+    rng[i] = make_int2((int32_t)first, (int32_t)last);  // the bottom-up pass writes the full records
     // an internal child's parent and sibling, so that the bottom-up climb reads one record per level
     if (lc >= 0) up[lc] = make_int2((int32_t)i, rc);
     if (rc >= 0) up[rc] = make_int2((int32_t)i, lc);
@@ -247,7 +246,7 @@ constexpr int kBuLocalThreads = 512;  // 8 waves per workgroup, 4 workgroups (32
 
 template <int PREC>
 __global__ __launch_bounds__(kBuLocalThreads) void k_bu_local(const uint4* __restrict__ parts, int64_t nI,
-                                                              uint4* __restrict__ geo,
+                                                              uint4* __restrict__ geo, const int2* __restrict__ rng,
                                                   const int2* __restrict__ kids, const int2* __restrict__ up,
                                                   int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                   double qp, double qm, int32_t* __restrict__ gfront,
@@ -267,7 +266,8 @@ __global__ __launch_bounds__(kBuLocalThreads) void k_bu_local(const uint4* __res
             // leaf children from the topology, not from flags[] (other chunks may already be arriving there)
             const int2 kc = kids[i];
             const int32_t leaves = (kc.x < 0 ? 1 : 0) + (kc.y < 0 ? 1 : 0);
-            const int64_t first = geo[2 * i].w, last = geo[2 * i + 1].w;
+            const int2 fl = rng[i];
+            const int64_t first = fl.x, last = fl.y;
             if (first >= base && last <= lastLeaf) {
                 c = leaves;
                 if (leaves == 2) start[atomicAdd(&nstart, 1)] = (uint16_t)k;
@@ -314,7 +314,8 @@ constexpr int64_t kBuSuper = (int64_t)kBuChunk * 64;
 
 template <int PREC>
 __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts, int64_t nI, uint4* __restrict__ geo,
-                                                 const int2* __restrict__ kids, const int2* __restrict__ up,
+                                                 const int2* __restrict__ rng, const int2* __restrict__ kids,
+                                                 const int2* __restrict__ up,
                                                  int32_t* __restrict__ flags, NodeSum* __restrict__ sums, void* mono,
                                                  double qp, double qm, const int32_t* __restrict__ starts,
                                                  int32_t* __restrict__ next, int32_t* __restrict__ ctrl) {
@@ -323,7 +324,8 @@ __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts
     for (int t = threadIdx.x; t < ns; t += blockDim.x) {
         int32_t p = starts[t];
         if (p < lo || p > hi) continue;  // a node's index lies in its own range
-        if ((int64_t)geo[2 * p].w < lo || (int64_t)geo[2 * p + 1].w > hi) {
+        const int2 fl = rng[p];
+        if (fl.x < lo || fl.y > hi) {
             next[atomicAdd(&ctrl[1], 1)] = p;  // ready, but it crosses the superchunk edge
             continue;
         }
@@ -336,8 +338,7 @@ __global__ __launch_bounds__(1024) void k_bu_mid(const uint4* __restrict__ parts
             const int2 u = up[p];
             const int32_t par = u.x;
             if (par < 0 || par >= nI) break;
-            const bool inside = par >= lo && par <= hi && (int64_t)geo[2 * par].w >= lo &&
-                                (int64_t)geo[2 * par + 1].w <= hi;
+            const bool inside = par >= lo && par <= hi && rng[par].x >= lo && rng[par].y <= hi;
             if (inside) {
                 wg_release();
                 if (atomicAdd(&flags[par], 1) != 1) break;
@@ -494,7 +495,8 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
     if (n >= 2) {
         const int64_t nI = n - 1;
         const unsigned nbI = (unsigned)((nI + kBlock - 1) / kBlock);
-        k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, c.node_geo, c.node_kids, c.up, c.flags);
+        int2* rng = (int2*)c.keys1;  // leaf ranges; the sort's second key buffer is free until the next sort
+        k_karras<<<dim3(nbI), dim3(kBlock), 0, s>>>(c.keys0, n, rng, c.node_kids, c.up, c.flags);
         int32_t* ctrl = c.dev_counters + kCtrRounds;
         hipError_t e = launch_fill_i32(ctrl, kMaxRounds + 2, 0, s);
         if (e != hipSuccess) return e;
@@ -503,17 +505,17 @@ hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s) {
         const double qp = c.p.g_constant * c.m_plus, qm = c.p.g_constant * c.m_minus;
         NodeSum* sums = (NodeSum*)c.node_sum;
         if (c.p.precision == JTP_FP64) {
-            k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(kBuLocalThreads), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
+            k_bu_local<JTP_FP64><<<dim3(nbLocal), dim3(kBuLocalThreads), 0, s>>>(c.parts, nI, c.node_geo, rng, c.node_kids, c.up,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
-            k_bu_mid<JTP_FP64><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
+            k_bu_mid<JTP_FP64><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, rng, c.node_kids, c.up,
                                                                     c.flags, sums, c.node_mono, qp, qm, c.frontier0,
                                                                     c.frontier1, ctrl);
             k_bu_top<JTP_FP64><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up, c.flags,
                                                               sums, c.node_mono, qp, qm, c.frontier1, ctrl);
         } else {
-            k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(kBuLocalThreads), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
+            k_bu_local<JTP_FP32><<<dim3(nbLocal), dim3(kBuLocalThreads), 0, s>>>(c.parts, nI, c.node_geo, rng, c.node_kids, c.up,
                                                                      c.flags, sums, c.node_mono, qp, qm, c.frontier0, ctrl);
-            k_bu_mid<JTP_FP32><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up,
+            k_bu_mid<JTP_FP32><<<dim3(nbSuper), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, rng, c.node_kids, c.up,
                                                                     c.flags, sums, c.node_mono, qp, qm, c.frontier0,
                                                                     c.frontier1, ctrl);
             k_bu_top<JTP_FP32><<<dim3(1), dim3(1024), 0, s>>>(c.parts, nI, c.node_geo, c.node_kids, c.up, c.flags,
