@@ -1341,16 +1341,16 @@ static hipError_t interp_t(Ctx& c, int64_t n, hipStream_t s) {
                                                   c.logG, (float)inv, (float*)c.pm_acc);
         kl<<<dim3(ntile + 8), dim3(256), 0, s>>>(c.parts, c.cell_start, light, count, (const float*)c.phi, p2,
                                                  c.logG, (float)inv, (float*)c.pm_acc);
-        return hipGetLastError();
+    } else {  // FP64: gradient grids, then a gather per particle
+        const unsigned ng = (unsigned)((c.G3() + kBlock - 1) / kBlock);
+        auto kg = four ? k_grad<R, true> : k_grad<R, false>;
+        kg<<<dim3(ng), dim3(kBlock), 0, s>>>((const R*)c.phi, c.logG, inv, (V*)c.grad);
+        if (c.p.interlace) kg<<<dim3(ng), dim3(kBlock), 0, s>>>((const R*)c.phi2, c.logG, inv, (V*)c.grad2);
+        const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+        k_interp<R><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const V*)c.grad,
+                                                      c.p.interlace ? (const V*)c.grad2 : nullptr, c.logG,
+                                                      (R*)c.pm_acc);
     }
-    const unsigned ng = (unsigned)((c.G3() + kBlock - 1) / kBlock);
-    auto kg = four ? k_grad<R, true> : k_grad<R, false>;
-    kg<<<dim3(ng), dim3(kBlock), 0, s>>>((const R*)c.phi, c.logG, inv, (V*)c.grad);
-    if (c.p.interlace) kg<<<dim3(ng), dim3(kBlock), 0, s>>>((const R*)c.phi2, c.logG, inv, (V*)c.grad2);
-    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
-    k_interp<R><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, n, (const V*)c.grad,
-                                                  c.p.interlace ? (const V*)c.grad2 : nullptr, c.logG,
-                                                  (R*)c.pm_acc);
     return hipGetLastError();
 }
 
