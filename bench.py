@@ -147,7 +147,8 @@ def accuracy(w, name, d_pos, d_sp, pos_fixed):
 
 def run_ensemble(args):
     """Uneven ensemble of 48 boxes (synth.ensemble_c3): total particle-force-updates/s of the FP32 form
-    with the boxes run sequentially on one stream, concurrently over 4 streams, and as one replayed graph."""
+    with the boxes run sequentially on one stream, concurrently over --ensemble-streams streams (default
+    4), and as one replayed graph."""
     t0 = time.perf_counter()
     ws = synth.ensemble_c3()
     log(f"ensemble generated in {time.perf_counter() - t0:.1f} s ({len(ws)} boxes, N={sum(w.n for w in ws)})")
@@ -158,7 +159,7 @@ def run_ensemble(args):
         for k, v in OVERRIDES.items():
             setattr(p, k, v)
         params.append(p)
-    ens = j.Ensemble(params, [w.m_plus for w in ws], [w.m_minus_abs for w in ws], n_streams=4)
+    ens = j.Ensemble(params, [w.m_plus for w in ws], [w.m_minus_abs for w in ws], n_streams=args.ensemble_streams)
     dpos = [j.DeviceArray.from_host(j.to_fixed(w.pos, w.box_size)) for w in ws]
     dsp = [j.DeviceArray.from_host(w.species) for w in ws]
     dacc = [j.DeviceArray((w.n, 3), np.float32) for w in ws]
@@ -317,6 +318,7 @@ def main():
     ap.add_argument("--workload", default="clustered_22M",
                     choices=sorted(WORKLOADS) + ["ensemble_c3", "multistep_41M"])
     ap.add_argument("--multisteps", type=int, default=100, help="KDK steps of the multistep_41M workload")
+    ap.add_argument("--ensemble-streams", type=int, default=4, help="streams of the ensemble_c3 workload")
     ap.add_argument("--autotune", action="store_true",
                     help="multistep_41M: pick leaf and group size with janus_treepm.autotune first")
     ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
