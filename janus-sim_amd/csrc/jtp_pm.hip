@@ -1079,18 +1079,18 @@ __device__ inline float lds_stencil(const float* p, int b, int st, float inv) {
 // phi at nodes o - 2 .. o + NP - 3 per axis (periodic), staged row by row: each x-row is read as the
 // four aligned float4 groups at x = o - 4 .. o + 11 (o is a multiple of 8 and G of 4, so no group
 // straddles the periodic seam), four lanes per row, and the NP wanted values of a row are written to
-// LDS (x stride 1, y stride NP, z stride NP^2).  Item j = threadIdx.x + 256 k is group j & 3 of row
+// LDS (x stride 1, y stride NP, z stride NP^2).  Item j = threadIdx.x + NT k is group j & 3 of row
 // j >> 2; every load is issued before the first is used, so a thread waits for one round trip.
-template <int NP>
-constexpr int stage_per_thread() { return (NP * NP * 4 + 255) / 256; }
+template <int NP, int NT>
+constexpr int stage_per_thread() { return (NP * NP * 4 + NT - 1) / NT; }
 
-template <int NP>
+template <int NP, int NT>
 __device__ inline void stage_load(const float* __restrict__ phi, int32_t ox, int32_t oy, int32_t oz, int logG,
-                                  float4 (&v)[stage_per_thread<NP>()]) {
+                                  float4 (&v)[stage_per_thread<NP, NT>()]) {
     const int32_t msk = (1 << logG) - 1;
 #pragma unroll
-    for (int k = 0; k < stage_per_thread<NP>(); ++k) {
-        const int j = threadIdx.x + 256 * k;
+    for (int k = 0; k < stage_per_thread<NP, NT>(); ++k) {
+        const int j = threadIdx.x + NT * k;
         if (j < NP * NP * 4) {
             const int row = j >> 2, ly = row % NP - 2, lz = row / NP - 2;
             const int64_t rb = (((int64_t)((oz + lz) & msk)) << logG | ((oy + ly) & msk)) << logG;
@@ -1099,11 +1099,11 @@ __device__ inline void stage_load(const float* __restrict__ phi, int32_t ox, int
     }
 }
 
-template <int NP>
-__device__ inline void stage_store(const float4 (&v)[stage_per_thread<NP>()], float* p) {
+template <int NP, int NT>
+__device__ inline void stage_store(const float4 (&v)[stage_per_thread<NP, NT>()], float* p) {
 #pragma unroll
-    for (int k = 0; k < stage_per_thread<NP>(); ++k) {
-        const int j = threadIdx.x + 256 * k;
+    for (int k = 0; k < stage_per_thread<NP, NT>(); ++k) {
+        const int j = threadIdx.x + NT * k;
         if (j < NP * NP * 4) {
             float* row = p + (j >> 2) * NP + 2;  // x = 0 of the row
             const int x0 = 4 * (j & 3) - 4;        // x of v[k].x
@@ -1116,9 +1116,9 @@ __device__ inline void stage_store(const float4 (&v)[stage_per_thread<NP>()], fl
 }
 
 // the stencil gradient at nodes o .. o + NG - 1 per axis from the staged potential p -> g
-template <bool FOUR, int NP, int NG>
+template <bool FOUR, int NP, int NG, int NT>
 __device__ inline void stage_grad(const float* p, float inv, float4* g) {
-    for (int i = threadIdx.x; i < NG * NG * NG; i += 256) {
+    for (int i = threadIdx.x; i < NG * NG * NG; i += NT) {
         const int nx = i % NG, ny = (i / NG) % NG, nz = i / (NG * NG);
         const int b = ((nz + 2) * NP + ny + 2) * NP + nx + 2;
         float4 v;
@@ -1217,8 +1217,10 @@ __device__ inline int cell2(uint32_t u, int sh, int32_t o, uint32_t msk) {
     return (int)min(((u >> sh) - (uint32_t)o) & msk, (uint32_t)kTile);
 }
 
+constexpr int kHeavyThreads = 512;  // 8 waves per workgroup: 24 waves per CU at 3 workgroups
+
 template <bool FOUR, bool BOTH>
-__global__ __launch_bounds__(256) void k_interp_heavy(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+__global__ __launch_bounds__(kHeavyThreads) void k_interp_heavy(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
                                                       const int2* __restrict__ list, const int32_t* __restrict__ count,
                                                       const float* __restrict__ phi, const float* __restrict__ phi2,
                                                       int logG, float inv, float* __restrict__ acc) {
@@ -1234,19 +1236,19 @@ __global__ __launch_bounds__(256) void k_interp_heavy(const uint4* __restrict__ 
     const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
                   oz = (int32_t)compact3_21(base >> 2);
     {
-        float4 v1[stage_per_thread<kIp1>()], v2[stage_per_thread<kIp2>()];
-        stage_load<kIp1>(phi, ox, oy, oz, logG, v1);
-        if (BOTH) stage_load<kIp2>(phi2, ox, oy, oz, logG, v2);
-        stage_store<kIp1>(v1, p1);
-        if (BOTH) stage_store<kIp2>(v2, p2);
+        float4 v1[stage_per_thread<kIp1, kHeavyThreads>()], v2[stage_per_thread<kIp2, kHeavyThreads>()];
+        stage_load<kIp1, kHeavyThreads>(phi, ox, oy, oz, logG, v1);
+        if (BOTH) stage_load<kIp2, kHeavyThreads>(phi2, ox, oy, oz, logG, v2);
+        stage_store<kIp1, kHeavyThreads>(v1, p1);
+        if (BOTH) stage_store<kIp2, kHeavyThreads>(v2, p2);
     }
     __syncthreads();
-    stage_grad<FOUR, kIp1, kIg1>(p1, inv, g1);
-    if (BOTH) stage_grad<FOUR, kIp2, kIg2>(p2, inv, g2);
+    stage_grad<FOUR, kIp1, kIg1, kHeavyThreads>(p1, inv, g1);
+    if (BOTH) stage_grad<FOUR, kIp2, kIg2, kHeavyThreads>(p2, inv, g2);
     __syncthreads();
     const int sh = 32 - logG;
     const uint32_t msk = (1u << logG) - 1u, half = 1u << (31 - logG);
-    for (int32_t i = pbeg + (int32_t)threadIdx.x; i < pend; i += 256) {
+    for (int32_t i = pbeg + (int32_t)threadIdx.x; i < pend; i += kHeavyThreads) {
         const uint4 r = parts[i];
         float a[3];
         tile_cic<kIg1>(g1, cell1(r.x, sh, ox), cell1(r.y, sh, oy), cell1(r.z, sh, oz), frac_of<float>(r.x, logG),
@@ -1288,11 +1290,11 @@ __global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ 
     uint4 r = make_uint4(0u, 0u, 0u, 0u);
     if (have) r = parts[i];
     {
-        float4 v1[stage_per_thread<kIp1>()], v2[stage_per_thread<kIp2>()];
-        stage_load<kIp1>(phi, ox, oy, oz, logG, v1);
-        if (BOTH) stage_load<kIp2>(phi2, ox, oy, oz, logG, v2);
-        stage_store<kIp1>(v1, p1);
-        if (BOTH) stage_store<kIp2>(v2, p2);
+        float4 v1[stage_per_thread<kIp1, 256>()], v2[stage_per_thread<kIp2, 256>()];
+        stage_load<kIp1, 256>(phi, ox, oy, oz, logG, v1);
+        if (BOTH) stage_load<kIp2, 256>(phi2, ox, oy, oz, logG, v2);
+        stage_store<kIp1, 256>(v1, p1);
+        if (BOTH) stage_store<kIp2, 256>(v2, p2);
     }
     __syncthreads();
     if (!have) return;  // no barrier follows
@@ -1337,7 +1339,7 @@ static hipError_t interp_t(Ctx& c, int64_t n, hipStream_t s) {
                        : (c.p.interlace ? k_interp_heavy<false, true> : k_interp_heavy<false, false>);
         auto kl = four ? (c.p.interlace ? k_interp_light<true, true> : k_interp_light<true, false>)
                        : (c.p.interlace ? k_interp_light<false, true> : k_interp_light<false, false>);
-        kh<<<dim3(nheavy + 8), dim3(256), 0, s>>>(c.parts, c.cell_start, heavy, count + 1, (const float*)c.phi, p2,
+        kh<<<dim3(nheavy + 8), dim3(kHeavyThreads), 0, s>>>(c.parts, c.cell_start, heavy, count + 1, (const float*)c.phi, p2,
                                                   c.logG, (float)inv, (float*)c.pm_acc);
         kl<<<dim3(ntile + 8), dim3(256), 0, s>>>(c.parts, c.cell_start, light, count, (const float*)c.phi, p2,
                                                  c.logG, (float)inv, (float*)c.pm_acc);
