@@ -776,6 +776,11 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
     const R* D = gtab + 3 * G;   // transfer function of the gradient stencil along one axis
     const int ky = y;
     const int aky = ky <= G / 2 ? ky : G - ky;
+    const R Ty = T[ky], K2y = K2[ky], Ksy = Ks[ky], Dy = D[ky];  // the block's row: one ky
+    // with KB <= 256 every element of a thread has the same kx (256 is a multiple of KB)
+    const int kx_t = kx0 + (threadIdx.x & (KB - 1));
+    const int kx_c = kx_t < H ? kx_t : 0;
+    const R Tx_t = T[kx_c], K2x_t = K2[kx_c], Ksx_t = Ks[kx_c], Dx_t = D[kx_c];
     for (int e = threadIdx.x; e < (1 << LP); e += 256) {
         const int kz = e >> LKB, jb = e & (KB - 1);
         const int kx = kx0 + jb;
@@ -784,13 +789,19 @@ __global__ __launch_bounds__(256) void k_fft_z_green(Cplx<R>* __restrict__ spec,
         const int kmax = max(akx, max(aky, akz));
         R f = 0;
         if (kx < H && !(akx == 0 && aky == 0 && akz == 0) && kmax >= k_min) {
-            f = norm * T[kx] * T[ky] * T[kz] / (K2[kx] + K2[ky] + K2[kz]);
+            const bool same = KB <= 256;
+            const R Tx = same ? Tx_t : T[kx], K2x = same ? K2x_t : K2[kx];
+            const R k2 = K2x + K2y + K2[kz];
+            f = norm * Tx * Ty * T[kz];
             if (matched) {
                 // least-squares match of the stencil gradient D to i k: f *= (k . D) / |D|^2, with the
-                // denominator floored at |k|^2 / 4 where D vanishes (Nyquist planes)
-                const R dd = D[kx] * D[kx] + D[ky] * D[ky] + D[kz] * D[kz];
-                const R fl = (R)0.25 * (K2[kx] + K2[ky] + K2[kz]);
-                f *= (Ks[kx] * D[kx] + Ks[ky] * D[ky] + Ks[kz] * D[kz]) / (dd > fl ? dd : fl);
+                // denominator floored at |k|^2 / 4 where D vanishes (Nyquist planes); one division
+                const R Ksx = same ? Ksx_t : Ks[kx], Dx = same ? Dx_t : D[kx];
+                const R dd = Dx * Dx + Dy * Dy + D[kz] * D[kz];
+                const R fl = (R)0.25 * k2;
+                f = f * (Ksx * Dx + Ksy * Dy + Ks[kz] * D[kz]) / (k2 * (dd > fl ? dd : fl));
+            } else {
+                f = f / k2;
             }
         }
         const Cplx<R> v = buf[e];
