@@ -643,10 +643,23 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const R* __restrict__ in, Cpl
     const R* src = in + line0 * G;
     // G = 16: the grid's G^2 = 256 real rows fill half of one block; the rest is zero and not stored
     constexpr int rows = 2 * npair < G * G ? 2 * npair : G * G;
-    for (int e = threadIdx.x; e < (2 << LP); e += 256) {
-        const int line = e >> LG, x = e & (G - 1);
-        const R v = line < rows ? src[e] : (R)0;
-        if (line & 1) buf[(line >> 1) * G + x].y = v; else buf[(line >> 1) * G + x].x = v;
+    // 16-byte loads: VW consecutive x of one real line (G >= 16, so a line holds whole vectors)
+    constexpr int VW = 16 / (int)sizeof(R);
+    for (int e4 = threadIdx.x; e4 < (2 << LP) / VW; e4 += 256) {
+        const int e = e4 * VW, line = e >> LG, x = e & (G - 1);
+        R v[VW];
+        if (line < rows) {
+            const uint4 w = *(const uint4*)(src + e);
+            __builtin_memcpy(v, &w, 16);
+        } else {
+#pragma unroll
+            for (int i = 0; i < VW; ++i) v[i] = (R)0;
+        }
+        Cplx<R>* b = &buf[(line >> 1) * G + x];
+#pragma unroll
+        for (int i = 0; i < VW; ++i) {
+            if (line & 1) b[i].y = v[i]; else b[i].x = v[i];
+        }
     }
     __syncthreads();
     lds_fft<R, LG, LP, false>(buf, stw, false);
@@ -678,35 +691,35 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const Cplx<R>* __restrict__ i
     const int64_t line0 = (int64_t)blockIdx.x * npair * 2;
     const Cplx<R>* src = in + line0 * H;
     constexpr int rows = 2 * npair < G * G ? 2 * npair : G * G;  // G = 16: half a block
-    for (int e = threadIdx.x; e < (1 << LP); e += 256) {
-        const int p = e >> LG, k = e & (G - 1);
-        const Cplx<R>* ra = src + (int64_t)(2 * p) * H;
-        const Cplx<R>* rb = ra + H;
-        Cplx<R> a, b;
-        if (2 * p >= rows) {
-            a = b = Cplx<R>{0, 0};
-        } else if (k <= G / 2) {
+    // each half-spectrum entry read once: it fills point k and, conjugated, point G - k
+    for (int e = threadIdx.x; e < npair * H; e += 256) {
+        const int p = e / H, k = e - p * H;
+        Cplx<R> a = {0, 0}, b = {0, 0};
+        if (2 * p < rows) {
+            const Cplx<R>* ra = src + (int64_t)(2 * p) * H;
             a = ra[k];
-            b = rb[k];
-            if (k == 0 || k == G / 2) {
-                a.y = 0;
-                b.y = 0;
-            }
-        } else {
-            a = ra[G - k];
-            b = rb[G - k];
-            a.y = -a.y;
-            b.y = -b.y;
+            b = ra[H + k];
+        }
+        if (k == 0 || k == G / 2) {
+            a.y = 0;
+            b.y = 0;
         }
         buf[p * G + k] = {a.x - b.y, a.y + b.x};
+        if (k > 0 && k < G / 2) buf[p * G + G - k] = {a.x + b.y, -a.y + b.x};  // conj A + i conj B
     }
     __syncthreads();
     lds_fft<R, LG, LP, false>(buf, stw, true);
     R* dst = out + line0 * G;
-    for (int e = threadIdx.x; e < rows * G; e += 256) {
-        const int line = e >> LG, x = e & (G - 1);
-        const Cplx<R> z = buf[(line >> 1) * G + x];
-        dst[e] = (line & 1) ? z.y : z.x;
+    constexpr int VW = 16 / (int)sizeof(R);  // 16-byte stores of VW consecutive x
+    for (int e4 = threadIdx.x; e4 < rows * G / VW; e4 += 256) {
+        const int e = e4 * VW, line = e >> LG, x = e & (G - 1);
+        const Cplx<R>* z = &buf[(line >> 1) * G + x];
+        R v[VW];
+#pragma unroll
+        for (int i = 0; i < VW; ++i) v[i] = (line & 1) ? z[i].y : z[i].x;
+        uint4 w;
+        __builtin_memcpy(&w, v, 16);
+        *(uint4*)(dst + e) = w;
     }
 }
 
