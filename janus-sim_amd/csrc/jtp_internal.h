@@ -22,7 +22,7 @@ constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
 constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by wavefronts,
 constexpr int kDenseChunk = 1024;    // one per chunk of this many particles
 constexpr int kInterpChunk = 4096;   // FP32 interpolation: particles per workgroup of a heavy tile,
-constexpr int kLightMax = 256;       // tiles with at most this many are light (one particle per thread)
+constexpr int kLightMax = 1024;      // tiles with at most this many are light (four particles per thread)
 // int2 entries of Ctx::interp_list for G^3 cells and up to n particles: the light-tile list, then
 // sum over tiles of ceil(np / kInterpChunk) <= n / kInterpChunk + n / (kLightMax + 1) + 1 chunks
 inline int64_t interp_list_size(int64_t G3, int64_t n) {

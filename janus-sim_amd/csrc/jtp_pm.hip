@@ -1203,7 +1203,7 @@ __device__ inline void tile_cic_phi(const float* p, int ix, int iy, int iz, floa
     a[2] = az;
 }
 
-// Tiles are classed by particle count: light tiles (1 .. kLightMax particles, one per thread) are
+// Tiles are classed by particle count: light tiles (1 .. kLightMax particles, kLightMax / 256 per thread) are
 // listed for k_interp_light; every chunk of kInterpChunk particles of a heavier tile (a halo core
 // holds 1e4-1e5) is listed, as (tile, first particle), for a k_interp_heavy workgroup.  The list
 // order varies between runs; each particle is still computed by one thread with the same
@@ -1309,10 +1309,12 @@ __global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ 
     const int32_t pbeg = cs[base], np = cs[base + 512] - pbeg;  // 1 .. kLightMax
     const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
                   oz = (int32_t)compact3_21(base >> 2);
-    const int32_t i = pbeg + (int32_t)threadIdx.x;
-    const bool have = (int32_t)threadIdx.x < np;
-    uint4 r = make_uint4(0u, 0u, 0u, 0u);
-    if (have) r = parts[i];
+    constexpr int kPer = kLightMax / 256;  // particles per thread
+    static_assert(kLightMax % 256 == 0, "light tiles hold whole multiples of the block");
+    uint4 rs[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+        if ((int32_t)threadIdx.x + 256 * u < np) rs[u] = parts[pbeg + (int32_t)threadIdx.x + 256 * u];
     {
         float4 v1[stage_per_thread<kIp1, 256>()], v2[stage_per_thread<kIp2, 256>()];
         stage_load<kIp1, 256>(phi, ox, oy, oz, logG, v1);
@@ -1320,26 +1322,33 @@ __global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ 
         stage_store<kIp1, 256>(v1, p1);
         if (BOTH) stage_store<kIp2, 256>(v2, p2);
     }
-    __syncthreads();
-    if (!have) return;  // no barrier follows
+    __syncthreads();  // no barrier follows
     const int sh = 32 - logG;
     const uint32_t msk = (1u << logG) - 1u, half = 1u << (31 - logG);
-    float a[3];
-    tile_cic_phi<FOUR, kIp1>(p1, cell1(r.x, sh, ox), cell1(r.y, sh, oy), cell1(r.z, sh, oz),
-                             frac_of<float>(r.x, logG), frac_of<float>(r.y, logG), frac_of<float>(r.z, logG), inv, a);
-    float sg = (int32_t)r.w > 0 ? -1.0f : 1.0f;
-    if (BOTH) {
-        const uint32_t ux = r.x + half, uy = r.y + half, uz = r.z + half;
-        float b[3];
-        tile_cic_phi<FOUR, kIp2>(p2, cell2(ux, sh, ox, msk), cell2(uy, sh, oy, msk), cell2(uz, sh, oz, msk),
-                                 frac_of<float>(ux, logG), frac_of<float>(uy, logG), frac_of<float>(uz, logG), inv, b);
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) a[ax] += b[ax];
-        sg *= 0.5f;
+    for (int u = 0; u < kPer; ++u) {
+        if ((int32_t)threadIdx.x + 256 * u >= np) continue;
+        const int32_t i = pbeg + (int32_t)threadIdx.x + 256 * u;
+        const uint4 r = rs[u];
+        float a[3];
+        tile_cic_phi<FOUR, kIp1>(p1, cell1(r.x, sh, ox), cell1(r.y, sh, oy), cell1(r.z, sh, oz),
+                                 frac_of<float>(r.x, logG), frac_of<float>(r.y, logG), frac_of<float>(r.z, logG),
+                                 inv, a);
+        float sg = (int32_t)r.w > 0 ? -1.0f : 1.0f;
+        if (BOTH) {
+            const uint32_t ux = r.x + half, uy = r.y + half, uz = r.z + half;
+            float b[3];
+            tile_cic_phi<FOUR, kIp2>(p2, cell2(ux, sh, ox, msk), cell2(uy, sh, oy, msk), cell2(uz, sh, oz, msk),
+                                     frac_of<float>(ux, logG), frac_of<float>(uy, logG), frac_of<float>(uz, logG),
+                                     inv, b);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) a[ax] += b[ax];
+            sg *= 0.5f;
+        }
+        acc[3 * (int64_t)i] = sg * a[0];
+        acc[3 * (int64_t)i + 1] = sg * a[1];
+        acc[3 * (int64_t)i + 2] = sg * a[2];
     }
-    acc[3 * (int64_t)i] = sg * a[0];
-    acc[3 * (int64_t)i + 1] = sg * a[1];
-    acc[3 * (int64_t)i + 2] = sg * a[2];
 }
 
 template <typename R>

@@ -65,10 +65,10 @@ def test_pm_chain_smallest_mesh(jtp, orc, precision, interlace):
     print(f"case: PM chain at G=16 precision={precision} interlace={interlace}", flush=True)
     G, L = 16, 100.0
     rng = np.random.default_rng(16)
-    pos = rng.integers(0, 2 ** 32, (1200, 3), dtype=np.uint64).astype(np.uint32)
-    pos[:400] = pos[0] + rng.integers(0, 2 ** 22, (400, 3), dtype=np.uint64).astype(np.uint32)  # a dense cell
+    pos = rng.integers(0, 2 ** 32, (2000, 3), dtype=np.uint64).astype(np.uint32)
+    pos[:1100] = pos[0] + rng.integers(0, 2 ** 22, (1100, 3), dtype=np.uint64).astype(np.uint32)  # a dense cell
     tiles = np.bincount((orc.morton(pos) >> np.uint64(63 - 12 + 9)).astype(np.int64), minlength=8)
-    assert tiles.max() > 256 and ((tiles > 0) & (tiles <= 256)).any()  # heavy and light tiles
+    assert tiles.max() > 1024 and ((tiles > 0) & (tiles <= 1024)).any()  # heavy and light tiles
     sp = rng.choice(np.array([1, -1], np.int8), len(pos))
     p = jtp.default_params(L, G)
     p.r_s = 0.5 * L / G
@@ -92,7 +92,7 @@ def test_pm_chain_smallest_mesh(jtp, orc, precision, interlace):
 
 @pytest.mark.parametrize("four,interlace", [(1, 1), (1, 0), (0, 1), (0, 0)])
 def test_interp_tile_classes(jtp, orc, four, interlace):
-    """The FP32 interpolation works per 8^3-cell tile: empty tiles, light tiles (<= 256 particles),
+    """The FP32 interpolation works per 8^3-cell tile: empty tiles, light tiles (<= 1024 particles),
     heavy tiles of one chunk and heavy tiles cut into several 4096-particle chunks must all agree
     with the oracle's interpolation (pm_grid.rs:275-359) of the same potential, the GPU's own, so
     that only the interpolation's FP32 rounding (~1e-7 of max |a| here) is left."""
@@ -103,7 +103,8 @@ def test_interp_tile_classes(jtp, orc, four, interlace):
     sp = w.species[keep]
     G = w.n_cells
     c = np.bincount((orc.morton(pos) >> np.uint64(63 - 3 * 6 + 9)).astype(np.int64), minlength=G ** 3 // 512)
-    assert (c == 0).sum() >= 8 and ((c > 0) & (c <= 256)).any() and ((c > 256) & (c <= 1024)).any()
+    assert (c == 0).sum() >= 8 and ((c > 0) & (c <= 256)).any() and ((c > 512) & (c <= 1024)).any()
+    assert ((c > 1024) & (c <= 4096)).any()  # heavy, one chunk
     assert (c > 2 * 4096).any()  # three chunks or more
     ctx = _ctx(jtp, w, precision=0, tree_enabled=0, interlace=interlace,
                gradient=jtp.GRAD_4PT if four else jtp.GRAD_2PT)
