@@ -22,11 +22,11 @@ from janus_treepm import synth  # noqa: E402
 from oracle import oracle_lib  # noqa: E402
 
 WORKLOADS = {
-    # configs[1]/[2]: clustered N = 2.2e7, L = 250, PM 256^3
+    # the headline box: clustered N = 2.2e7, L = 250, PM 256^3
     "clustered_22M": lambda: synth.clustered(n_total=22_000_000, box_size=250.0, n_cells=256, n_halos=2000, seed=7),
     # the same recipe at 1/64 volume (same density, cell size, r_s, eps): quick accuracy studies
     "clustered_344k": lambda: synth.clustered(n_total=343_750, box_size=62.5, n_cells=64, n_halos=31, seed=99),
-    # lattice + Zel'dovich, 2 x 64^3, L = 250, PM 64^3
+    # the small reference-CPU-sized box: lattice + Zel'dovich, 2 x 64^3, L = 250, PM 64^3
     "lattice_64": lambda: synth.lattice_zeldovich(n_side=64, box_size=250.0, n_cells=64, seed=42),
 }
 
