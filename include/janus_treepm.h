@@ -116,6 +116,27 @@ typedef struct jtp_ctx jtp_ctx;
  * G = 1, leaf 4, group 32, FP32, erfc+Gaussian split, 4-point gradient, deconvolution on,
  * interlacing on, theta_cut = 0.66 theta. */
 void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells);
+/* ---- YAML drop-in: the reference's JanusConfig fields that set the force (config.rs:21-40,94-102)
+ * plus two optional split fields the reference lacks.  jtp_params_from_config fills every jtp_params
+ * field: the ones below from the config, the rest as jtp_default_params.  Without the split fields it
+ * reproduces the reference's derivation: r_cut = 2 L / n_cells (janus_optim.rs:74) and
+ * r_s = r_cut / 3 (nbody_gpu_twopass.rs:1694, treepm_force.rs:36), i.e. split_scale_cells = 2/3 and
+ * r_cut_over_r_s = 3.  Returns JTP_ERR_ARG (and leaves *p unspecified) if the result is invalid. */
+typedef struct jtp_config_fields {
+    double  box_size_mpc;      /* simulation.box_size_mpc (config.rs:23)                            */
+    int64_t n_particles;       /* simulation.n_particles (config.rs:25); only used for the default
+                                  softening                                                         */
+    double  theta;             /* simulation.theta (config.rs:36); < 0: the YAML default 0.7 (:42)   */
+    double  softening_mpc;     /* simulation.softening_mpc (config.rs:39); <= 0 (absent):
+                                  box_size_mpc / n_particles^(1/3) / 30 (config.rs:191-196)          */
+    int32_t n_cells;           /* pm_grid.n_cells (config.rs:98)                                    */
+    int32_t k_min;             /* pm_grid.k_min (config.rs:101)                                     */
+    double  split_scale_cells; /* optional pm_grid.split_scale_cells: r_s in PM cells;
+                                  <= 0 (absent): r_cut_cells / r_cut_over_r_s with r_cut_cells = 2   */
+    double  r_cut_over_r_s;    /* optional pm_grid.r_cut_over_r_s; <= 0 (absent): 3                 */
+} jtp_config_fields;
+int  jtp_params_from_config(const jtp_config_fields* f, jtp_params* p);
+
 int  jtp_create(const jtp_params* p, jtp_ctx** out);
 int  jtp_destroy(jtp_ctx* ctx);
 const char* jtp_last_error(const jtp_ctx* ctx);
@@ -229,6 +250,15 @@ int  jtp_pm_solve(jtp_ctx* ctx, const void* d_rho, void* d_phi, void* stream);
  * nbody_gpu_twopass.rs:1447-1487): 21 bits/axis of (x + box_half) * inv_cell_size. */
 int  jtp_morton_keys_f32(int64_t n, const float* d_pos, float box_half, float inv_cell_size,
                          uint64_t* d_keys, void* stream);
+
+/* ---- reference-compatible 30-bit keys with the index tie-break (compute_morton_f32,
+ * nbody_gpu_twopass.rs:302-326, the key of the reference's per-species tree build
+ * build_single_sign_tree :2338-2545): per axis c = min(sat_u32((x + box_half) * (1024 / (2 box_half))), 1023)
+ * in f32 arithmetic (sat_u32: truncation toward zero, negative and NaN -> 0), the 10-bit interleave
+ * mc (x in bit 0), key[i] = mc << 32 | i and indices[i] = i.  inv_cell_size is ignored, as in the
+ * reference kernel.  0 <= n <= 2^31 - 1 (the index is a 32-bit field). */
+int  jtp_morton30_keys_f32(int64_t n, const float* d_pos, float box_half, float inv_cell_size,
+                           uint64_t* d_keys, int32_t* d_indices, void* stream);
 
 /* ---- diagnostics */
 int  jtp_get_stats(jtp_ctx* ctx, jtp_stats* out);            /* synchronises the last call */

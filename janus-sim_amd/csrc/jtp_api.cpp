@@ -195,6 +195,17 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     return JTP_OK;
 }
 
+// The three segregation numbers from the device sums of launch_segregation (janus_mu19_production.rs:240-272).
+void segregation_metrics(const double v[9], int64_t cells, double out[3]) {
+    const double C = (double)cells, mp = v[7] / C, mm = v[8] / C;
+    const double var_diff = v[0] / C, pois = mp + mm;
+    out[0] = pois > 0 ? var_diff / pois : 0.0;
+    const double var_p = v[3] / C - (v[1] / C) * (v[1] / C), var_m = v[4] / C - (v[2] / C) * (v[2] / C);
+    const double cov = v[5] / C - (v[1] / C) * (v[2] / C);
+    out[1] = (var_p > 0 && var_m > 0) ? cov / (std::sqrt(var_p) * std::sqrt(var_m)) : 0.0;
+    out[2] = mm > 0 ? (v[6] / C - mm) / mm : 0.0;
+}
+
 static int check_params(const jtp_params* p, std::string& msg) {
     if (!p) { msg = "params is null"; return JTP_ERR_ARG; }
     if (!(p->box_size > 0) || !std::isfinite(p->box_size)) { msg = "box_size must be > 0"; return JTP_ERR_ARG; }
@@ -256,6 +267,30 @@ void jtp_default_params(jtp_params* p, double box_size, int32_t n_cells) {
     p->interlace = 1;
     p->theta_cut = 0.66 * p->theta;
     p->fft_tile = 0;
+}
+
+int jtp_params_from_config(const jtp_config_fields* f, jtp_params* p) {
+    if (!f || !p) return JTP_ERR_ARG;
+    if (!(f->box_size_mpc > 0) || !std::isfinite(f->box_size_mpc) || f->n_cells <= 0) return JTP_ERR_ARG;
+    jtp_default_params(p, f->box_size_mpc, f->n_cells);
+    const double h = f->box_size_mpc / f->n_cells;
+    // r_cut = 2 L / n_cells (janus_optim.rs:74) and r_s = r_cut / 3 (nbody_gpu_twopass.rs:1694) unless the
+    // optional split fields are present
+    const double ratio = f->r_cut_over_r_s > 0 ? f->r_cut_over_r_s : 3.0;
+    const double rs_cells = f->split_scale_cells > 0 ? f->split_scale_cells : 2.0 / ratio;
+    p->r_s = rs_cells * h;
+    p->r_cut = ratio * p->r_s;
+    p->theta = f->theta < 0 ? 0.7 : f->theta;  // default_theta (config.rs:42)
+    p->theta_cut = 0.66 * p->theta;
+    if (f->softening_mpc > 0) {
+        p->softening = f->softening_mpc;
+    } else {  // config.rs:191-196
+        if (f->n_particles <= 0) return JTP_ERR_ARG;
+        p->softening = f->box_size_mpc / std::pow((double)f->n_particles, 1.0 / 3.0) / 30.0;
+    }
+    p->k_min = f->k_min;
+    std::string msg;
+    return check_params(p, msg);
 }
 
 int jtp_create(const jtp_params* p, jtp_ctx** out) {
@@ -643,14 +678,7 @@ int jtp_segregation(jtp_ctx* h, int64_t n, const uint32_t* d_pos, const int8_t* 
     double v[9];
     JTP_HIP(&c, hipMemcpyAsync(v, res, sizeof(v), hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipStreamSynchronize(s));
-    // janus_mu19_production.rs:240-272
-    const double C = (double)cells, mp = v[7] / C, mm = v[8] / C;
-    const double var_diff = v[0] / C, pois = mp + mm;
-    out[0] = pois > 0 ? var_diff / pois : 0.0;
-    const double var_p = v[3] / C - (v[1] / C) * (v[1] / C), var_m = v[4] / C - (v[2] / C) * (v[2] / C);
-    const double cov = v[5] / C - (v[1] / C) * (v[2] / C);
-    out[1] = (var_p > 0 && var_m > 0) ? cov / (std::sqrt(var_p) * std::sqrt(var_m)) : 0.0;
-    out[2] = mm > 0 ? (v[6] / C - mm) / mm : 0.0;
+    segregation_metrics(v, cells, out);
     return JTP_OK;
 }
 
@@ -762,6 +790,15 @@ int jtp_morton_keys_f32(int64_t n, const float* d_pos, float box_half, float inv
     if (n < 0 || (n > 0 && (!d_pos || !d_keys))) return JTP_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     if (launch_morton_keys_f32(d_pos, d_keys, n, box_half, inv_cell_size, s) != hipSuccess) return JTP_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return JTP_ERR_HIP;
+    return JTP_OK;
+}
+
+int jtp_morton30_keys_f32(int64_t n, const float* d_pos, float box_half, float /*inv_cell_size*/, uint64_t* d_keys,
+                          int32_t* d_indices, void* stream) {
+    if (n < 0 || n > (int64_t)0x7fffffff || (n > 0 && (!d_pos || !d_keys || !d_indices))) return JTP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (launch_morton30_keys_f32(d_pos, d_keys, d_indices, n, box_half, s) != hipSuccess) return JTP_ERR_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) return JTP_ERR_HIP;
     return JTP_OK;
 }

@@ -197,6 +197,8 @@ hipError_t launch_morton_keys(const uint32_t* pos, const int8_t* species, uint64
                               uint4* rec, int64_t n, hipStream_t s);
 hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, float box_half,
                                   float inv_cell, hipStream_t s);
+hipError_t launch_morton30_keys_f32(const float* pos, uint64_t* keys, int32_t* idx, int64_t n, float box_half,
+                                    hipStream_t s);
 hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s);   // keys0/vals0 -> keys0/vals0 (sorted)
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, hipStream_t s);
@@ -227,6 +229,8 @@ hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* 
 constexpr int kDiagScratch = 1024 * 8 + 16;  // doubles
 hipError_t launch_segregation(int64_t n, const uint32_t* pos, const int8_t* species, int32_t n_cells,
                               int32_t* counts, double* scratch, double* out, hipStream_t s);
+// sums out[0..8] of launch_segregation over `cells` cells -> the three metrics of jtp_segregation
+void segregation_metrics(const double v[9], int64_t cells, double out[3]);
 hipError_t launch_energy_terms(Ctx& c, int64_t n, const double* kself, const double* pot_short, double* scratch,
                                double* out, hipStream_t s);
 hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s);

@@ -74,6 +74,33 @@ __global__ void k_morton_keys_f32(const float* __restrict__ pos, uint64_t* __res
     keys[i] = spread3_21(c[0]) | (spread3_21(c[1]) << 1) | (spread3_21(c[2]) << 2);
 }
 
+// compute_morton_f32 (nbody_gpu_twopass.rs:302-326): 10 bits per axis of (x + box_half) * 1024 / L in f32,
+// the float-to-u32 conversion saturating (negative and NaN -> 0) and clamped to 1023, key = mc << 32 | i.
+__device__ inline uint32_t cell10(float x) {
+    return !(x > 0.0f) ? 0u : (x >= 1024.0f ? 1023u : (uint32_t)x);
+}
+__global__ void k_morton30_keys_f32(const float* __restrict__ pos, uint64_t* __restrict__ keys,
+                                    int32_t* __restrict__ idx, int64_t n, float box_half) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float scale = __fdiv_rn(1024.0f, __fmul_rn(2.0f, box_half));
+    const uint32_t cx = cell10(__fmul_rn(__fadd_rn(pos[3 * i], box_half), scale));
+    const uint32_t cy = cell10(__fmul_rn(__fadd_rn(pos[3 * i + 1], box_half), scale));
+    const uint32_t cz = cell10(__fmul_rn(__fadd_rn(pos[3 * i + 2], box_half), scale));
+    // a 10-bit value sits in the low bits of the 21-bit spread, so spread3_21 gives the 30-bit interleave
+    const uint64_t mc = spread3_21(cx) | (spread3_21(cy) << 1) | (spread3_21(cz) << 2);
+    keys[i] = (mc << 32) | (uint64_t)(uint32_t)i;
+    idx[i] = (int32_t)i;
+}
+
+hipError_t launch_morton30_keys_f32(const float* pos, uint64_t* keys, int32_t* idx, int64_t n, float box_half,
+                                    hipStream_t s) {
+    if (n > 0)
+        k_morton30_keys_f32<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(pos, keys, idx, n,
+                                                                                                box_half);
+    return hipGetLastError();
+}
+
 hipError_t launch_morton_keys(const uint32_t* pos, const int8_t* species, uint64_t* keys, int32_t* vals,
                               uint4* rec, int64_t n, hipStream_t s) {
     if (n > 0)

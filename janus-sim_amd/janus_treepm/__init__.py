@@ -44,7 +44,8 @@ EXPORTED = [
     "jtp_ensemble_create", "jtp_ensemble_destroy", "jtp_ensemble_last_error", "jtp_ensemble_compute_forces",
     "jtp_ensemble_get_stats", "jtp_diagnostics", "jtp_potential_energy", "jtp_device_memory",
     "jtp_segregation", "jtp_snap_meta_default", "jtp_snapshot_write", "jtp_snapshot_read_meta", "jtp_snapshot_read",
-    "jtp_io_last_error", "jtp_ic_default_params", "jtp_generate_ics",
+    "jtp_io_last_error", "jtp_ic_default_params", "jtp_generate_ics", "jtp_params_from_config",
+    "jtp_morton30_keys_f32",
 ]
 
 
@@ -62,6 +63,16 @@ class Params(C.Structure):
         ("precision", C.c_int32), ("split_mode", C.c_int32), ("gradient", C.c_int32),
         ("deconvolve", C.c_int32), ("pm_enabled", C.c_int32), ("tree_enabled", C.c_int32),
         ("interlace", C.c_int32), ("theta_cut", C.c_double), ("fft_tile", C.c_int32),
+    ]
+
+
+class ConfigFields(C.Structure):
+    """jtp_config_fields: the YAML fields of JanusConfig that set the force (config.rs:21-40,94-102) plus the
+    optional pm_grid.split_scale_cells / pm_grid.r_cut_over_r_s (absent: <= 0)."""
+    _fields_ = [
+        ("box_size_mpc", C.c_double), ("n_particles", C.c_int64), ("theta", C.c_double),
+        ("softening_mpc", C.c_double), ("n_cells", C.c_int32), ("k_min", C.c_int32),
+        ("split_scale_cells", C.c_double), ("r_cut_over_r_s", C.c_double),
     ]
 
 
@@ -127,6 +138,8 @@ def load_library(path: str = LIB_PATH):
         "jtp_pos_to_fixed_f32": (C.c_int, [vp, i64, vp, vp, vp]),
         "jtp_pm_solve": (C.c_int, [vp, vp, vp, vp]),
         "jtp_morton_keys_f32": (C.c_int, [i64, vp, C.c_float, C.c_float, vp, vp]),
+        "jtp_morton30_keys_f32": (C.c_int, [i64, vp, C.c_float, C.c_float, vp, vp, vp]),
+        "jtp_params_from_config": (C.c_int, [C.POINTER(ConfigFields), C.POINTER(Params)]),
         "jtp_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
         "jtp_stage_size": (C.c_int, [vp, i32, C.POINTER(sz)]),
         "jtp_stage_copy": (C.c_int, [vp, i32, vp, sz]),
@@ -280,6 +293,21 @@ def default_params(box_size: float, n_cells: int, **kw) -> Params:
     return p
 
 
+def params_from_config(box_size_mpc: float, n_cells: int, n_particles: int = 0, theta: float | None = None,
+                       softening_mpc: float | None = None, k_min: int = 0, split_scale_cells: float | None = None,
+                       r_cut_over_r_s: float | None = None) -> Params:
+    """jtp_params from the YAML fields (jtp_params_from_config); None means the field is absent."""
+    f = ConfigFields(float(box_size_mpc), int(n_particles), -1.0 if theta is None else float(theta),
+                     0.0 if softening_mpc is None else float(softening_mpc), int(n_cells), int(k_min),
+                     0.0 if split_scale_cells is None else float(split_scale_cells),
+                     0.0 if r_cut_over_r_s is None else float(r_cut_over_r_s))
+    p = Params()
+    rc = load_library().jtp_params_from_config(C.byref(f), C.byref(p))
+    if rc != JTP_OK:
+        raise JtpError(rc, "invalid config fields")
+    return p
+
+
 class ForceContext:
     """One jtp context: the device force call on fixed-point particle arrays."""
 
@@ -304,19 +332,35 @@ class ForceContext:
     def set_species_mass(self, m_plus: float, m_minus_abs: float):
         self._check(self.lib.jtp_set_species_mass(self.h, float(m_plus), float(m_minus_abs)))
 
+    def _arg(self, d, dtype, count, name):
+        """Raw pointer of a device argument; a DeviceArray must hold `count` elements of `dtype` (a wrong
+        precision or a short buffer would make the kernels write past the allocation)."""
+        if not isinstance(d, DeviceArray):
+            return d
+        if d.dtype != np.dtype(dtype):
+            raise ValueError(f"{name}: dtype {d.dtype}, expected {np.dtype(dtype)}")
+        if d.nbytes < count * np.dtype(dtype).itemsize:
+            raise ValueError(f"{name}: {d.nbytes} bytes, need {count * np.dtype(dtype).itemsize}")
+        return d.ptr
+
     def compute_forces_device(self, n: int, d_pos_fixed, d_species, d_acc, stream=None):
-        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
-        self._check(self.lib.jtp_compute_forces(self.h, int(n), ptr(d_pos_fixed), ptr(d_species), ptr(d_acc), stream))
+        n = int(n)
+        self._check(self.lib.jtp_compute_forces(self.h, n, self._arg(d_pos_fixed, np.uint32, 3 * n, "pos"),
+                                                self._arg(d_species, np.int8, n, "species"),
+                                                self._arg(d_acc, self.real, 3 * n, "acc"), stream))
 
     def step_device(self, n: int, d_pos_fixed, d_vel, d_species, d_acc, dt: float, hubble_plus: float = 0.0,
                     hubble_minus: float | None = None, dtau_dt: float = 1.0, scheme: int = STEP_DKD,
                     graph: bool = False, stream=None):
         """One integrator step in place on device arrays (jtp_step): DKD is step_treepm_gpu's order,
         KDK needs d_acc = a(t) on entry.  hubble_minus defaults to hubble_plus (the reference's one H)."""
-        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        n = int(n)
         hm = hubble_plus if hubble_minus is None else hubble_minus
         flags = int(scheme) | (STEP_GRAPH if graph else 0)
-        self._check(self.lib.jtp_step(self.h, int(n), ptr(d_pos_fixed), ptr(d_vel), ptr(d_species), ptr(d_acc),
+        self._check(self.lib.jtp_step(self.h, n, self._arg(d_pos_fixed, np.uint32, 3 * n, "pos"),
+                                      self._arg(d_vel, self.real, 3 * n, "vel"),
+                                      self._arg(d_species, np.int8, n, "species"),
+                                      self._arg(d_acc, self.real, 3 * n, "acc"),
                                       flags, float(dt), float(hubble_plus), float(hm), float(dtau_dt), stream))
 
     def diagnostics(self, n: int, d_vel, d_species, stream=None) -> dict:

@@ -76,6 +76,32 @@ void orc_morton_f32_ref(int64_t n, const float* pos, float box_half, float inv_c
     }
 }
 
+// compute_morton_f32 (nbody_gpu_twopass.rs:302-326): scale = 1024 / (2 box_half) in f32, per axis
+// (unsigned)((x + box_half) * scale) (CUDA's conversion saturates: negative and NaN -> 0), min 1023,
+// the bit loop expand10_tp, key = mc << 32 | i.
+void orc_morton30_f32_ref(int64_t n, const float* pos, float box_half, uint64_t* keys, int32_t* idx) {
+    volatile float two_bh = 2.0f * box_half;
+    volatile float scale = 1024.0f / two_bh;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t mc = 0;
+        for (int a = 0; a < 3; ++a) {
+            volatile float s = pos[3 * i + a] + box_half;
+            volatile float g = s * scale;
+            const float x = g;
+            uint32_t c;
+            if (!(x > 0.0f)) c = 0;
+            else if (x >= 4294967296.0f) c = 0xffffffffu;
+            else c = (uint32_t)x;
+            if (c > 0x3ffu) c = 0x3ffu;
+            uint64_t r = 0;
+            for (int b = 0; b < 10; ++b) r |= (uint64_t)((c >> b) & 1u) << (3 * b);
+            mc |= r << a;
+        }
+        keys[i] = (mc << 32) | (uint64_t)(uint32_t)i;
+        idx[i] = (int32_t)i;
+    }
+}
+
 void orc_radix_sort_ref(int64_t n, const uint64_t* keys_in, int32_t* perm) {
     std::vector<uint64_t> ka(keys_in, keys_in + n), kb(n);
     std::vector<int32_t> va(n), vb(n);

@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# JTP_ORACLE_LIB selects another build of the same source (the sanitizer build, oracle/Makefile `asan`)
+LIB = os.environ.get("JTP_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 def build():
@@ -25,6 +26,7 @@ class Oracle:
             "orc_fixed_from_f64": (None, [i64, vp, dbl, vp]),
             "orc_morton_fixed": (None, [i64, vp, vp]),
             "orc_morton_f32_ref": (None, [i64, vp, flt, flt, vp]),
+            "orc_morton30_f32_ref": (None, [i64, vp, flt, vp, vp]),
             "orc_radix_sort_ref": (None, [i64, vp, vp]),
             "orc_karras_ref": (None, [i64, vp, vp]),
             "orc_node_sums": (None, [i64, vp, vp, vp, vp, vp]),
@@ -72,6 +74,14 @@ class Oracle:
         k = np.empty(len(p), np.uint64)
         self.lib.orc_morton_f32_ref(len(p), p.ctypes.data, box_half, inv_cell, k.ctypes.data)
         return k
+
+    def morton30_f32_ref(self, pos_f32, box_half):
+        """(keys, indices) of compute_morton_f32: 30-bit key << 32 | index."""
+        p = np.ascontiguousarray(pos_f32, np.float32).reshape(-1, 3)
+        k = np.empty(len(p), np.uint64)
+        idx = np.empty(len(p), np.int32)
+        self.lib.orc_morton30_f32_ref(len(p), p.ctypes.data, box_half, k.ctypes.data, idx.ctypes.data)
+        return k, idx
 
     def radix_sort(self, keys):
         k = np.ascontiguousarray(keys, np.uint64)

@@ -73,6 +73,33 @@ def test_reference_f32_keys(jtp, orc):
     assert np.array_equal(d_keys.download(), orc.morton_f32_ref(pos, box_half, inv_cell))
 
 
+def test_reference_30bit_keys(jtp, orc):
+    """jtp_morton30_keys_f32 reproduces compute_morton_f32 (nbody_gpu_twopass.rs:302-326) bit for bit: the
+    10-bit-per-axis key of the reference's per-species tree build with the particle index in the low 32 bits,
+    including the saturating conversion of out-of-box and NaN coordinates."""
+    print("case: reference 30-bit Morton keys", flush=True)
+    import ctypes as C
+    rng = np.random.default_rng(12)
+    for L, n in ((250.0, 100_003), (99.1, 4097)):
+        pos = rng.uniform(-L / 2, L / 2, (n, 3)).astype(np.float32)
+        pos[:10] = np.float32(-L / 2)
+        pos[10:20] = np.float32(L / 2)
+        pos[20:30] = np.float32(-3 * L)
+        pos[30:40] = np.float32(1e30)
+        pos[40:50] = np.float32("nan")
+        pos[50:60] = np.float32(-np.inf)
+        box_half = np.float32(L / 2)
+        d_pos = jtp.DeviceArray.from_host(pos)
+        d_keys = jtp.DeviceArray((n,), np.uint64)
+        d_idx = jtp.DeviceArray((n,), np.int32)
+        lib = jtp.load_library()
+        assert lib.jtp_morton30_keys_f32(n, d_pos.ptr, C.c_float(box_half), C.c_float(0.0), d_keys.ptr, d_idx.ptr,
+                                         None) == 0
+        k_ref, i_ref = orc.morton30_f32_ref(pos, box_half)
+        assert np.array_equal(d_keys.download(), k_ref)
+        assert np.array_equal(d_idx.download(), i_ref)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_tree_topology_and_monopoles(jtp, orc, precision):
     print(f"case: tree topology precision={precision}", flush=True)

@@ -51,6 +51,38 @@ def test_morton_f32_reference_formula(orc):
     assert got[1] == (1 << 63) - 1 and got[2] == 0
 
 
+def test_morton30_reference_formula(orc):
+    """compute_morton_f32 (nbody_gpu_twopass.rs:302-326) evaluated independently in numpy IEEE single:
+    scale = 1024 / (2 box_half), saturating conversion, min 1023, 10-bit interleave, key << 32 | index."""
+    L = 250.0
+    bh = np.float32(L / 2)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-L / 2, L / 2, (5000, 3)).astype(np.float32)
+    x[0] = -L / 2
+    x[1] = L / 2            # 1024 * scale rounds to the top cell, clamps to 1023
+    x[2] = -200.0           # negative before the conversion: 0
+    x[3] = 1e30             # beyond 2^32: saturates, then 1023
+    x[4] = (0.5, 0.25, -0.125)
+    scale = np.float32(1024.0) / (np.float32(2.0) * bh)
+    g = ((x + bh) * scale).astype(np.float32)
+    c = np.where(g > 0, np.minimum(np.where(g >= np.float32(2 ** 32), np.float32(2 ** 32 - 1), g), 1023.0), 0)
+    c = c.astype(np.uint64)
+    mc = np.zeros(len(x), np.uint64)
+    for a in range(3):
+        for b in range(10):
+            mc |= ((c[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + a)
+    want = (mc << np.uint64(32)) | np.arange(len(x), dtype=np.uint64)
+    keys, idx = orc.morton30_f32_ref(x, bh)
+    assert np.array_equal(keys, want)
+    assert np.array_equal(idx, np.arange(len(x)))
+    assert keys[1] >> np.uint64(32) == (1 << 30) - 1 and keys[2] >> np.uint64(32) == 0
+    assert keys[3] >> np.uint64(32) == (1 << 30) - 1
+    # a known answer: cells (514, 513, 511) of x = (0.5, 0.25, -0.125) at L = 250 (cell 0.244 Mpc)
+    cx, cy, cz = 514, 513, 511
+    sp = lambda v: sum(((v >> b) & 1) << (3 * b) for b in range(10))
+    assert int(keys[4]) == ((sp(cx) | (sp(cy) << 1) | (sp(cz) << 2)) << 32) | 4
+
+
 def test_radix_sort_is_stable_argsort(orc):
     rng = np.random.default_rng(2)
     k = rng.integers(0, 2 ** 63, 30000, dtype=np.uint64)
