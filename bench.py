@@ -321,7 +321,8 @@ def main():
     ap.add_argument("--ensemble-streams", type=int, default=4, help="streams of the ensemble_c3 workload")
     ap.add_argument("--autotune", action="store_true",
                     help="multistep_41M: pick leaf and group size with janus_treepm.autotune first")
-    ap.add_argument("--accuracy", action="store_true", help="compare both forms with the FP64 Ewald truth fixture")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the comparison of both forms with the FP64 Ewald truth fixture")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's streaming copy kernel on MIB MiB, 5 times (HBM counter calibration)")
     ap.add_argument("--set", action="append", default=[], metavar="FIELD=VALUE",
@@ -423,7 +424,7 @@ def main():
                        "steps": args.fp64_steps, "phases_ms": phase_summary(st64)}
         out["fp32_over_fp64"] = out["fp64"]["ms_per_step"] / ms32
 
-    if args.accuracy:
+    if not args.no_accuracy:
         log("accuracy vs Ewald truth ...")
         out["accuracy_vs_ewald"] = accuracy(w, args.workload, d_pos, d_sp, pos)
     del pos

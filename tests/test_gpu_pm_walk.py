@@ -177,6 +177,45 @@ def test_pm_solve_numpy(jtp, precision, G):
     assert np.max(np.abs(phi - ref)) <= tol * np.max(np.abs(ref))
 
 
+def _green_numpy(G, L, rs):
+    """Closed-form Green function on numpy's real half spectrum (axes z, y, x) and the largest integer
+    wavenumber component of each mode."""
+    h = L / G
+    m = np.fft.fftfreq(G, d=1.0 / G)
+    mz, my, mx = m[:, None, None], m[None, :, None], np.fft.rfftfreq(G, d=1.0 / G)[None, None, :]
+    kz, ky, kx = (2 * np.pi / L) * mz, (2 * np.pi / L) * my, (2 * np.pi / L) * mx
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    sinc = lambda x: np.sinc(x / np.pi)
+    W = sinc(kx * h / 2) ** 2 * sinc(ky * h / 2) ** 2 * sinc(kz * h / 2) ** 2
+    D = [(8 * np.sin(K * h) - np.sin(2 * K * h)) / (6 * h) for K in (kx, ky, kz)]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        green = -4 * np.pi / k2 * np.exp(-k2 * rs ** 2) / W ** 2 / h ** 3
+        green *= (kx * D[0] + ky * D[1] + kz * D[2]) / np.maximum(D[0] ** 2 + D[1] ** 2 + D[2] ** 2, 0.25 * k2)
+    green[0, 0, 0] = 0
+    return green, np.maximum(np.abs(mx), np.maximum(np.abs(my), np.abs(mz)))
+
+
+@pytest.mark.parametrize("precision,G,k_min", [(1, 32, 2), (0, 32, 2), (1, 64, 3), (0, 64, 2)])
+def test_pm_solve_kmin_numpy(jtp, precision, G, k_min):
+    """The k_min filter of the Green function on the GPU (cufft_wrapper.cu:170-181: modes whose largest
+    integer wavenumber component max(|m_x|, |m_y|, |m_z|) is below k_min are zeroed, DC included)
+    against numpy's FFT with the same closed-form Green function."""
+    print(f"case: pm_solve k_min={k_min} vs numpy precision={precision} G={G}", flush=True)
+    L, rs = 100.0, 96.0 / G
+    rng = np.random.default_rng(2)
+    rho = rng.standard_normal((G, G, G))
+    ps = jtp.PoissonSolver(G, L, g_constant=1.0, r_s=rs, k_min=k_min, precision=precision, deconvolve=1)
+    phi = ps.solve(rho).astype(np.float64)
+    green, kmax = _green_numpy(G, L, rs)
+    spec = np.fft.rfftn(rho)
+    ref = np.fft.irfftn(spec * np.where(kmax < k_min, 0.0, green), s=(G, G, G), axes=(0, 1, 2))
+    tol = 1e-11 if precision == 1 else 3e-5
+    assert np.max(np.abs(phi - ref)) <= tol * np.max(np.abs(ref))
+    # the filter removes something: the unfiltered potential differs by far more than the tolerance
+    full = np.fft.irfftn(spec * green, s=(G, G, G), axes=(0, 1, 2))
+    assert np.max(np.abs(full - ref)) > 100 * tol * np.max(np.abs(ref))
+
+
 @pytest.mark.parametrize("precision,mode,gs", [(1, 0, 64), (0, 0, 64), (1, 1, 64), (0, 1, 64), (1, 2, 64),
                                                (0, 2, 64), (0, 0, 32), (1, 0, 16), (0, 0, 16), (0, 2, 32)])
 def test_walk_theta0_direct(jtp, orc, precision, mode, gs):
@@ -194,6 +233,35 @@ def test_walk_theta0_direct(jtp, orc, precision, mode, gs):
         assert np.median(e) < 3e-6 and np.percentile(e, 99) < 1e-4, (np.median(e), np.percentile(e, 99))
 
 
+@pytest.mark.parametrize("level,precision,gs", [(4, 1, 32), (4, 0, 32), (4, 1, 64), (4, 0, 64),
+                                                (3, 1, 32), (3, 0, 32), (3, 0, 64)])
+def test_walk_theta0_seed_levels(jtp, orc, level, precision, gs):
+    """theta = 0 on clustered boxes at the headline box's density per PM cell (1.31 particles) whose r_cut
+    seeds the walk from the Morton cells of level 3 (L = 100, PM 64^3, r_cut = 8.79:
+    L/16 < r_cut <= L/8) or level 4 (L = 100, PM 128^3, r_cut = 4.39 <= L/16, the seeding of the
+    headline box, whose r_cut = 5.49 <= 250/16): the 27 cells around each group's cell, their nodes
+    from the cell table, then the walk -- against the minimum-image direct sum on 2000 seeded targets,
+    leaf 4, groups of 32 and 64."""
+    G = 128 if level == 4 else 64
+    n = int(round(22_000_000 / 256 ** 3 * G ** 3))
+    print(f"case: walk theta=0 start level {level} precision={precision} group={gs} N={n} G={G}", flush=True)
+    w = synth.clustered(n_total=n, box_size=100.0, n_cells=G, n_halos=max(1, 2000 * G ** 3 // 256 ** 3),
+                        seed=31 + level)
+    assert w.box_size / 2 ** (level + 1) < w.r_cut <= w.box_size / 2 ** level
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=precision, pm_enabled=0, theta=0.0, group_size=gs, leaf_size=4)
+    acc = ctx.compute_forces(pos, w.species).astype(np.float64)
+    tg = np.random.default_rng(level * 100 + gs).choice(w.n, 2000, replace=False).astype(np.int64)
+    ref = orc.short_direct(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, w.r_cut, w.r_s, 0,
+                           targets=tg)
+    e = _rel(acc[tg], ref)
+    print("level %d: median %.2e p99 %.2e max %.2e" % (level, np.median(e), np.percentile(e, 99), e.max()))
+    if precision == 1:
+        assert np.median(e) < 1e-13 and np.max(e) < 1e-9, (np.median(e), np.max(e))
+    else:
+        assert np.median(e) < 3e-6 and np.percentile(e, 99) < 1e-4, (np.median(e), np.percentile(e, 99))
+
+
 @pytest.mark.parametrize("precision", [1, 0])
 def test_walk_theta_half(jtp, orc, precision):
     print(f"case: walk theta=0.5 precision={precision}", flush=True)
@@ -204,10 +272,12 @@ def test_walk_theta_half(jtp, orc, precision):
     ref = orc.short_direct(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, w.r_cut, w.r_s, 0)
     e = _rel(acc, ref)
     print("theta=0.5 short-range rel err median %.2e p99 %.2e" % (np.median(e), np.percentile(e, 99)))
-    # accepted monopoles keep the smooth tail of S beyond r_cut that the hard-cut direct sum drops,
-    # so this is a regression bound, not a parity bar (parity is the theta = 0 test; accuracy is
-    # measured against the Ewald sum of the full force)
-    assert np.median(e) < 5e-3 and np.percentile(e, 99) < 5e-2
+    # A regression bound, not the accuracy bar: accepted monopoles keep the smooth tail of S beyond
+    # r_cut that the hard-cut direct sum drops, so the two differ by design (measured: median 3.3e-3,
+    # p99 4.3e-2 in both forms).  Parity is the theta = 0 tests; the accuracy bar (median <= 1e-3,
+    # p99 <= 1e-2 per species) is asserted on the full force against the FP64 Ewald sum in
+    # test_gpu_headline.py.
+    assert np.median(e) < 4e-3 and np.percentile(e, 99) < 5e-2
 
 
 def test_walk_theta_half_forms_agree(jtp):
