@@ -97,6 +97,36 @@ def cpu_baseline(w_full):
     }
 
 
+def config0(warmup=2, steps=10):
+    """The small configuration timed in full: N = 2 x 64^3 lattice + Zel'dovich (P(k) ~ k^-2.5, rms 0.3
+    cell), L = 250, PM 64^3, r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, leaf 8, m+ = 1, |m-| = 19:
+    one force evaluation on the GPU in both forms (inputs resident in HBM), and one update + all forces
+    of the C++ restatement of the reference CPU TreePM (r_s = r_cut / 3 with r_s = 1.25 cells) on the
+    whole particle set, all host threads."""
+    from oracle.oracle_lib import RefCpu, load
+
+    w = synth.lattice_zeldovich(n_side=64, box_size=250.0, n_cells=64, seed=42, leaf_size=8)
+    pos = j.to_fixed(w.pos, w.box_size)
+    d_pos, d_sp = j.DeviceArray.from_host(pos), j.DeviceArray.from_host(w.species)
+    out = {"n_particles": w.n, "pm_grid": w.n_cells, "box": w.box_size, "leaf_size": 8}
+    for prec, label in ((j.FP64, "fp64"), (j.FP32, "fp32")):
+        t, _, d_acc = run_form(w, d_pos, d_sp, prec, warmup, steps)
+        d_acc.free()
+        out[label] = {"ms_per_eval": t / steps * 1e3, "updates_per_s": w.n * steps / t}
+    orc = load()
+    h = w.box_size / w.n_cells
+    mass = np.where(w.species > 0, w.m_plus, w.m_minus_abs)
+    ref = RefCpu(orc, 3.0 * w.r_s, w.n_cells, w.box_size, w.theta, w.softening, w.r_s, 1.0 / h ** 3, 1.0)
+    t0 = time.perf_counter()
+    ref.update(w.pos, mass, w.species)
+    ref.forces()
+    t = time.perf_counter() - t0
+    out["cpu_reference_port"] = {"ms_per_eval": t * 1e3, "updates_per_s": w.n / t,
+                                  "cores": int(orc.lib.orc_num_threads())}
+    out["fp64_gpu_over_cpu"] = out["cpu_reference_port"]["ms_per_eval"] / out["fp64"]["ms_per_eval"]
+    return out
+
+
 def measured_traffic(workload):
     """HBM bytes per walk launch from the committed FETCH_SIZE / WRITE_SIZE summary of this workload
     (profiles/walk_fp32_<workload>_hbm_traffic.txt, calibrated against a copy kernel of known size)."""
@@ -321,6 +351,7 @@ def main():
     ap.add_argument("--ensemble-streams", type=int, default=4, help="streams of the ensemble_c3 workload")
     ap.add_argument("--autotune", action="store_true",
                     help="multistep_41M: pick leaf and group size with janus_treepm.autotune first")
+    ap.add_argument("--no-c0", action="store_true", help="skip the full timing of the small configuration")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the comparison of both forms with the FP64 Ewald truth fixture")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -428,6 +459,10 @@ def main():
         log("accuracy vs Ewald truth ...")
         out["accuracy_vs_ewald"] = accuracy(w, args.workload, d_pos, d_sp, pos)
     del pos
+
+    if not args.no_c0:
+        log("config 0 (2 x 64^3, PM 64^3) ...")
+        out["config0"] = config0()
 
     if not args.no_cpu:
         log("cpu baseline ...")

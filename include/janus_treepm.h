@@ -84,7 +84,8 @@ typedef struct jtp_stats {
     double  ms_total;     /* force call, or the whole step (kicks, drifts) after jtp_step */
     double  ms_sort;      /* Morton keys + radix sort + particle gather        */
     double  ms_build;     /* Karras hierarchy + bottom-up monopoles + groups   */
-    double  ms_deposit;   /* cell offsets + deterministic CIC gather deposit   */
+    double  ms_deposit;   /* deterministic CIC deposit (FP32: tile lists, per-tile
+                             integer sums, face merge; FP64: cell offsets, gather) */
     double  ms_fft;       /* R2C, Green, C2R                                   */
     double  ms_interp;    /* gradient + CIC interpolation                      */
     double  ms_walk;      /* group walk (one kernel launch)                    */

@@ -135,7 +135,8 @@ void make_split_poly(Ctx& c) {
 static void free_particle_buffers(Ctx& c) {
     void* ptrs[] = {c.keys0, c.keys1, c.vals0, c.vals1, c.hist, c.hist_blk, c.parts, c.node_geo, c.node_kids,
                     c.up, c.flags, c.node_sum, c.node_mono, c.groups, c.pm_acc,
-                    c.frontier0, c.frontier1, c.dense_list, c.dense_corner, c.interp_list};
+                    c.frontier0, c.frontier1, c.dense_list, c.dense_corner, c.interp_list, c.dep_chunks,
+                    c.dep_cbuf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c.keys0 = c.keys1 = nullptr;
@@ -149,6 +150,8 @@ static void free_particle_buffers(Ctx& c) {
     c.frontier0 = c.frontier1 = c.dense_list = nullptr;
     c.dense_corner = nullptr;
     c.interp_list = nullptr;
+    c.dep_chunks = nullptr;
+    c.dep_cbuf = nullptr;
     c.node_sum = c.node_mono = nullptr;
     c.groups = nullptr;
     c.pm_acc = nullptr;
@@ -179,9 +182,14 @@ static int ensure_capacity(Ctx& c, int64_t n) {
     chk(dalloc(c.flags, cap));
     chk(dalloc(c.frontier0, cap));
     chk(dalloc(c.frontier1, cap));
-    chk(dalloc(c.dense_list, cap / kDenseCell + 1));
     chk(dalloc(c.interp_list, (size_t)interp_list_size(c.G3(), cap)));
-    chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * (8 + 27) * rs));
+    if (c.p.precision == JTP_FP64) {  // dense-cell pre-reduction of the gather deposit
+        chk(dalloc(c.dense_list, cap / kDenseCell + 1));
+        chk(dalloc_bytes(c.dense_corner, (size_t)(cap / kDenseCell + 1) * (8 + 27) * rs));
+    } else {  // chunks of the heavy tiles of the read-once deposit
+        chk(dalloc(c.dep_chunks, (size_t)dep_chunk_cap(cap)));
+        chk(dalloc(c.dep_cbuf, (size_t)dep_chunk_cap(cap) * dep_chunk_nodes()));
+    }
     chk(dalloc_bytes(c.node_sum, (size_t)cap * 64));
     chk(dalloc_bytes(c.node_mono, (size_t)cap * (rs == 8 ? 64 : 32)));
     chk(dalloc(c.groups, cap + 4096 + 64));
@@ -327,14 +335,19 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
     if (e == hipSuccess) e = dalloc_bytes(c.spec, (size_t)G * G * c.Hc() * 2 * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.twiddle, (size_t)2 * G * rs);
     if (e == hipSuccess) e = dalloc_bytes(c.gtab, (size_t)4 * G * rs);
-    if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
+    if (p->precision == JTP_FP64) {
+        if (e == hipSuccess) e = dalloc(c.cell_start, (size_t)G3 + 1);
+        if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
+    } else {
+        if (e == hipSuccess) e = dalloc(c.tile_start, (size_t)(G3 >> 9) + 1);
+        if (e == hipSuccess) e = dalloc(c.dep_tbuf, (size_t)(G3 >> 9) * dep_tile_slots());
+    }
     if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
     if (e == hipSuccess) e = dalloc(c.cellnode, (size_t)4096);
     c.start_level = 2;  // L/4 > L/5 >= r_cut
     while (c.start_level < kGroupLevelBits && p->box_size / (double)(1 << (c.start_level + 1)) >= p->r_cut)
         ++c.start_level;
     if (e == hipSuccess) e = dalloc(c.dev_counters, kCtrTotal);
-    if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
     if (e == hipSuccess) e = dalloc(c.dev_inter, 2);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c.host_pinned, 4 * sizeof(unsigned long long), 0);
     if (e == hipSuccess) e = dalloc(c.dev_step, 1);
@@ -368,6 +381,7 @@ int jtp_destroy(jtp_ctx* h) {
     if (c.graph.exec) (void)hipGraphExecDestroy(c.graph.exec);
     if (c.graph.capture_stream) (void)hipStreamDestroy(c.graph.capture_stream);
     void* ptrs[] = {c.rho, c.phi, c.rho2, c.phi2, c.grad, c.grad2, c.spec, c.twiddle, c.gtab, c.cell_start,
+                    c.tile_start, c.dep_tbuf,
                     c.cell4, c.cellnode, c.dev_counters, c.dev_inter, c.dense_slot, c.dev_step, c.diag_scratch,
                     c.pot_short, c.acc_scratch, c.kself, c.seg};
     for (void* p : ptrs)
@@ -407,7 +421,8 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
     uint4* rec = (uint4*)c.node_sum;
     JTP_HIP(&c, launch_morton_keys(d_pos, d_species, c.keys0, c.vals0, rec, n, s));
     JTP_HIP(&c, radix_sort_pairs(c, n, s));
-    JTP_HIP(&c, launch_gather_parts(rec, c.vals0, c.parts, n, s));
+    const bool tiles = c.p.pm_enabled && c.p.precision == JTP_FP32;  // tile starts for the FP32 PM
+    JTP_HIP(&c, launch_gather_parts(rec, c.vals0, c.parts, n, tiles ? c.tile_start : nullptr, c.logG, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[1], s));
     // The tree build and the PM phases only share read-only inputs (sorted keys and records), so the
     // build runs on the side stream while the PM phases run on `s`; the walk waits for both.

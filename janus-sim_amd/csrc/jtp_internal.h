@@ -21,7 +21,7 @@ constexpr int kGroupLevelBits = 4;   // walk groups never straddle a level-4 Mor
 constexpr int kPolyDeg = 10;         // FP32 split-function polynomial degree
 constexpr int kDenseCell = 32;       // PM cells with more particles are pre-reduced by wavefronts,
 constexpr int kDenseChunk = 1024;    // one per chunk of this many particles
-constexpr int kInterpChunk = 4096;   // FP32 interpolation: particles per workgroup of a heavy tile,
+constexpr int kInterpChunk = 4096;   // FP32 interpolation and deposit: particles per workgroup of a heavy tile,
 constexpr int kLightMax = 1024;      // tiles with at most this many are light (four particles per thread)
 // int2 entries of Ctx::interp_list for G^3 cells and up to n particles: the light-tile list, then
 // sum over tiles of ceil(np / kInterpChunk) <= n / kInterpChunk + n / (kLightMax + 1) + 1 chunks
@@ -31,7 +31,7 @@ inline int64_t interp_list_size(int64_t G3, int64_t n) {
 constexpr int kMaxRounds = 128;      // size of the bottom-up control block (frontier count ... root flag)
 constexpr int kXcds = 8;             // MI355X accelerator complex dies; workgroup i runs on XCD i % 8
 // device counter block (int32): offsets
-constexpr int kCtrNGroups = 0, kCtrErr = 1, kCtrDense = 3, kCtrInterp = 4;  // kCtrInterp: 2 words
+constexpr int kCtrNGroups = 0, kCtrErr = 1, kCtrDense = 3, kCtrInterp = 4;  // kCtrInterp: 3 words
 constexpr int kCtrRounds = 8;                       // [0] global frontier size, [kMaxRounds+1] root flag
 constexpr int kCtrRoot = kCtrRounds + kMaxRounds + 1;
 constexpr int kCtrXcd = kCtrRounds + kMaxRounds + 4;  // kXcds walk-queue heads
@@ -129,6 +129,9 @@ struct Ctx {
     int32_t *dense_list = nullptr;   // deposit: the PM cells holding more than kDenseCell particles
     int2 *interp_list = nullptr;     // FP32 interpolation: the light tiles (G^3 / 512 int32), then the
                                      // (tile, first particle) of the heavy tiles' chunks; interp_list_size
+    int2 *dep_chunks = nullptr;      // FP32 deposit: (tile, first particle) of the chunks of tiles holding
+                                     // more than kInterpChunk particles; dep_chunk_cap entries
+    long long *dep_cbuf = nullptr;   // FP32 deposit: int64 LDS grid of each such chunk (1729 per entry)
     void *node_sum = nullptr;        // NodeSum per internal node
     void *node_mono = nullptr;       // 2 x (uint4 | double4) per internal node
     int2 *groups = nullptr;
@@ -138,7 +141,9 @@ struct Ctx {
     int32_t *dev_counters = nullptr; // kCtr* layout
     unsigned long long *dev_inter = nullptr;  // interaction counter
     // PM
-    int32_t *cell_start = nullptr;   // G^3 + 1
+    int32_t *cell_start = nullptr;   // FP64: G^3 + 1 first sorted particle of each cell
+    int32_t *tile_start = nullptr;   // FP32: G^3 / 512 + 1 first sorted particle of each 8^3-cell tile
+    float *dep_tbuf = nullptr;       // FP32 deposit: the face and beyond-face node sums of each tile (1170 per tile)
     int32_t *dense_slot = nullptr;   // G^3: slot of a dense cell's corner sums (written for dense cells only)
     void *dense_corner = nullptr;    // real [n / kDenseCell + 1][8 + 27] corner sums of dense-cell chunks
     void *rho = nullptr;             // real G^3 deposited mass per cell
@@ -201,7 +206,10 @@ hipError_t launch_morton30_keys_f32(const float* pos, uint64_t* keys, int32_t* i
                                     hipStream_t s);
 hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s);   // keys0/vals0 -> keys0/vals0 (sorted)
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
-hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, hipStream_t s);
+// parts[i] = rec[perm[i]]; with tile_start (FP32 PM), also the first sorted particle of every
+// Morton-aligned 8^3-cell tile of the 2^logG mesh, tile_start[ntile] = n
+hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, int32_t* tile_start,
+                               int logG, hipStream_t s);
 
 hipError_t build_tree(Ctx& c, int64_t n, hipStream_t s);            // Karras + bottom-up + mono + groups
 hipError_t launch_fill_i32(int32_t* out, int64_t m, int32_t v, hipStream_t s);
@@ -209,6 +217,9 @@ hipError_t launch_cell_start(const uint64_t* keys, int64_t n, int shift, int64_t
                              hipStream_t s);
 
 hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s);           // grid <- sum q W
+int64_t dep_chunk_cap(int64_t n);  // FP32 deposit: chunk entries for up to n particles
+int64_t dep_chunk_nodes();         // FP32 deposit: int64 node sums per chunk entry
+int64_t dep_tile_slots();          // FP32 deposit: face and beyond-face node sums per tile
 // phi <- Green * rho; matched = 1 multiplies by the gradient-stencil match (the force's Green
 // function when deconvolving; -1 = as the context's force), 0 leaves the potential unmatched
 hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched = -1);

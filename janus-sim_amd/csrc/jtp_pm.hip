@@ -3,8 +3,10 @@
 //
 // Reference behaviour replaced:
 //   cic_scatter (FP64 CAS atomics into two grids), nbody_gpu_twopass.rs:85-158,
-//     and PmGrid::assign_mass, src/treepm/pm_grid.rs:63-103 -> k_deposit: one net grid
-//     rho+ - rho- gathered per node from Morton-sorted particles; no float atomics.
+//     and PmGrid::assign_mass, src/treepm/pm_grid.rs:63-103 -> one net grid rho+ - rho- from the
+//     Morton-sorted particles, no float atomics: FP32 k_dep_tile (each particle read once, integer
+//     LDS sums per 8^3-cell tile, a fixed-order merge of the tile faces); FP64 k_deposit(_both)
+//     (each node gathers its source cells in a fixed order).
 //   cufft_solve_device_filtered (cuFFT D2Z / apply_green_kernel / Z2D / normalize),
 //     cuda/cufft_wrapper.cu:148-208,290-295,350-398, and the rustfft passes of
 //     PmGrid::solve_poisson_filtered, pm_grid.rs:132-271 -> five LDS Stockham passes
@@ -31,7 +33,7 @@ __device__ inline R frac_of(uint32_t u, int logG) {
 }
 
 // ------------------------------------------------------------------ deposit
-// Deterministic CIC deposit ("sort + segmented reduction").  The particles are
+// FP64 form: deterministic CIC deposit ("sort + segmented reduction").  The particles are
 // in Morton order and G is a power of two, so every PM cell is a contiguous run
 // (cell_start).  Cells holding more than kDenseCell particles (halo cores hold up
 // to ~1e5) are first reduced by wavefronts into their 8 corner sums, in a
@@ -268,183 +270,414 @@ __global__ void k_deposit_both(const uint4* __restrict__ parts, const int32_t* _
     rho2[(kk * G + jj) * G + ii] = acc2;
 }
 
-// FP32 form: one workgroup per tile of 8^3 cells (a Morton-aligned block), owning the 8^3 nodes of
-// each grid that take mass from its cells: node-grid nodes 0..7 per axis (cell corners) and the
-// interlaced nodes at the centres of cells 0..7.  Those nodes also take mass from the one-cell
-// shell around the tile (cells -1 .. 8 per axis), so the workgroup reads the particles of all
-// 10^3 cells and keeps only the contributions to its own nodes: every node is written by one
-// workgroup with a plain store, no global atomics and no zeroing pass.  Within the workgroup the
-// sparse cells' contributions are summed in LDS as 32-bit fixed point: integer addition is
-// associative, so the sums do not depend on the order in which the LDS atomics land and the grid
-// is bit-reproducible.  Charges arrive pre-scaled by 2^k with |q| 2^k <= 2^22, and a node takes at
-// most 2 x 2 x 2 sparse cells' worth of unit weight per axis (node grid: cells n-1, n with weights
-// <= 1; interlaced grid: cells m-1, m, m+1 with weights <= 1/2, 1, 1/2) of <= kDenseCell = 32
-// particles each, so |sum| <= 8 * 32 * 2^22 = 2^30 fits an int32; the unit 2^-k is 2^-22 of the
-// largest |q| or finer, and rounding each contribution to it adds a relative error of ~2^-23 per
-// particle, as FP32 itself does.  Dense cells (> kDenseCell particles) enter through their corner
-// sums from k_dense_reduce*, computed with the same pre-scaled charges and added per owned node in
-// a fixed cell order when the node is stored.  Each thread takes a contiguous run of the tile's
-// sparse particles, so the lanes of a wavefront sit in different cells and seldom hit one LDS
-// address together.  Tiles are dealt to the 8 XCDs in contiguous Morton ranges (blocks are
-// dispatched round-robin over the XCDs), so the shell cells a tile reads were mostly read by its
-// predecessor on the same XCD and sit in that XCD's L2.  LDS: 20 KB, so 7 workgroups per CU hide
-// the latency of the cell-table and particle reads.
-constexpr int kTile = 8;
-constexpr int kShell = kTile + 2;                      // cells -1 .. 8 per axis
-constexpr int kShellCells = kShell * kShell * kShell;  // 1000
-constexpr int kN1 = kTile + 3;                         // node-grid nodes -1 .. 9 per axis
-constexpr int kN2 = kTile + 4;                         // centres of cells -2 .. 9 per axis
-constexpr int kFixBits = 22;                           // |q| 2^k <= 2^kFixBits
-static_assert(kFixBits + 8 <= 30, "a node sum must fit an int32");
-static_assert(kDenseCell <= 32, "the int32 bound assumes <= 32 particles per sparse cell");
-// the largest LDS index a shell particle reaches: cell 8 (index 9) plus corner offset 1 per axis in
-// a1; cell 8 above its centre (index 10) plus offset 1 per axis in a2
-static_assert(((kTile + 2) * kN1 + kTile + 2) * kN1 + kTile + 2 < kN1 * kN1 * kN1, "a1 too small");
-static_assert(((kTile + 3) * kN2 + kTile + 3) * kN2 + kTile + 3 < kN2 * kN2 * kN2, "a2 too small");
+constexpr int kTile = 8;     // deposit and interpolation tiles: Morton-aligned blocks of 8^3 cells
+constexpr int kFixBits = 22;  // FP32 deposit: |q| 2^k <= 2^kFixBits, so one contribution fits an int32
 
+// Tiles are classed by particle count: light tiles (1 .. kLightMax particles, kLightMax / 256 per thread) are
+// listed for k_interp_light; every chunk of kInterpChunk particles of a heavier tile (a halo core
+// holds 1e4-1e5) is listed, as (tile, first particle), for a k_interp_heavy workgroup.  The list
+// order varies between runs; each particle is still computed by one thread with the same
+// arithmetic, so the results do not.
+//
+// Tile lists of the FP32 deposit and interpolation, from the tile starts ts[t] (first sorted particle
+// of 8^3-cell tile t, ts[ntile] = n): light tiles (1 .. kLightMax particles) and the kInterpChunk-particle
+// chunks of the others for the interpolation; for the deposit, the chunks of tiles holding more than
+// kInterpChunk particles, each tile's chunks in consecutive entries in chunk order.
+// count: light tiles, interpolation chunks, deposit chunks.
+__global__ __launch_bounds__(256) void k_tile_mark(const int32_t* __restrict__ ts, int32_t ntile,
+                                                   int32_t* __restrict__ light, int2* __restrict__ heavy,
+                                                   int2* __restrict__ dchunks, int32_t* __restrict__ count) {
+    const int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    int32_t beg = 0, np = 0;
+    if (t < ntile) {
+        beg = ts[t];
+        np = ts[t + 1] - beg;
+    }
+    const bool lt = np > 0 && np <= kLightMax;
+    const int32_t nh = np > kLightMax ? (np + kInterpChunk - 1) / kInterpChunk : 0;
+    const int at_l = wave_reserve(lt ? 1 : 0, count);
+    if (lt) light[at_l] = t;
+    const int at_h = wave_reserve(nh, count + 1);
+    for (int32_t k = 0; k < nh; ++k) heavy[at_h + k] = make_int2(t, beg + k * kInterpChunk);
+    const int32_t nd = np > kInterpChunk ? nh : 0;
+    const int at_d = wave_reserve(nd, count + 2);
+    for (int32_t k = 0; k < nd; ++k) dchunks[at_d + k] = make_int2(t, beg + k * kInterpChunk);
+}
+
+// ---- FP32 deposit, each particle read once.
+// A workgroup takes the contiguous Morton run of one 8^3-cell tile's particles (a tile of more
+// than kInterpChunk particles is cut into chunks of that many, one workgroup each) and adds every
+// CIC contribution into LDS as 64-bit fixed point (charges pre-scaled by 2^k, |q| 2^k <= 2^22,
+// each contribution rounded to an int32 once): the node-grid nodes 0..8 per axis its cells touch
+// and the interlaced nodes at the centres of cells -1..8.  Integer addition is associative, so the
+// sums do not depend on the order in which the LDS atomics land.  Nodes only this tile reaches
+// (node grid 1..7, interlaced 1..6 per axis) are final and stored as they are; the tile's other
+// nodes -- its own nodes on the faces, which neighbours reach too, and the nodes beyond its faces,
+// which belong to neighbours -- go to a per-tile slot list, and k_dep_faces adds, for every own
+// face node, the neighbours' slots in a fixed order.  A chunked tile's chunks write their whole LDS
+// grid as int64 and k_dep_merge adds them exactly, in chunk order, before the same split.  The
+// result is bit-reproducible and needs no cell table, no zeroing pass and no float atomics.
+// Each thread takes a contiguous share of the run; the particles of one (cell, half-cell octant) all
+// reach the same 8 + 8 nodes, so a thread sums consecutive ones in registers and adds one LDS atomic
+// per node and run (dep_range).
+constexpr int kDepPer = kInterpChunk / 256;        // at most 16 particles per thread
+static_assert((int64_t)kDepPer << kFixBits <= (int64_t)1 << 30, "a thread's run sum fits an int32");
+constexpr int kD1 = kTile + 1;                     // node-grid nodes 0..8 per axis
+constexpr int kD2 = kTile + 2;                     // centres of cells -1..8 per axis (index m + 1)
+constexpr int kDN1 = kD1 * kD1 * kD1;              // 729
+constexpr int kDN2 = kD2 * kD2 * kD2;              // 1000
+constexpr int kDepB1 = kTile * kTile * kTile - (kTile - 1) * (kTile - 1) * (kTile - 1);  // 169 own face nodes
+constexpr int kDepH1 = kDN1 - kTile * kTile * kTile;                                      // 217 beyond the faces
+constexpr int kDepB2 = kTile * kTile * kTile - (kTile - 2) * (kTile - 2) * (kTile - 2);  // 296
+constexpr int kDepH2 = kDN2 - kTile * kTile * kTile;                                      // 488
+constexpr int kDepSlots1 = kDepB1 + kDepH1;
+constexpr int kDepSlots = kDepSlots1 + kDepB2 + kDepH2;  // 1170 per tile (node grid only: kDepSlots1)
+
+// Slots of a tile's nodes that other tiles also reach, in three slabs per grid so that both
+// directions are arithmetic.  Node grid (LDS coordinates 0..8, interior 1..7 per axis): z in {0, 8}
+// (2 x 81), then y in {0, 8} for z in 1..7 (2 x 63), then x in {0, 8} for y, z in 1..7 (2 x 49).
+// Interlaced grid (LDS coordinates 0..9 = centres of cells -1..8, interior 2..7): z in {0, 1, 8, 9}
+// (4 x 100), then y in {0, 1, 8, 9} for z in 2..7 (4 x 60), then x in {0, 1, 8, 9} for y, z in 2..7
+// (4 x 36), after the node grid's kDepSlots1.  -1: a node only this tile reaches.
+__host__ __device__ inline int dep_slot1(int x, int y, int z) {
+    if (z == 0 || z == kD1 - 1) return (z ? 81 : 0) + y * kD1 + x;
+    if (y == 0 || y == kD1 - 1) return 162 + (y ? 63 : 0) + (z - 1) * kD1 + x;
+    if (x == 0 || x == kD1 - 1) return 288 + (x ? 49 : 0) + (z - 1) * 7 + (y - 1);
+    return -1;
+}
+__host__ __device__ inline void dep_node1(int sl, int& x, int& y, int& z) {
+    if (sl < 162) {
+        z = sl < 81 ? 0 : kD1 - 1; y = (sl % 81) / kD1; x = sl % kD1;
+    } else if (sl < 288) {
+        const int q = sl - 162;
+        y = q < 63 ? 0 : kD1 - 1; z = 1 + (q % 63) / kD1; x = q % kD1;
+    } else {
+        const int q = sl - 288;
+        x = q < 49 ? 0 : kD1 - 1; z = 1 + (q % 49) / 7; y = 1 + q % 7;
+    }
+}
+__host__ __device__ inline int dep_q2(int c) { return c < 2 ? c : c - (kD2 - 4); }  // 0, 1, 8, 9 -> 0..3
+__host__ __device__ inline int dep_c2(int q) { return q < 2 ? q : q + (kD2 - 4); }  // 0..3 -> 0, 1, 8, 9
+__host__ __device__ inline bool dep_edge2(int c) { return c < 2 || c >= kD2 - 2; }
+__host__ __device__ inline int dep_slot2(int x, int y, int z) {
+    if (dep_edge2(z)) return kDepSlots1 + dep_q2(z) * 100 + y * kD2 + x;
+    if (dep_edge2(y)) return kDepSlots1 + 400 + dep_q2(y) * 60 + (z - 2) * kD2 + x;
+    if (dep_edge2(x)) return kDepSlots1 + 640 + dep_q2(x) * 36 + (z - 2) * 6 + (y - 2);
+    return -1;
+}
+__host__ __device__ inline void dep_node2(int sl, int& x, int& y, int& z) {
+    const int q = sl - kDepSlots1;
+    if (q < 400) {
+        z = dep_c2(q / 100); y = (q % 100) / kD2; x = q % kD2;
+    } else if (q < 640) {
+        const int r = q - 400;
+        y = dep_c2(r / 60); z = 2 + (r % 60) / kD2; x = r % kD2;
+    } else {
+        const int r = q - 640;
+        x = dep_c2(r / 36); z = 2 + (r % 36) / 6; y = 2 + r % 6;
+    }
+}
+static_assert(162 + 126 + 98 == kDepSlots1 && 400 + 240 + 144 == kDepB2 + kDepH2, "slab sizes");
+
+// tile t's origin cell, from its Morton index
+__device__ inline void tile_origin(uint32_t t, int32_t o[3]) {
+    const uint64_t base = (uint64_t)t << 9;
+    o[0] = (int32_t)compact3_21(base);
+    o[1] = (int32_t)compact3_21(base >> 1);
+    o[2] = (int32_t)compact3_21(base >> 2);
+}
+
+__device__ inline float dep_value(unsigned long long v, double unit) { return (float)((double)(long long)v * unit); }
+
+// The CIC contributions of one particle as int32 fixed point, v[0..7] for the node grid (corner
+// (dx, dy, dz) = (c & 1, c >> 1 & 1, c >> 2) from its cell's lower corner) and v[8..15] for the
+// interlaced grid (same corner order from the centre of cell (c - 1 + h) per axis, h = 1 if the particle
+// is in the upper half of its cell); key = its (cell, half-cell octant), whose particles all reach the
+// same 8 + 8 nodes; n1, n2 = the LDS index of the lower corner node in each grid.
 template <bool BOTH>
-__global__ __launch_bounds__(256) void k_deposit_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
-                                                      const int32_t* __restrict__ slot,
-                                                      const float* __restrict__ corner8,
-                                                      const float* __restrict__ corner27, int logG, float qp,
-                                                      float qm, double unit, float* __restrict__ rho,
-                                                      float* __restrict__ rho2) {
-    // every node the shell's particles reach, so that the corner offsets are compile-time constants
-    // (LDS instruction offsets) and no contribution needs a bounds test; only the tile's own nodes
-    // are stored at the end
-    __shared__ int32_t a1[kN1 * kN1 * kN1];           // node-grid nodes -1 .. 9, index l + 1
-    __shared__ int32_t a2[BOTH ? kN2 * kN2 * kN2 : 1];  // centres of cells -2 .. 9, index l + 2
-    __shared__ int32_t cst[kShellCells];      // particle start of a sparse shell cell; -1 - slot of a dense one
-    __shared__ int32_t pre[kShellCells + 1];  // prefix of the sparse cells' counts
-    __shared__ int32_t wsum[4];
-    __shared__ int32_t any_dense;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+__device__ inline void dep_contrib(const uint4 r, const int32_t o[3], int sh, int logG, float qp, float qm, int v[16],
+                                   int& key, int& n1, int& n2) {
+    const int cx = (int)(((r.x >> sh) - (uint32_t)o[0]) & (kTile - 1));
+    const int cy = (int)(((r.y >> sh) - (uint32_t)o[1]) & (kTile - 1));
+    const int cz = (int)(((r.z >> sh) - (uint32_t)o[2]) & (kTile - 1));
+    const float fx = frac_of<float>(r.x, logG), fy = frac_of<float>(r.y, logG), fz = frac_of<float>(r.z, logG);
+    const float q = (int32_t)r.w > 0 ? qp : qm;
+    const bool hx = fx >= 0.5f, hy = fy >= 0.5f, hz = fz >= 0.5f;
+    {
+        const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {q * (1.0f - fz), q * fz};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = __float2int_rn(wz[c >> 2] * wy[(c >> 1) & 1] * wx[c & 1]);
+    }
+    if (BOTH) {
+        const float vx[2] = {hx ? 1.5f - fx : 0.5f - fx, hx ? fx - 0.5f : 0.5f + fx};
+        const float vy[2] = {hy ? 1.5f - fy : 0.5f - fy, hy ? fy - 0.5f : 0.5f + fy};
+        const float vz[2] = {q * (hz ? 1.5f - fz : 0.5f - fz), q * (hz ? fz - 0.5f : 0.5f + fz)};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[8 + c] = __float2int_rn(vz[c >> 2] * vy[(c >> 1) & 1] * vx[c & 1]);
+    }
+    key = ((cz * kTile + cy) * kTile + cx) << 3 | (hz ? 4 : 0) | (hy ? 2 : 0) | (hx ? 1 : 0);
+    n1 = (cz * kD1 + cy) * kD1 + cx;
+    // the centre of cell m has index m + 1; the lower node is the centre of cell c - 1 + h
+    n2 = kDN1 + ((cz + (hz ? 1 : 0)) * kD2 + cy + (hy ? 1 : 0)) * kD2 + cx + (hx ? 1 : 0);
+}
+
+// Adds one run's summed contributions to the tile's LDS sums.
+template <bool BOTH>
+__device__ inline void dep_flush(unsigned long long* a, const int acc[16], int n1, int n2) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int d = ((c >> 2) * kD1 + ((c >> 1) & 1)) * kD1 + (c & 1);
+        atomicAdd(&a[n1 + d], (unsigned long long)(long long)acc[c]);
+    }
+    if (BOTH) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int d = ((c >> 2) * kD2 + ((c >> 1) & 1)) * kD2 + (c & 1);
+            atomicAdd(&a[n2 + d], (unsigned long long)(long long)acc[8 + c]);
+        }
+    }
+}
+
+// A thread's contiguous share [p, pe) of a tile's particles (at most kDepPer): the contributions of
+// consecutive particles of one (cell, octant) run are summed in registers (int32: at most
+// kDepPer x 2^22) and added to LDS once per run, so a halo core costs one LDS atomic per node and
+// kDepPer particles, and the lanes of a wavefront, kDepPer particles apart in Morton order, seldom
+// meet at one LDS address.  Loads go four at a time.
+template <bool BOTH>
+__device__ inline void dep_range(const uint4* __restrict__ parts, int32_t p, int32_t pe, const int32_t o[3], int sh,
+                                 int logG, float qp, float qm, unsigned long long* a) {
+    int acc[16];
+    int cur = -1, c1 = 0, c2 = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0;
+    for (int32_t q0 = p; q0 < pe; q0 += 4) {
+        uint4 r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = parts[q0 + u < pe ? q0 + u : p];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (q0 + u >= pe) break;
+            int v[16], key, n1, n2;
+            dep_contrib<BOTH>(r[u], o, sh, logG, qp, qm, v, key, n1, n2);
+            if (key != cur) {
+                if (cur >= 0) dep_flush<BOTH>(a, acc, c1, c2);
+                cur = key;
+                c1 = n1;
+                c2 = n2;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc[k] = v[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc[k] += v[k];
+            }
+        }
+    }
+    if (cur >= 0) dep_flush<BOTH>(a, acc, c1, c2);
+}
+
+// Stores a tile's sums: the nodes only it reaches straight into the grids (node grid 1..7, interlaced
+// centres 1..6 per axis), the others to its slots.
+template <bool BOTH>
+__device__ inline void dep_store_tile(const unsigned long long* a, const int32_t o[3],
+                                      int logG, double unit, float* __restrict__ rho, float* __restrict__ rho2,
+                                      float* __restrict__ tb) {
+    const int tid = threadIdx.x;
     const int64_t G = 1ll << logG;
     const int32_t msk = (int32_t)G - 1;
-    const unsigned ntile = gridDim.x;
-    const unsigned tile = (blockIdx.x & 7u) * (ntile >> 3) + (blockIdx.x >> 3);
-    const uint64_t base = (uint64_t)tile << 9;  // Morton index of the tile's first cell
-    const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
-                  oz = (int32_t)compact3_21(base >> 2);
-    for (int i = tid; i < kN1 * kN1 * kN1; i += 256) a1[i] = 0;
+    constexpr int I1 = kTile - 1, I2 = kTile - 2;
+    for (int k = tid; k < I1 * I1 * I1; k += 256) {
+        const int x = 1 + k % I1, y = 1 + (k / I1) % I1, z = 1 + k / (I1 * I1);
+        rho[((int64_t)(o[2] + z) * G + (o[1] + y)) * G + (o[0] + x)] = dep_value(a[(z * kD1 + y) * kD1 + x], unit);
+    }
     if (BOTH)
-        for (int i = tid; i < kN2 * kN2 * kN2; i += 256) a2[i] = 0;
-    if (tid == 0) any_dense = 0;
-    // shell cells 4 tid .. 4 tid + 3 (s = (cz + 1) 100 + (cy + 1) 10 + cx + 1)
-    int cnt[4] = {0, 0, 0, 0};
-    bool dense = false;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int sc = 4 * tid + u;
-        if (sc >= kShellCells) continue;
-        const int cx = sc % kShell - 1, cy = (sc / kShell) % kShell - 1, cz = sc / (kShell * kShell) - 1;
-        const uint64_t cm = spread3_21((uint32_t)((ox + cx) & msk)) | (spread3_21((uint32_t)((oy + cy) & msk)) << 1) |
-                            (spread3_21((uint32_t)((oz + cz) & msk)) << 2);
-        const int32_t st = cs[cm], k = cs[cm + 1] - st;
-        if (k <= kDenseCell) {
-            cst[sc] = st;
-            cnt[u] = k;
+        for (int k = tid; k < I2 * I2 * I2; k += 256) {
+            const int mx = 1 + k % I2, my = 1 + (k / I2) % I2, mz = 1 + k / (I2 * I2);  // centre of cell m
+            const float val = dep_value(a[kDN1 + ((mz + 1) * kD2 + my + 1) * kD2 + mx + 1], unit);
+            rho2[((int64_t)((o[2] + mz + 1) & msk) * G + ((o[1] + my + 1) & msk)) * G + ((o[0] + mx + 1) & msk)] = val;
+        }
+    for (int sl = tid; sl < (BOTH ? kDepSlots : kDepSlots1); sl += 256) {
+        int x, y, z;
+        if (sl < kDepSlots1) {
+            dep_node1(sl, x, y, z);
+            tb[sl] = dep_value(a[(z * kD1 + y) * kD1 + x], unit);
         } else {
-            cst[sc] = -1 - slot[cm];
-            dense = true;
+            dep_node2(sl, x, y, z);
+            tb[sl] = dep_value(a[kDN1 + (z * kD2 + y) * kD2 + x], unit);
         }
     }
-    // block exclusive scan of the sparse counts in shell-cell order
-    const int mine = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-    int incl = mine;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
+}
+
+// Work items: block b < ntile takes tile (b & 7) ntile / 8 + b / 8 (contiguous Morton ranges per
+// XCD) if it holds at most kInterpChunk particles; block ntile + e takes chunk e of the chunk list.
+template <bool BOTH>
+__global__ __launch_bounds__(256) void k_dep_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ ts,
+                                                  const int2* __restrict__ chunks, const int32_t* __restrict__ nchunk,
+                                                  int32_t ntile, int logG, float qp, float qm, double unit,
+                                                  float* __restrict__ rho, float* __restrict__ rho2,
+                                                  float* __restrict__ tbuf, long long* __restrict__ cbuf) {
+    constexpr int NN = kDN1 + (BOTH ? kDN2 : 0);
+    constexpr int NS = BOTH ? kDepSlots : kDepSlots1;
+    __shared__ unsigned long long a[NN];
+    const uint32_t b = blockIdx.x;
+    int32_t t, pbeg, pend, e = -1;
+    if (b < (uint32_t)ntile) {
+        t = (int32_t)((b & 7u) * ((uint32_t)ntile >> 3) + (b >> 3));
+        pbeg = ts[t];
+        pend = ts[t + 1];
+        if (pend - pbeg > kInterpChunk) return;  // chunked: the chunk blocks take it
+    } else {
+        e = (int32_t)(b - (uint32_t)ntile);
+        if (e >= *nchunk) return;
+        const int2 w = chunks[e];
+        t = w.x;
+        pbeg = w.y;
+        pend = min(ts[t + 1], pbeg + kInterpChunk);
     }
-    if (lane == 63) wsum[wv] = incl;
-    __syncthreads();  // also orders the zeroing before the atomics below
-    if (dense) any_dense = 1;
-    int woff = 0;
-    for (int w = 0; w < wv; ++w) woff += wsum[w];
-    int run = woff + incl - mine;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int sc = 4 * tid + u;
-        if (sc < kShellCells) pre[sc] = run;
-        run += cnt[u];
-    }
-    if (tid == 0) pre[kShellCells] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NN; i += 256) a[i] = 0ull;
+    int32_t o[3];
+    tile_origin((uint32_t)t, o);
+    const int sh = 32 - logG;
     __syncthreads();
-    const int32_t total = pre[kShellCells];
-    const int32_t per = (total + 255) >> 8;
-    int32_t t = tid * per;
-    const int32_t t_end = min(total, t + per);
-    if (t < t_end) {
-        int lo = 0, hi = kShellCells - 1;  // last shell cell with pre <= t
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pre[mid] <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        int32_t cell_end = pre[lo + 1];
-        for (; t < t_end; ++t) {
-            while (t >= cell_end) cell_end = pre[++lo + 1];
-            const int cx = lo % kShell - 1, cy = (lo / kShell) % kShell - 1, cz = lo / (kShell * kShell) - 1;
-            const uint4 r = parts[cst[lo] + (t - pre[lo])];
-            const float fx = frac_of<float>(r.x, logG), fy = frac_of<float>(r.y, logG), fz = frac_of<float>(r.z, logG);
-            const float q = (int32_t)r.w > 0 ? qp : qm;
-            {
-                const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {q * (1.0f - fz), q * fz};
-                int32_t* a = &a1[((cz + 1) * kN1 + cy + 1) * kN1 + cx + 1];
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                    atomicAdd(a + (dz * kN1 + dy) * kN1 + dx, __float2int_rn(wz[dz] * wy[dy] * wx[dx]));
-                }
-            }
-            if (BOTH) {
-                // the two interlaced nodes around the particle per axis: centres of cells (c-1, c) if
-                // f < 1/2, else (c, c+1); the centre of cell m has index m + 2
-                const bool lox = fx < 0.5f, loy = fy < 0.5f, loz = fz < 0.5f;
-                const int bx = cx + (lox ? 1 : 2), by = cy + (loy ? 1 : 2), bz = cz + (loz ? 1 : 2);
-                const float vx[2] = {lox ? 0.5f - fx : 1.5f - fx, lox ? 0.5f + fx : fx - 0.5f};
-                const float vy[2] = {loy ? 0.5f - fy : 1.5f - fy, loy ? 0.5f + fy : fy - 0.5f};
-                const float vz[2] = {q * (loz ? 0.5f - fz : 1.5f - fz), q * (loz ? 0.5f + fz : fz - 0.5f)};
-                int32_t* a = &a2[(bz * kN2 + by) * kN2 + bx];
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-                    atomicAdd(a + (dz * kN2 + dy) * kN2 + dx, __float2int_rn(vz[dz] * vy[dy] * vx[dx]));
-                }
-            }
-        }
+    {
+        const int32_t per = (pend - pbeg + 255) >> 8;  // <= kInterpChunk / 256
+        const int32_t p = pbeg + tid * per;
+        dep_range<BOTH>(parts, p, min(pend, p + per), o, sh, logG, qp, qm, a);
     }
     __syncthreads();
-    // node (x, y, z) of the tile: the node grid's node at cell corner o + l, the interlaced grid's
-    // node at the centre of cell o + l; the dense cells around it add their corner sums in a fixed
-    // order (node grid: cells l - e, e in {0,1}^3; interlaced: cells l - e, e in {-1,0,1}^3)
-    const bool dn = any_dense != 0;
-    for (int i = tid; i < kTile * kTile * kTile; i += 256) {
-        const int lx = i % kTile, ly = (i / kTile) % kTile, lz = i / (kTile * kTile);
-        double v1 = (double)a1[((lz + 1) * kN1 + ly + 1) * kN1 + lx + 1];
-        if (dn) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {  // cell l - e (shell index l - e + 1), its corner e
-                const int ex = q & 1, ey = (q >> 1) & 1, ez = q >> 2;
-                const int32_t c = cst[((lz - ez + 1) * kShell + ly - ey + 1) * kShell + lx - ex + 1];
-                if (c < 0) v1 += (double)corner8[8 * (int64_t)(-1 - c) + q];
-            }
-        }
-        rho[((int64_t)(oz + lz) * G + (oy + ly)) * G + (ox + lx)] = (float)(v1 * unit);
-        if (BOTH) {
-            double v2 = (double)a2[((lz + 2) * kN2 + ly + 2) * kN2 + lx + 2];
-            if (dn) {
-                for (int q = 0; q < 27; ++q) {  // cell l - e, e = (q % 3, q / 3 % 3, q / 9) - 1: its entry q
-                    const int ex = q % 3 - 1, ey = (q / 3) % 3 - 1, ez = q / 9 - 1;
-                    const int32_t c = cst[((lz - ez + 1) * kShell + ly - ey + 1) * kShell + lx - ex + 1];
-                    if (c < 0) v2 += (double)corner27[27 * (int64_t)(-1 - c) + q];
-                }
-            }
-            rho2[((int64_t)((oz + lz + 1) & msk) * G + ((oy + ly + 1) & msk)) * G + ((ox + lx + 1) & msk)] =
-                (float)(v2 * unit);
+    if (e >= 0) {  // a chunk: its whole grid, exact
+        long long* out = cbuf + (int64_t)e * NN;
+        for (int i = tid; i < NN; i += 256) out[i] = (long long)a[i];
+        return;
+    }
+    dep_store_tile<BOTH>(a, o, logG, unit, rho, rho2, tbuf + (int64_t)t * NS);
+}
+
+// The chunks of a chunked tile summed in chunk order (exact), then stored like a whole tile: the
+// tile's nc chunk blocks split its nodes into nc slices, block k summing slice k over all chunks.
+template <bool BOTH>
+__global__ __launch_bounds__(256) void k_dep_merge(const int32_t* __restrict__ ts, const int2* __restrict__ chunks,
+                                                   const int32_t* __restrict__ nchunk, int logG, double unit,
+                                                   const long long* __restrict__ cbuf, float* __restrict__ rho,
+                                                   float* __restrict__ rho2, float* __restrict__ tbuf) {
+    constexpr int NN = kDN1 + (BOTH ? kDN2 : 0);
+    constexpr int NS = BOTH ? kDepSlots : kDepSlots1;
+    const int32_t e = (int32_t)blockIdx.x;
+    if (e >= *nchunk) return;
+    const int2 w = chunks[e];
+    const int32_t t = w.x, beg = ts[t];
+    const int32_t k = (w.y - beg) / kInterpChunk;  // this chunk's index in its tile
+    const int32_t nc = (ts[t + 1] - beg + kInterpChunk - 1) / kInterpChunk;
+    const int32_t first = e - k;  // the tile's chunks are consecutive entries
+    const int32_t per = (NN + nc - 1) / nc;
+    const int32_t n0 = k * per, n1 = min(NN, n0 + per);
+    int32_t o[3];
+    tile_origin((uint32_t)t, o);
+    const int64_t G = 1ll << logG;
+    const int32_t msk = (int32_t)G - 1;
+    float* tb = tbuf + (int64_t)t * NS;
+    for (int32_t n = n0 + (int32_t)threadIdx.x; n < n1; n += 256) {
+        long long acc = 0;
+        for (int32_t j = 0; j < nc; ++j) acc += cbuf[(int64_t)(first + j) * NN + n];
+        const float val = (float)((double)acc * unit);
+        if (n < kDN1) {
+            const int x = n % kD1, y = (n / kD1) % kD1, z = n / (kD1 * kD1);
+            const int sl = dep_slot1(x, y, z);
+            if (sl >= 0) tb[sl] = val;
+            else rho[((int64_t)(o[2] + z) * G + (o[1] + y)) * G + (o[0] + x)] = val;
+        } else {
+            const int jn = n - kDN1;  // interlaced node at the centre of cell m, index m + 1, stored at m + 1
+            const int x = jn % kD2, y = (jn / kD2) % kD2, z = jn / (kD2 * kD2);
+            const int sl = dep_slot2(x, y, z);
+            if (sl >= 0) tb[sl] = val;
+            else rho2[((int64_t)((o[2] + z) & msk) * G + ((o[1] + y) & msk)) * G + ((o[0] + x) & msk)] = val;
         }
     }
+}
+
+// Own face nodes of one category: M* per axis 0 (interior coordinate), 1 (coordinate 0, which the
+// neighbour below also reaches: its local 8) or 2 (interlaced only: centre of cell 7, which the
+// neighbour above also reaches: its centre -1).  The value is the tile's own slot plus, for every
+// non-empty set of the category's face axes in a fixed order, the slot of the neighbour across them;
+// everything but the node coordinates is fixed at compile time.
+template <bool G2, int NS, int MX, int MY, int MZ>
+__device__ inline void dep_face_cat(const float* __restrict__ tbuf, uint32_t t, const uint32_t* nbr,
+                                    const int32_t o[3], int logG, float* __restrict__ grid) {
+    constexpr int NF = G2 ? kTile - 2 : kTile - 1;  // interior coordinates per axis: 1 .. NF
+    constexpr int M[3] = {MX, MY, MZ};
+    constexpr int nfree = (MX == 0) + (MY == 0) + (MZ == 0);
+    constexpr int count = nfree == 2 ? NF * NF : nfree == 1 ? NF : 1;
+    static_assert(nfree < 3, "a face category has at least one face axis");
+    const int64_t G = 1ll << logG;
+    const int32_t msk = (int32_t)G - 1;
+    for (int k = threadIdx.x; k < count; k += 256) {
+        int r = k, l[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (M[ax] == 0) {
+                l[ax] = 1 + r % NF;
+                r /= NF;
+            } else {
+                l[ax] = M[ax] == 1 ? 0 : kTile - 1;
+            }
+        }
+        const int own = G2 ? dep_slot2(l[0] + 1, l[1] + 1, l[2] + 1) : dep_slot1(l[0], l[1], l[2]);
+        float v[8];
+        v[0] = tbuf[(int64_t)t * NS + own];
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {
+            v[c] = 0.0f;
+            if (((c & 1) && MX == 0) || ((c & 2) && MY == 0) || ((c & 4) && MZ == 0)) continue;
+            const int dx = (c & 1) ? (MX == 1 ? -1 : 1) : 0;
+            const int dy = (c & 2) ? (MY == 1 ? -1 : 1) : 0;
+            const int dz = (c & 4) ? (MZ == 1 ? -1 : 1) : 0;
+            const uint32_t nt = nbr[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+            const int ax = l[0] - kTile * dx, ay = l[1] - kTile * dy, az = l[2] - kTile * dz;
+            const int sl = G2 ? dep_slot2(ax + 1, ay + 1, az + 1) : dep_slot1(ax, ay, az);
+            v[c] = tbuf[(int64_t)nt * NS + sl];
+        }
+        const float sum = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        const int64_t at = G2 ? ((int64_t)((o[2] + l[2] + 1) & msk) * G + ((o[1] + l[1] + 1) & msk)) * G +
+                                    ((o[0] + l[0] + 1) & msk)
+                              : ((int64_t)(o[2] + l[2]) * G + (o[1] + l[1])) * G + (o[0] + l[0]);
+        grid[at] = sum;
+    }
+}
+
+// tbuf holds NS slots per tile (both grids' slots when interlacing, else the node grid's)
+template <bool G2, int NS, int C>
+__device__ inline void dep_face_cats(const float* __restrict__ tbuf, uint32_t t, const uint32_t* nbr,
+                                     const int32_t o[3], int logG, float* __restrict__ grid) {
+    // categories C = MX + 3 MY + 9 MZ (base 3; the node grid uses digits 0 and 1 only)
+    if constexpr (C < 27) {
+        constexpr int MX = C % 3, MY = (C / 3) % 3, MZ = C / 9;
+        if constexpr (C != 0 && (G2 || (MX < 2 && MY < 2 && MZ < 2)))
+            dep_face_cat<G2, NS, MX, MY, MZ>(tbuf, t, nbr, o, logG, grid);
+        dep_face_cats<G2, NS, C + 1>(tbuf, t, nbr, o, logG, grid);
+    }
+}
+
+// The own face nodes of every tile (k_dep_faces): node grid 169, interlaced 296 per tile.
+template <bool BOTH>
+__global__ __launch_bounds__(256) void k_dep_faces(const float* __restrict__ tbuf, int logG, float* __restrict__ rho,
+                                                   float* __restrict__ rho2) {
+    __shared__ uint32_t nbr[27];  // tile index at offset (dx, dy, dz), index (dz + 1) 9 + (dy + 1) 3 + dx + 1
+    const uint32_t ntile = gridDim.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = (b & 7u) * (ntile >> 3) + (b >> 3);
+    const int32_t tm = (1 << (logG - 3)) - 1;  // tiles per axis - 1
+    int32_t o[3];
+    tile_origin(t, o);
+    if (threadIdx.x < 27) {
+        const int dx = (int)threadIdx.x % 3 - 1, dy = ((int)threadIdx.x / 3) % 3 - 1, dz = (int)threadIdx.x / 9 - 1;
+        nbr[threadIdx.x] = (uint32_t)(spread3_21((uint32_t)(((o[0] >> 3) + dx) & tm)) |
+                                      (spread3_21((uint32_t)(((o[1] >> 3) + dy) & tm)) << 1) |
+                                      (spread3_21((uint32_t)(((o[2] >> 3) + dz) & tm)) << 2));
+    }
+    __syncthreads();
+    constexpr int NS = BOTH ? kDepSlots : kDepSlots1;
+    dep_face_cats<false, NS, 0>(tbuf, t, nbr, o, logG, rho);
+    if constexpr (BOTH) dep_face_cats<true, NS, 0>(tbuf, t, nbr, o, logG, rho2);
 }
 
 // Thread per grid node, nodes enumerated in Morton order so that the eight
@@ -856,13 +1089,27 @@ __global__ __launch_bounds__(256) void k_fft_z_inv(Cplx<R>* __restrict__ spec, c
     }
 }
 
+// One Poisson solve's device buffers and constants: the context's, or the scratch of the
+// allocation-free entry point.
+struct FftPlan {
+    void* spec;           // half spectrum, complex [G][G][G/2+1]
+    const void* twiddle;  // complex [G]
+    const void* gtab;     // real [4 G]: T, k^2, k, stencil transfer D per axis index
+    double green_norm;
+    int logG, k_min, tile, precision;
+};
+
+static FftPlan plan_of(const Ctx& c) {
+    return FftPlan{c.spec, c.twiddle, c.gtab, c.green_norm, c.logG, c.p.k_min, c.p.fft_tile, c.p.precision};
+}
+
 template <typename R, int LG, int LP>
-static hipError_t inverse_fft_lg(Ctx& c, R* out, hipStream_t s) {
+static hipError_t inverse_fft_lg(const FftPlan& f, R* out, hipStream_t s) {
     constexpr int G = 1 << LG, H = G / 2 + 1, NPT = 1 << LP;
     constexpr int KB = NPT / G, nkb = (H + KB - 1) / KB;
     constexpr unsigned nx = (unsigned)(G * G > 2 * (NPT / G) ? (int64_t)G * G / (2 * (NPT / G)) : 1);
-    const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
-    Cplx<R>* spec = (Cplx<R>*)c.spec;
+    const Cplx<R>* tw = (const Cplx<R>*)f.twiddle;
+    Cplx<R>* spec = (Cplx<R>*)f.spec;
     k_fft_z_inv<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw);
     k_fft_y<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
     k_fft_x_c2r<R, LG, LP><<<dim3(nx), dim3(256), 0, s>>>(spec, out, tw);
@@ -870,16 +1117,16 @@ static hipError_t inverse_fft_lg(Ctx& c, R* out, hipStream_t s) {
 }
 
 template <typename R, int LG, int LP>
-static hipError_t fft_solve_lg(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
+static hipError_t fft_solve_lg(const FftPlan& f, const R* rho, R* phi, hipStream_t s, int matched) {
     constexpr int G = 1 << LG, H = G / 2 + 1, NPT = 1 << LP;
     constexpr int KB = NPT / G, nkb = (H + KB - 1) / KB;
     constexpr unsigned nx = (unsigned)(G * G > 2 * (NPT / G) ? (int64_t)G * G / (2 * (NPT / G)) : 1);
-    const Cplx<R>* tw = (const Cplx<R>*)c.twiddle;
-    Cplx<R>* spec = (Cplx<R>*)c.spec;
+    const Cplx<R>* tw = (const Cplx<R>*)f.twiddle;
+    Cplx<R>* spec = (Cplx<R>*)f.spec;
     k_fft_x_r2c<R, LG, LP><<<dim3(nx), dim3(256), 0, s>>>(rho, spec, tw);
     k_fft_y<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 0);
-    k_fft_z_green<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, (const R*)c.gtab,
-                                                                         (R)c.green_norm, c.p.k_min, matched);
+    k_fft_z_green<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, (const R*)f.gtab,
+                                                                         (R)f.green_norm, f.k_min, matched);
     k_fft_y<R, LG, LP><<<dim3((unsigned)(G * nkb)), dim3(256), 0, s>>>(spec, tw, 1);
     k_fft_x_c2r<R, LG, LP><<<dim3(nx), dim3(256), 0, s>>>(spec, phi, tw);
     return hipGetLastError();
@@ -887,7 +1134,7 @@ static hipError_t fft_solve_lg(Ctx& c, const R* rho, R* phi, hipStream_t s, int 
 
 // the mesh sizes the context accepts (G = 16 .. 512) and FFT tiles (2^11, 2^12 points; 2^13 for FP32)
 #define JTP_FFT_DISPATCH_LG(CALL, LP)          \
-    switch (c.logG) {                          \
+    switch (f.logG) {                          \
         case 4: return CALL(4, LP);            \
         case 5: return CALL(5, LP);            \
         case 6: return CALL(6, LP);            \
@@ -898,39 +1145,87 @@ static hipError_t fft_solve_lg(Ctx& c, const R* rho, R* phi, hipStream_t s, int 
     }
 
 template <typename R>
-static hipError_t inverse_fft_t(Ctx& c, R* out, hipStream_t s) {
-#define JTP_INV(LG, LP) inverse_fft_lg<R, LG, LP>(c, out, s)
+static hipError_t inverse_fft_t(const FftPlan& f, R* out, hipStream_t s) {
+#define JTP_INV(LG, LP) inverse_fft_lg<R, LG, LP>(f, out, s)
     if constexpr (sizeof(R) == 4) {
-        if (c.p.fft_tile == 13) JTP_FFT_DISPATCH_LG(JTP_INV, 13)
+        if (f.tile == 13) JTP_FFT_DISPATCH_LG(JTP_INV, 13)
     }
-    if (c.p.fft_tile == 12) JTP_FFT_DISPATCH_LG(JTP_INV, 12)
+    if (f.tile == 12) JTP_FFT_DISPATCH_LG(JTP_INV, 12)
     JTP_FFT_DISPATCH_LG(JTP_INV, 11)
 #undef JTP_INV
 }
 
 hipError_t pm_inverse_fft(Ctx& c, void* out, hipStream_t s) {
-    if (c.p.precision == JTP_FP64) return inverse_fft_t<double>(c, (double*)out, s);
-    return inverse_fft_t<float>(c, (float*)out, s);
+    const FftPlan f = plan_of(c);
+    if (c.p.precision == JTP_FP64) return inverse_fft_t<double>(f, (double*)out, s);
+    return inverse_fft_t<float>(f, (float*)out, s);
 }
 
 template <typename R>
-static hipError_t fft_solve_t(Ctx& c, const R* rho, R* phi, hipStream_t s, int matched) {
-#define JTP_SOLVE(LG, LP) fft_solve_lg<R, LG, LP>(c, rho, phi, s, matched)
+static hipError_t fft_solve_t(const FftPlan& f, const R* rho, R* phi, hipStream_t s, int matched) {
+#define JTP_SOLVE(LG, LP) fft_solve_lg<R, LG, LP>(f, rho, phi, s, matched)
     if constexpr (sizeof(R) == 4) {
-        if (c.p.fft_tile == 13) JTP_FFT_DISPATCH_LG(JTP_SOLVE, 13)
+        if (f.tile == 13) JTP_FFT_DISPATCH_LG(JTP_SOLVE, 13)
     }
-    if (c.p.fft_tile == 12) JTP_FFT_DISPATCH_LG(JTP_SOLVE, 12)
+    if (f.tile == 12) JTP_FFT_DISPATCH_LG(JTP_SOLVE, 12)
     JTP_FFT_DISPATCH_LG(JTP_SOLVE, 11)
 #undef JTP_SOLVE
 }
 
+static hipError_t plan_solve(const FftPlan& f, const void* rho, void* phi, hipStream_t s, int matched) {
+    if (f.precision == JTP_FP64) return fft_solve_t<double>(f, (const double*)rho, (double*)phi, s, matched);
+    return fft_solve_t<float>(f, (const float*)rho, (float*)phi, s, matched);
+}
+
 hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched) {
     if (matched < 0) matched = c.p.deconvolve;  // the force's Green function
-    if (c.p.precision == JTP_FP64) return fft_solve_t<double>(c, (const double*)rho, (double*)phi, s, matched);
-    return fft_solve_t<float>(c, (const float*)rho, (float*)phi, s, matched);
+    return plan_solve(plan_of(c), rho, phi, s, matched);
+}
+
+// deposit chunks of tiles holding more than kInterpChunk particles: sum over those tiles of
+// ceil(np / kInterpChunk) <= 2 n / kInterpChunk
+int64_t dep_chunk_cap(int64_t n) { return 2 * n / kInterpChunk + 2; }
+int64_t dep_chunk_nodes() { return kDN1 + kDN2; }
+int64_t dep_tile_slots() { return kDepSlots; }
+
+static hipError_t deposit_fp32(Ctx& c, int64_t n, hipStream_t s) {
+    const int64_t G3 = c.G3();
+    const int32_t ntile = (int32_t)(G3 >> 9);  // 8^3-cell tiles (G >= 16: a multiple of 8)
+    int32_t* count = c.dev_counters + kCtrInterp;  // light tiles, interpolation chunks, deposit chunks
+    hipError_t e = launch_fill_i32(count, 3, 0, s);
+    if (e != hipSuccess) return e;
+    int32_t* light = (int32_t*)c.interp_list;
+    int2* heavy = c.interp_list + (ntile + 1) / 2;
+    k_tile_mark<<<dim3((unsigned)((ntile + 255) / 256)), dim3(256), 0, s>>>(c.tile_start, ntile, light, heavy,
+                                                                           c.dep_chunks, count);
+    // charges scaled by 2^k, |q| 2^k <= 2^kFixBits (exact: a power of two); the grids' unit is 2^-k
+    int ex = 0;
+    (void)std::frexp(c.m_plus > c.m_minus ? c.m_plus : c.m_minus, &ex);  // max |q| <= 2^ex
+    const int k = kFixBits - ex;
+    const float qp = (float)std::ldexp(c.m_plus, k), qm = (float)-std::ldexp(c.m_minus, k);
+    const double unit = std::ldexp(1.0, -k);
+    const unsigned nd = (unsigned)dep_chunk_cap(n);
+    float* r1 = (float*)c.rho;
+    float* r2 = (float*)c.rho2;
+    if (c.p.interlace) {
+        k_dep_tile<true><<<dim3((unsigned)ntile + nd), dim3(256), 0, s>>>(
+            c.parts, c.tile_start, c.dep_chunks, count + 2, ntile, c.logG, qp, qm, unit, r1, r2, c.dep_tbuf, c.dep_cbuf);
+        k_dep_merge<true><<<dim3(nd), dim3(256), 0, s>>>(c.tile_start, c.dep_chunks, count + 2, c.logG, unit,
+                                                         c.dep_cbuf, r1, r2, c.dep_tbuf);
+        k_dep_faces<true><<<dim3((unsigned)ntile), dim3(256), 0, s>>>(c.dep_tbuf, c.logG, r1, r2);
+    } else {
+        k_dep_tile<false><<<dim3((unsigned)ntile + nd), dim3(256), 0, s>>>(
+            c.parts, c.tile_start, c.dep_chunks, count + 2, ntile, c.logG, qp, qm, unit, r1, nullptr, c.dep_tbuf,
+            c.dep_cbuf);
+        k_dep_merge<false><<<dim3(nd), dim3(256), 0, s>>>(c.tile_start, c.dep_chunks, count + 2, c.logG, unit,
+                                                          c.dep_cbuf, r1, nullptr, c.dep_tbuf);
+        k_dep_faces<false><<<dim3((unsigned)ntile), dim3(256), 0, s>>>(c.dep_tbuf, c.logG, r1, nullptr);
+    }
+    return hipGetLastError();
 }
 
 hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
+    if (c.p.precision == JTP_FP32) return deposit_fp32(c, n, s);
     const int64_t G3 = c.G3();
     hipError_t e = launch_cell_start(c.keys0, n, 63 - 3 * c.logG, G3, c.cell_start, s);
     if (e != hipSuccess) return e;
@@ -944,49 +1239,22 @@ hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s) {
     unsigned nbd = (unsigned)((nd_max * 64 + kBlock - 1) / kBlock);
     if (nbd > 8192) nbd = 8192;
     const int64_t nd_cap = n / kDenseCell + 1;
-    if (c.p.precision == JTP_FP64) {
-        double* c8 = (double*)c.dense_corner;
-        double* c27 = c8 + 8 * nd_cap;
-        if (c.p.interlace) {
-            k_dense_reduce_both<double><<<dim3(nbd), dim3(256), 0, s>>>(
-                c.parts, c.cell_start, list, c.dense_slot, count, c.logG, c.m_plus, -c.m_minus, c8, c27);
-            k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
-            k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 27, c27);
-            k_deposit_both<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
-                                                                     c.logG, c.m_plus, -c.m_minus, (double*)c.rho,
-                                                                     (double*)c.rho2);
-        } else {
-            k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot,
-                                                                   count, c.logG, c.m_plus, -c.m_minus, c8);
-            k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
-            k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG,
-                                                                c.m_plus, -c.m_minus, (double*)c.rho);
-        }
+    double* c8 = (double*)c.dense_corner;
+    double* c27 = c8 + 8 * nd_cap;
+    if (c.p.interlace) {
+        k_dense_reduce_both<double><<<dim3(nbd), dim3(256), 0, s>>>(
+            c.parts, c.cell_start, list, c.dense_slot, count, c.logG, c.m_plus, -c.m_minus, c8, c27);
+        k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
+        k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 27, c27);
+        k_deposit_both<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
+                                                                 c.logG, c.m_plus, -c.m_minus, (double*)c.rho,
+                                                                 (double*)c.rho2);
     } else {
-        float* c8 = (float*)c.dense_corner;
-        float* c27 = c8 + 8 * nd_cap;
-        // charges scaled by 2^k, |q| 2^k <= 2^kFixBits (exact: a power of two); the grids' unit is 2^-k
-        int ex = 0;
-        (void)std::frexp(c.m_plus > c.m_minus ? c.m_plus : c.m_minus, &ex);  // max |q| <= 2^ex
-        const int k = kFixBits - ex;
-        const float qp = (float)std::ldexp(c.m_plus, k), qm = (float)-std::ldexp(c.m_minus, k);
-        const double unit = std::ldexp(1.0, -k);
-        const unsigned ntile = (unsigned)(G3 >> 9);  // 8^3-cell tiles (G >= 16: a multiple of 8)
-        if (c.p.interlace) {
-            k_dense_reduce_both<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot,
-                                                                       count, c.logG, qp, qm, c8, c27);
-            k_dense_combine<float><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
-            k_dense_combine<float><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 27, c27);
-            k_deposit_tile<true><<<dim3(ntile), dim3(256), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c27,
-                                                                   c.logG, qp, qm, unit, (float*)c.rho,
-                                                                   (float*)c.rho2);
-        } else {
-            k_dense_reduce<float><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot,
-                                                                  count, c.logG, qp, qm, c8);
-            k_dense_combine<float><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
-            k_deposit_tile<false><<<dim3(ntile), dim3(256), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, nullptr,
-                                                                    c.logG, qp, qm, unit, (float*)c.rho, nullptr);
-        }
+        k_dense_reduce<double><<<dim3(nbd), dim3(256), 0, s>>>(c.parts, c.cell_start, list, c.dense_slot, count,
+                                                               c.logG, c.m_plus, -c.m_minus, c8);
+        k_dense_combine<double><<<dim3(nbd), dim3(256), 0, s>>>(c.cell_start, list, c.dense_slot, count, 8, c8);
+        k_deposit<double><<<dim3(nb), dim3(kBlock), 0, s>>>(c.parts, c.cell_start, c.dense_slot, c8, c.logG,
+                                                            c.m_plus, -c.m_minus, (double*)c.rho);
     }
     return hipGetLastError();
 }
@@ -1177,53 +1445,39 @@ __device__ inline void tile_cic(const float4* g, int ix, int iy, int iz, float f
     a[2] = az;
 }
 
-// CIC sum over the 8 corners around local node (ix, iy, iz) with each corner's gradient taken from
-// the staged potential p (NP nodes per axis, node 0 at local -2) by the stencil: light tiles, where
-// computing every node's gradient first would cost more than it saves.
+// Gradient along the axis of stride s summed over the two CIC corners of that axis (weights w0, w1)
+// and bilinearly over the other two (strides s1, s2; weights u, v): the corners' stencils combine
+// into one filter over the potential's lines k = -2 .. 3 (4-point) or -1 .. 2 (2-point) of the
+// bilinear interpolation across the other axes.  Times the stencil's 1 / (12 h) (1 / (2 h)) later.
+template <bool FOUR>
+__device__ inline float tile_axis_sum(const float* p, int b, int s, int s1, int s2, float w0, float w1, float u0,
+                                      float u1, float v0, float v1) {
+    const float q00 = u0 * v0, q10 = u1 * v0, q01 = u0 * v1, q11 = u1 * v1;
+    auto line = [&](int k) {
+        const int o = b + k * s;
+        return q00 * p[o] + q10 * p[o + s1] + q01 * p[o + s2] + q11 * p[o + s1 + s2];
+    };
+    if (FOUR) {
+        const float lm2 = line(-2), lm1 = line(-1), l0 = line(0), l1 = line(1), l2 = line(2), l3 = line(3);
+        return w0 * (8.0f * (l1 - lm1) - (l2 - lm2)) + w1 * (8.0f * (l2 - l0) - (l3 - lm1));
+    }
+    const float lm1 = line(-1), l0 = line(0), l1 = line(1), l2 = line(2);
+    return w0 * (l1 - lm1) + w1 * (l2 - l0);
+}
+
+// CIC sum over the 8 corners around local node (ix, iy, iz) of each corner's stencil gradient of the
+// staged potential p (NP nodes per axis, node 0 at local -2), one axis at a time (tile_axis_sum):
+// light tiles, where computing every node's gradient first would cost more than it saves.
 template <bool FOUR, int NP>
 __device__ inline void tile_cic_phi(const float* p, int ix, int iy, int iz, float fx, float fy, float fz, float inv,
                                     float a[3]) {
-    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    const float wx0 = 1.0f - fx, wy0 = 1.0f - fy, wz0 = 1.0f - fz;
     const int b0 = ((iz + 2) * NP + iy + 2) * NP + ix + 2;
-    float ax = 0, ay = 0, az = 0;
-#pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const float w = wx[dx] * wy[dy] * wz[dz];
-                const int b = b0 + (dz * NP + dy) * NP + dx;
-                ax += w * lds_stencil<FOUR>(p, b, 1, inv);
-                ay += w * lds_stencil<FOUR>(p, b, NP, inv);
-                az += w * lds_stencil<FOUR>(p, b, NP * NP, inv);
-            }
-    a[0] = ax;
-    a[1] = ay;
-    a[2] = az;
+    a[0] = inv * tile_axis_sum<FOUR>(p, b0, 1, NP, NP * NP, wx0, fx, wy0, fy, wz0, fz);
+    a[1] = inv * tile_axis_sum<FOUR>(p, b0, NP, 1, NP * NP, wy0, fy, wx0, fx, wz0, fz);
+    a[2] = inv * tile_axis_sum<FOUR>(p, b0, NP * NP, 1, NP, wz0, fz, wx0, fx, wy0, fy);
 }
 
-// Tiles are classed by particle count: light tiles (1 .. kLightMax particles, kLightMax / 256 per thread) are
-// listed for k_interp_light; every chunk of kInterpChunk particles of a heavier tile (a halo core
-// holds 1e4-1e5) is listed, as (tile, first particle), for a k_interp_heavy workgroup.  The list
-// order varies between runs; each particle is still computed by one thread with the same
-// arithmetic, so the results do not.
-__global__ __launch_bounds__(256) void k_interp_mark(const int32_t* __restrict__ cs, int32_t ntile,
-                                                     int32_t* __restrict__ light, int2* __restrict__ heavy,
-                                                     int32_t* __restrict__ count) {
-    const int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    int32_t beg = 0, np = 0;
-    if (t < ntile) {
-        beg = cs[(int64_t)t << 9];
-        np = cs[((int64_t)t + 1) << 9] - beg;
-    }
-    const bool lt = np > 0 && np <= kLightMax;
-    const int32_t nh = np > kLightMax ? (np + kInterpChunk - 1) / kInterpChunk : 0;
-    const int at_l = wave_reserve(lt ? 1 : 0, count);
-    if (lt) light[at_l] = t;
-    const int at_h = wave_reserve(nh, count + 1);
-    for (int32_t k = 0; k < nh; ++k) heavy[at_h + k] = make_int2(t, beg + k * kInterpChunk);
-}
 
 // List entry of block b, or -1: the entries are dealt to the 8 XCDs in contiguous ranges (block b
 // runs on XCD b % 8), so neighbouring tiles share that XCD's L2.  Needs gridDim.x >= cnt + 7.
@@ -1244,7 +1498,7 @@ __device__ inline int cell2(uint32_t u, int sh, int32_t o, uint32_t msk) {
 constexpr int kHeavyThreads = 512;  // 8 waves per workgroup: 24 waves per CU at 3 workgroups
 
 template <bool FOUR, bool BOTH>
-__global__ __launch_bounds__(kHeavyThreads) void k_interp_heavy(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+__global__ __launch_bounds__(kHeavyThreads) void k_interp_heavy(const uint4* __restrict__ parts, const int32_t* __restrict__ ts,
                                                       const int2* __restrict__ list, const int32_t* __restrict__ count,
                                                       const float* __restrict__ phi, const float* __restrict__ phi2,
                                                       int logG, float inv, float* __restrict__ acc) {
@@ -1256,7 +1510,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_interp_heavy(const uint4* __r
     if (e < 0) return;  // the same for every thread of the block
     const int2 w = list[e];
     const uint64_t base = (uint64_t)w.x << 9;
-    const int32_t pbeg = w.y, pend = min(cs[base + 512], pbeg + kInterpChunk);
+    const int32_t pbeg = w.y, pend = min(ts[w.x + 1], pbeg + kInterpChunk);
     const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
                   oz = (int32_t)compact3_21(base >> 2);
     {
@@ -1297,7 +1551,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_interp_heavy(const uint4* __r
 // loads' latency), the particle is loaded together with the halo, and each corner's gradient is
 // taken from LDS directly.
 template <bool FOUR, bool BOTH>
-__global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ parts, const int32_t* __restrict__ cs,
+__global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ parts, const int32_t* __restrict__ ts,
                                                       const int32_t* __restrict__ list, const int32_t* __restrict__ count,
                                                       const float* __restrict__ phi, const float* __restrict__ phi2,
                                                       int logG, float inv, float* __restrict__ acc) {
@@ -1305,8 +1559,9 @@ __global__ __launch_bounds__(256) void k_interp_light(const uint4* __restrict__ 
     __shared__ float p2[BOTH ? kIp2 * kIp2 * kIp2 : 1];
     const int32_t e = xcd_entry(blockIdx.x, *count);
     if (e < 0) return;  // the same for every thread of the block
-    const uint64_t base = (uint64_t)list[e] << 9;
-    const int32_t pbeg = cs[base], np = cs[base + 512] - pbeg;  // 1 .. kLightMax
+    const int32_t tl = list[e];
+    const uint64_t base = (uint64_t)tl << 9;
+    const int32_t pbeg = ts[tl], np = ts[tl + 1] - pbeg;  // 1 .. kLightMax
     const int32_t ox = (int32_t)compact3_21(base), oy = (int32_t)compact3_21(base >> 1),
                   oz = (int32_t)compact3_21(base >> 2);
     constexpr int kPer = kLightMax / 256;  // particles per thread
@@ -1358,23 +1613,20 @@ static hipError_t interp_t(Ctx& c, int64_t n, hipStream_t s) {
     const bool four = c.p.gradient == JTP_GRAD_4PT;
     const R inv = (R)(four ? 1.0 / (12.0 * h) : 1.0 / (2.0 * h));
     if constexpr (sizeof(R) == 4) {
+        // the tile lists were built by the deposit of the same evaluation (k_tile_mark)
         const unsigned ntile = (unsigned)(c.G3() >> 9);  // 8^3-cell tiles (G >= 16)
-        int32_t* count = c.dev_counters + kCtrInterp;    // light tiles, heavy chunks
-        hipError_t e = launch_fill_i32(count, 2, 0, s);
-        if (e != hipSuccess) return e;
-        int32_t* light = (int32_t*)c.interp_list;  // ntile entries, then the heavy chunks
-        int2* heavy = c.interp_list + (ntile + 1) / 2;
-        k_interp_mark<<<dim3((ntile + 255) / 256), dim3(256), 0, s>>>(c.cell_start, (int32_t)ntile, light, heavy,
-                                                                       count);
+        const int32_t* count = c.dev_counters + kCtrInterp;  // light tiles, heavy chunks
+        const int32_t* light = (const int32_t*)c.interp_list;  // ntile entries, then the heavy chunks
+        const int2* heavy = c.interp_list + (ntile + 1) / 2;
         const unsigned nheavy = (unsigned)(interp_list_size(c.G3(), n) - (ntile + 1) / 2);
         const float* p2 = c.p.interlace ? (const float*)c.phi2 : nullptr;
         auto kh = four ? (c.p.interlace ? k_interp_heavy<true, true> : k_interp_heavy<true, false>)
                        : (c.p.interlace ? k_interp_heavy<false, true> : k_interp_heavy<false, false>);
         auto kl = four ? (c.p.interlace ? k_interp_light<true, true> : k_interp_light<true, false>)
                        : (c.p.interlace ? k_interp_light<false, true> : k_interp_light<false, false>);
-        kh<<<dim3(nheavy + 8), dim3(kHeavyThreads), 0, s>>>(c.parts, c.cell_start, heavy, count + 1, (const float*)c.phi, p2,
+        kh<<<dim3(nheavy + 8), dim3(kHeavyThreads), 0, s>>>(c.parts, c.tile_start, heavy, count + 1, (const float*)c.phi, p2,
                                                   c.logG, (float)inv, (float*)c.pm_acc);
-        kl<<<dim3(ntile + 8), dim3(256), 0, s>>>(c.parts, c.cell_start, light, count, (const float*)c.phi, p2,
+        kl<<<dim3(ntile + 8), dim3(256), 0, s>>>(c.parts, c.tile_start, light, count, (const float*)c.phi, p2,
                                                  c.logG, (float)inv, (float*)c.pm_acc);
     } else {  // FP64: gradient grids, then a gather per particle
         const unsigned ng = (unsigned)((c.G3() + kBlock - 1) / kBlock);
@@ -1399,32 +1651,44 @@ hipError_t pm_interp(Ctx& c, int64_t n, hipStream_t s) {
 // only with deconvolution), K2[i] = k_i^2, Ks[i] = k_i and D[i] = the gradient stencil's transfer
 // function (4-point: (8 sin kh - sin 2kh) / 6h; 2-point: sin kh / h), k_i = 2 pi i'/L with i' the
 // signed frequency; norm = -4 pi G / (V_cell G^3) (density = mass/V_cell, unnormalised inverse FFT).
+// Twiddle m (e^{-2 pi i m / G}) and the Green-function tables at axis index i (T, k^2, k, D), in FP64
+// on the host (pm_tables) or the device (k_pm_tables).
+__host__ __device__ inline void green_table_entry(int i, int G, double L, double r_s, int deconvolve, int four,
+                                                  double tw[2], double tab[4]) {
+    const double h = L / G;
+    const double a = -2.0 * M_PI * (double)i / (double)G;
+    tw[0] = cos(a);
+    tw[1] = sin(a);
+    const int is = i <= G / 2 ? i : i - G;
+    const double k = (2.0 * M_PI / L) * is;
+    double t = exp(-k * k * r_s * r_s);
+    if (deconvolve && is != 0) {
+        const double x = 0.5 * k * h;
+        const double sc = sin(x) / x;
+        t /= sc * sc * sc * sc;
+    }
+    tab[0] = t;
+    tab[1] = k * k;
+    tab[2] = k;
+    tab[3] = four ? (8.0 * sin(k * h) - sin(2.0 * k * h)) / (6.0 * h) : sin(k * h) / h;
+}
+
+static double green_norm_of(const jtp_params& p) {
+    const double h = p.box_size / p.n_cells;
+    return -4.0 * M_PI * p.g_constant / (h * h * h * (double)p.n_cells * (double)p.n_cells * (double)p.n_cells);
+}
+
 hipError_t pm_tables(Ctx& c) {
     const int G = (int)c.G();
-    const double L = c.p.box_size, h = L / G;
     std::vector<double> tw(2 * G), tab(4 * G);
-    for (int m = 0; m < G; ++m) {
-        const double a = -2.0 * M_PI * (double)m / (double)G;
-        tw[2 * m] = std::cos(a);
-        tw[2 * m + 1] = std::sin(a);
-    }
-    const double dk = 2.0 * M_PI / L;
-    const bool four = c.p.gradient == JTP_GRAD_4PT;
     for (int i = 0; i < G; ++i) {
-        const int is = i <= G / 2 ? i : i - G;
-        const double k = dk * is;
-        double t = std::exp(-k * k * c.p.r_s * c.p.r_s);
-        if (c.p.deconvolve && is != 0) {
-            const double x = 0.5 * k * h;
-            const double sc = std::sin(x) / x;
-            t /= sc * sc * sc * sc;
-        }
-        tab[i] = t;
-        tab[G + i] = k * k;
-        tab[2 * G + i] = k;
-        tab[3 * G + i] = four ? (8.0 * std::sin(k * h) - std::sin(2.0 * k * h)) / (6.0 * h) : std::sin(k * h) / h;
+        double t2[2], t4[4];
+        green_table_entry(i, G, c.p.box_size, c.p.r_s, c.p.deconvolve, c.p.gradient == JTP_GRAD_4PT, t2, t4);
+        tw[2 * i] = t2[0];
+        tw[2 * i + 1] = t2[1];
+        for (int q = 0; q < 4; ++q) tab[q * G + i] = t4[q];
     }
-    c.green_norm = -4.0 * M_PI * c.p.g_constant / (h * h * h * (double)G * (double)G * (double)G);
+    c.green_norm = green_norm_of(c.p);
     hipError_t e;
     if (c.p.precision == JTP_FP64) {
         e = hipMemcpy(c.twiddle, tw.data(), 2 * G * sizeof(double), hipMemcpyHostToDevice);

@@ -319,17 +319,44 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
 
 // ------------------------------------------------------------------ particle gather
 // parts[i] = (x, y, z, species) of the i-th particle in key order (reorder_by_idx_*,
-// nbody_gpu_twopass.rs:1490-1514, fused into one 16-byte record).
+// nbody_gpu_twopass.rs:1490-1514, fused into one 16-byte record).  With ts, also the start of every
+// Morton-aligned tile of 8^3 PM cells (lt = log2 G - 3 tile bits per axis; the tile index is a
+// prefix of the key): the particle that begins a run writes the starts of the tiles up to its own,
+// so every entry is written once, ts[ntile] = n.  Lanes read the previous particle's tile from
+// their neighbour; lane 0 gathers it again.
 __global__ void k_gather_parts(const uint4* __restrict__ rec, const int32_t* __restrict__ perm,
-                               uint4* __restrict__ parts, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    parts[i] = rec[perm[i]];
+                               uint4* __restrict__ parts, int64_t n, int32_t* __restrict__ ts, int lt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < n;
+    uint4 r = make_uint4(0u, 0u, 0u, 0u);
+    if (in) {
+        r = rec[perm[i]];
+        parts[i] = r;
+    }
+    if (!ts) return;
+    const int sh = 32 - lt;
+    const int32_t ntile = 1 << (3 * lt);
+    auto tile_of = [&](const uint4& q) {
+        return (int32_t)(spread3_21(q.x >> sh) | (spread3_21(q.y >> sh) << 1) | (spread3_21(q.z >> sh) << 2));
+    };
+    const int32_t tc = in ? tile_of(r) : ntile;
+    int32_t tp = __shfl_up(tc, 1, 64);
+    if ((threadIdx.x & 63) == 0) tp = (i == 0 || !in) ? -1 : tile_of(rec[perm[i - 1]]);
+    if (!in) return;
+    for (int32_t t = tp + 1; t <= tc; ++t) ts[t] = (int32_t)i;
+    if (i == n - 1)
+        for (int32_t t = tc + 1; t <= ntile; ++t) ts[t] = (int32_t)n;
 }
 
-hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, hipStream_t s) {
-    if (n > 0)
-        k_gather_parts<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(rec, perm, parts, n);
+hipError_t launch_gather_parts(const uint4* rec, const int32_t* perm, uint4* parts, int64_t n, int32_t* ts, int logG,
+                               hipStream_t s) {
+    if (n > 0) {
+        k_gather_parts<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(rec, perm, parts, n, ts,
+                                                                                         logG - 3);
+    } else if (ts) {
+        const int64_t ntile = (int64_t)1 << (3 * (logG - 3));
+        return launch_fill_i32(ts, ntile + 1, 0, s);
+    }
     return hipGetLastError();
 }
 

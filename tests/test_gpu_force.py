@@ -74,7 +74,10 @@ def test_full_force_vs_ewald(jtp, orc, precision):
     er = _rel_rms(acc[tg], ref)
     print("TreePM vs Ewald (2x24^3, G=32): median %.2e p99 %.2e; |da| / rms|a|: median %.2e p99 %.2e"
           % (np.median(e), np.percentile(e, 99), np.median(er), np.percentile(er, 99)))
-    assert np.median(e) < 2e-2 and np.percentile(e, 99) < 2e-1
+    # a regression bound on a small near-uniform lattice (measured: 6.9e-3 / 4.8e-2 relative, 4.3e-3 /
+    # 1.7e-2 of the rms force); the accuracy bar is asserted on the clustered boxes in test_gpu_headline.py
+    assert np.median(e) < 1e-2 and np.percentile(e, 99) < 6e-2
+    assert np.median(er) < 6e-3 and np.percentile(er, 99) < 2.5e-2
 
 
 def test_compat_vs_reference_cpu(jtp, orc):
@@ -110,7 +113,15 @@ def test_config0_full_force_and_step(jtp, orc, precision):
     """The small configuration: N = 2 x 64^3 lattice + Zel'dovich (P(k) ~ k^-2.5, rms 0.3 cell, m- on the
     half-cell-offset lattice with the opposite displacement), L = 250, PM 64^3, r_s = 1.25 cells,
     r_cut = 4.5 r_s, theta = 0.5, leaf 8, m+ = 1, |m-| = 19: one force evaluation against the FP64 Ewald
-    sum on 300 targets, then one DKD step (finite, momentum of the kick conserved)."""
+    sum on 1000 targets, then one DKD step (finite, momentum of the kick conserved).
+
+    A near-uniform lattice's forces are small residuals of cancelling contributions, and at r_s = 1.25
+    cells its error is set by the mesh part: theta = 0 (no tree approximation) still gives a relative
+    median of 3.6e-3, theta = 0.5 4.1e-3, interlacing off 1.8e-2 (measured).  So the per-particle
+    relative bar of the headline box (1e-3 / 1e-2, clustered, test_gpu_headline.py) is not this
+    configuration's; it is held to the error over the rms force of the sample, |da| / rms|a|: median
+    <= 2.5e-3 and 99th percentile <= 1e-2 (measured 1.6e-3 / 4.1e-3), and to a relative median
+    <= 6e-3, p99 <= 5e-2 (measured 4.1e-3 / 2.3e-2)."""
     print(f"case: config 0 precision={precision}", flush=True)
     w = synth.lattice_zeldovich(n_side=64, box_size=250.0, n_cells=64, seed=42, leaf_size=8)
     assert w.n == 2 * 64 ** 3
@@ -120,14 +131,15 @@ def test_config0_full_force_and_step(jtp, orc, precision):
     ctx = jtp.ForceContext(p, w.m_plus, w.m_minus_abs)
     acc = ctx.compute_forces(pos, w.species).astype(np.float64)
     rng = np.random.default_rng(7)
-    tg = rng.choice(w.n, 300, replace=False)
+    tg = rng.choice(w.n, 1000, replace=False)
     ref = orc.ewald(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, targets=tg,
                     rc=w.box_size / 3)
     e = _rel(acc[tg], ref)
     er = _rel_rms(acc[tg], ref)
     print("config 0 vs Ewald: median %.2e p99 %.2e; |da| / rms|a|: median %.2e p99 %.2e"
           % (np.median(e), np.percentile(e, 99), np.median(er), np.percentile(er, 99)))
-    assert np.median(e) < 1e-2 and np.percentile(e, 99) < 1e-1
+    assert np.median(er) <= 2.5e-3 and np.percentile(er, 99) <= 1e-2
+    assert np.median(e) <= 6e-3 and np.percentile(e, 99) <= 5e-2
     rt = np.float64 if precision == 1 else np.float32
     n = w.n
     dp, dv = jtp.DeviceArray.from_host(pos), jtp.DeviceArray.from_host(np.zeros((n, 3), rt))
