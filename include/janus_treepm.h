@@ -247,6 +247,17 @@ int  jtp_pos_to_fixed_f32(jtp_ctx* ctx, int64_t n, const float* d_pos, uint32_t*
  * (split r_s, k_min, deconvolution).  real = float or double per the context precision. */
 int  jtp_pm_solve(jtp_ctx* ctx, const void* d_rho, void* d_phi, void* stream);
 
+/* ---- allocation-free PM-only Poisson solve.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same result as jtp_pm_solve on a context created with *p (the Green function
+ * from p's box_size, n_cells, r_s, g_constant, k_min, gradient, deconvolve; precision and fft_tile
+ * as there; the other fields are ignored), without a context: the half spectrum and the tables live
+ * in a static device array of 17 MB per GPU (one FP64 half spectrum of 128^3), so n_cells <= 128
+ * (larger meshes return JTP_ERR_ARG).  Calls no allocator.  Asynchronous on the stream: launch
+ * errors are returned, nothing is synchronised.  Calls that overlap on one GPU share the array and
+ * race; calls on one stream are ordered. */
+int  jtp_pm_solve_noalloc(const jtp_params* p, const void* d_rho, void* d_phi, void* stream);
+
 /* ---- reference-compatible Morton keys of f32 positions (compute_morton_all,
  * nbody_gpu_twopass.rs:1447-1487): 21 bits/axis of (x + box_half) * inv_cell_size. */
 int  jtp_morton_keys_f32(int64_t n, const float* d_pos, float box_half, float inv_cell_size,

@@ -21,6 +21,8 @@
 #include "jtp_device.h"
 
 #include <cmath>
+#include <cstdio>
+#include <mutex>
 #include <vector>
 
 namespace jtp {
@@ -1703,4 +1705,99 @@ hipError_t pm_tables(Ctx& c) {
     return e;
 }
 
+// ------------------------------------------------------------------ allocation-free Poisson solve
+// For callers that cannot allocate device memory (code under graph capture, serving loops): the
+// half spectrum and the tables of one solve live in a file-scope device array, one per GPU, held
+// until the process ends.  Its size is fixed by the largest mesh it serves: one FP64 half spectrum
+// of kNoAllocMaxG^3 (17.0 MB at 128^3) plus the FP64 twiddle and Green tables.  Every call
+// overwrites all of it before reading it (the tables kernel writes every table entry, the R2C pass
+// every half-spectrum entry), so nothing needs clearing between calls; calls that overlap on one
+// GPU share the array and race.
+constexpr int kNoAllocMaxG = 128;
+constexpr size_t kNoAllocSpecBytes = (size_t)kNoAllocMaxG * kNoAllocMaxG * (kNoAllocMaxG / 2 + 1) * 2 * sizeof(double);
+constexpr size_t kNoAllocTabOff = kNoAllocSpecBytes;                                      // real [4 G]
+constexpr size_t kNoAllocTwOff = kNoAllocTabOff + 4 * kNoAllocMaxG * sizeof(double);      // complex [G]
+constexpr size_t kNoAllocBytes = kNoAllocTwOff + 2 * kNoAllocMaxG * sizeof(double);
+static_assert(kNoAllocTabOff % 256 == 0 && kNoAllocTwOff % 256 == 0, "scratch sections stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each GPU, looked up once per device with that device current.  In
+// host code the array's name is only a stand-in for the symbol, never a device address.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static hipError_t scratch_ptr_current(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+template <typename R>
+__global__ void k_pm_tables(R* __restrict__ tw, R* __restrict__ tab, int G, double L, double r_s, int deconvolve,
+                            int four) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    double t2[2], t4[4];
+    green_table_entry(i, G, L, r_s, deconvolve, four, t2, t4);
+    tw[2 * i] = (R)t2[0];
+    tw[2 * i + 1] = (R)t2[1];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tab[q * G + i] = (R)t4[q];
+}
+
 }  // namespace jtp
+
+// Allocation-free form of jtp_pm_solve (no counterpart in the reference): see janus_treepm.h.
+extern "C" int jtp_pm_solve_noalloc(const jtp_params* p, const void* d_rho, void* d_phi, void* stream) {
+    using namespace jtp;
+    const char* why = nullptr;
+    if (!p) why = "params is null";
+    else if (!d_rho || !d_phi) why = "null grid pointer";
+    else if (p->n_cells < 16 || (p->n_cells & (p->n_cells - 1))) why = "n_cells must be a power of two >= 16";
+    else if (p->n_cells > kNoAllocMaxG) why = "n_cells exceeds 128, the largest mesh the allocation-free scratch holds";
+    else if (p->precision != JTP_FP32 && p->precision != JTP_FP64) why = "precision must be JTP_FP32 or JTP_FP64";
+    else if (!(p->box_size > 0) || !std::isfinite(p->box_size)) why = "box_size must be > 0";
+    else if (!(p->r_s >= 0) || !std::isfinite(p->r_s)) why = "r_s must be >= 0";
+    else if (!std::isfinite(p->g_constant)) why = "g_constant must be finite";
+    else if (p->k_min < 0) why = "k_min must be >= 0";
+    else if (p->gradient != JTP_GRAD_4PT && p->gradient != JTP_GRAD_2PT) why = "bad gradient";
+    else if (p->deconvolve != 0 && p->deconvolve != 1) why = "deconvolve must be 0 or 1";
+    else if (p->fft_tile != 0 && p->fft_tile != 11 && p->fft_tile != 12 &&
+             !(p->fft_tile == 13 && p->precision == JTP_FP32))
+        why = "fft_tile must be 0, 11, 12 or (FP32) 13";
+    if (why) {
+        std::fprintf(stderr, "jtp_pm_solve_noalloc: %s\n", why);
+        return JTP_ERR_ARG;
+    }
+    void* base = nullptr;
+    if (scratch_ptr_current(&base) != hipSuccess) return JTP_ERR_HIP;
+    unsigned char* b = (unsigned char*)base;
+    hipStream_t s = (hipStream_t)stream;
+    const int G = p->n_cells;
+    int logG = 0;
+    while ((1 << logG) < G) ++logG;
+    const FftPlan f{b, b + kNoAllocTwOff, b + kNoAllocTabOff, green_norm_of(*p), logG, p->k_min, p->fft_tile,
+                    p->precision};
+    const int four = p->gradient == JTP_GRAD_4PT;
+    if (p->precision == JTP_FP64)
+        k_pm_tables<double><<<dim3((G + 127) / 128), dim3(128), 0, s>>>((double*)f.twiddle, (double*)f.gtab, G,
+                                                                         p->box_size, p->r_s, p->deconvolve, four);
+    else
+        k_pm_tables<float><<<dim3((G + 127) / 128), dim3(128), 0, s>>>((float*)f.twiddle, (float*)f.gtab, G,
+                                                                       p->box_size, p->r_s, p->deconvolve, four);
+    if (hipGetLastError() != hipSuccess) return JTP_ERR_HIP;
+    if (plan_solve(f, d_rho, d_phi, s, p->deconvolve) != hipSuccess) return JTP_ERR_HIP;
+    return JTP_OK;
+}

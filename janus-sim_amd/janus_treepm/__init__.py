@@ -45,7 +45,7 @@ EXPORTED = [
     "jtp_ensemble_get_stats", "jtp_diagnostics", "jtp_potential_energy", "jtp_device_memory",
     "jtp_segregation", "jtp_snap_meta_default", "jtp_snapshot_write", "jtp_snapshot_read_meta", "jtp_snapshot_read",
     "jtp_io_last_error", "jtp_ic_default_params", "jtp_generate_ics", "jtp_params_from_config",
-    "jtp_morton30_keys_f32",
+    "jtp_morton30_keys_f32", "jtp_pm_solve_noalloc",
 ]
 
 
@@ -137,6 +137,7 @@ def load_library(path: str = LIB_PATH):
         "jtp_pos_to_fixed_f64": (C.c_int, [vp, i64, vp, vp, vp]),
         "jtp_pos_to_fixed_f32": (C.c_int, [vp, i64, vp, vp, vp]),
         "jtp_pm_solve": (C.c_int, [vp, vp, vp, vp]),
+        "jtp_pm_solve_noalloc": (C.c_int, [C.POINTER(Params), vp, vp, vp]),
         "jtp_morton_keys_f32": (C.c_int, [i64, vp, C.c_float, C.c_float, vp, vp]),
         "jtp_morton30_keys_f32": (C.c_int, [i64, vp, C.c_float, C.c_float, vp, vp, vp]),
         "jtp_params_from_config": (C.c_int, [C.POINTER(ConfigFields), C.POINTER(Params)]),
@@ -582,6 +583,13 @@ class PoissonSolver:
         p.r_cut = box_size / 8.0
         self.ctx = ForceContext(p)
         self.grid_size = grid_size
+
+    def solve_noalloc_device(self, d_rho, d_phi, stream=None):
+        """The same solve through jtp_pm_solve_noalloc (no context buffers, no allocation; asynchronous)."""
+        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
+        rc = load_library().jtp_pm_solve_noalloc(C.byref(self.ctx.params), ptr(d_rho), ptr(d_phi), stream)
+        if rc != JTP_OK:
+            raise JtpError(rc, "jtp_pm_solve_noalloc failed")
 
     def solve(self, rho: np.ndarray) -> np.ndarray:
         """rho: mass per cell, [G,G,G] indexed [z][y][x]; returns the potential."""
