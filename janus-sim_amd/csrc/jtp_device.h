@@ -24,6 +24,19 @@ __device__ inline T wave_incl_scan(T v) {
     return v;
 }
 
+// Inclusive prefix sum of an int across the 64 lanes with DPP row shifts and row broadcasts (no
+// LDS round trips): within each row of 16 lanes by shifts of 1, 2, 4, 8, then the last lane of row
+// 0 (2) is added to rows 1 and 3 (row_bcast:15) and lane 31 to rows 2 and 3 (row_bcast:31).
+__device__ inline int wave_incl_scan_dpp(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

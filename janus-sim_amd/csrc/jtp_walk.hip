@@ -428,7 +428,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             }
             // appends: one scan over (particles | monopoles << 16)
             const int packed = napp | (napm << 16);
-            const int incl = wave_incl_scan(packed);
+            const int incl = wave_incl_scan_dpp(packed);
             const int offp = lp + (incl & 0xffff) - napp;
             const int offm = lm + (incl >> 16) - napm;
             const int totl = __shfl(incl, 63, 64);
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             lp = __builtin_amdgcn_readfirstlane(lp + (totl & 0xffff));
             lm = __builtin_amdgcn_readfirstlane(lm + (totl >> 16));
             sp = __builtin_amdgcn_readfirstlane(sp);
-            mono_inside += wave_sum(nmono_p);
+            mono_inside += __popcll(__ballot(nmono_p > 0)) + __popcll(__ballot(nmono_p > 1));
             wave_lds_fence();
             if (lp + lm >= kFlushAt) {
                 flush(lp, lm);
