@@ -37,6 +37,19 @@ __device__ inline int wave_incl_scan_dpp(int x) {
     return x;
 }
 
+// Inclusive running maximum of a uint32 across the 64 lanes (same DPP pattern as the sum; 0 is
+// the identity that lanes outside a shift or broadcast receive).
+__device__ inline uint32_t wave_incl_max_dpp(uint32_t x) {
+    int v = (int)x;
+    v = (int)max((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+    v = (int)max((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+    v = (int)max((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+    v = (int)max((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+    v = (int)max((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = (int)max((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return (uint32_t)v;
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -175,8 +175,6 @@ __device__ inline double4 mono_entry(const double4* mono, int32_t node, int s, c
     return make_double4(wrap_fixed(m.x - (double)gc[0]) * scale, wrap_fixed(m.y - (double)gc[1]) * scale,
                         wrap_fixed(m.z - (double)gc[2]) * scale, m.w);
 }
-__device__ inline float mono_q(const uint4* mono, int32_t node, int s) { return __uint_as_float(mono[2 * node + s].w); }
-__device__ inline double mono_q(const double4* mono, int32_t node, int s) { return mono[2 * node + s].w; }
 
 // LPT lanes share one target: a group holds up to 64 / LPT targets and each target's
 // interaction list is split over its LPT lanes (entries j = sub, sub + LPT, ...).
@@ -187,11 +185,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     using M = typename std::conditional<sizeof(R) == 4, uint4, double4>::type;
     __shared__ E list_s[WAVES][kListCap];
     __shared__ int32_t stack_s[WAVES][kStackPhys];
+    __shared__ uint32_t owner_s[WAVES][64];  // bucket-particle owner lanes, one round at a time
 
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     E* L = list_s[w];
     int32_t* S = stack_s[w];
+    uint32_t* O = owner_s[w];
     const M* mono = (const M*)a.mono;
     const R scale = (R)a.scale;
     const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2, thetac2 = (R)a.theta_cut2;
@@ -348,38 +348,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             k = k < ks ? k : ks;
             sp -= k;
             const bool has = lane < k;
-            const int32_t node = has ? S[sp + lane] : 0;
+            const int32_t popped = has ? S[sp + lane] : a.root;
             wave_lds_fence();
+            const bool bad = has && (popped >= (int32_t)(a.n - 1) || ~popped >= (int32_t)a.n);
+            if (bad) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skipped
+            const bool live = has && !bad;
+            const int32_t node = live ? popped : a.root;
 
-            // act: 1 accept the monopoles (tail form), 4 accept them as cut pairs (FP32: node entirely
-            // within r_cut of the group box, so the cut never applies and the cheaper pair law is exact),
-            // 2 take the bucket's particles, 3 open
-            int act = 0, napp = 0, napm = 0, nmono_p = 0;
-            int32_t first = 0;
-            bool has0 = false, has1 = false;  // the accepted node's monopoles with nonzero charge
-            E pre0 = {}, pre1 = {};           // FP32: their list entries, built when accepted
-            if (has && (node >= (int32_t)(a.n - 1) || ~node >= (int32_t)a.n)) {
-                if (lane == 0) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skip it
-            } else if (has) {
-                uint32_t nmn[3], nmx[3];
-                int32_t cnt;
-                if (node >= 0) {
-                    const uint4 g0 = a.geo[2 * node], g1 = a.geo[2 * node + 1];
-                    nmn[0] = g0.x; nmn[1] = g0.y; nmn[2] = g0.z;
-                    nmx[0] = g1.x; nmx[1] = g1.y; nmx[2] = g1.z;
-                    first = (int32_t)g0.w;
-                    cnt = (int32_t)g1.w - first + 1;
-                } else {
-                    first = ~node;
-                    const uint4 pr = a.parts[first];
-                    nmn[0] = nmx[0] = pr.x;
-                    nmn[1] = nmx[1] = pr.y;
-                    nmn[2] = nmx[2] = pr.z;
-                    cnt = 1;
-                }
-                // 32-bit integer box arithmetic: |centre offset| <= 2^31, group half-extent <= 2^27 (a
-                // group lies in one level-4 cell) and node half-extent <= 2^31, so adc, t and gap fit
-                R gap2 = 0, far2 = 0, ext = 0;
+            // Classification, free of branches except the memory traffic: an internal node's record is
+            // (min corner, first) and (max corner, last); a particle child's is its own record twice.
+            const bool isnode = node >= 0;
+            const uint4* rec0 = isnode ? a.geo + 2 * node : a.parts + ~node;
+            const uint4 g0 = rec0[0];
+            const uint4 g1 = (isnode ? rec0 + 1 : rec0)[0];
+            const int32_t first = isnode ? (int32_t)g0.w : ~node;
+            const int32_t cnt = isnode ? (int32_t)g1.w - first + 1 : 1;
+            // 32-bit integer box arithmetic: |centre offset| <= 2^31, group half-extent <= 2^27 (a
+            // group lies in one level-4 cell) and node half-extent <= 2^31, so adc, t and gap fit
+            R gap2 = 0, far2 = 0, ext = 0;
+            {
+                const uint32_t nmn[3] = {g0.x, g0.y, g0.z}, nmx[3] = {g1.x, g1.y, g1.z};
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) {
                     const uint32_t e = nmx[ax] - nmn[ax];
@@ -393,78 +381,72 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                     gap2 += gr_ * gr_;
                     const R fr_ = ((R)adc + (R)t) * scale;
                     far2 += fr_ * fr_;
-                    const R er = (R)e * scale;
-                    ext = er > ext ? er : ext;
-                }
-                if (gap2 < rcut2) {
-                    const bool inside = far2 < rcut2;
-                    const R th2 = inside ? theta2 : thetac2;
-                    if (node >= 0 && ext * ext < th2 * gap2) {
-                        if (sizeof(R) == 4) {
-                            pre0 = mono_entry(mono, node, 0, gc, scale);
-                            pre1 = mono_entry(mono, node, 1, gc, scale);
-                            has0 = pre0.w != (R)0;
-                            has1 = pre1.w != (R)0;
-                        } else {
-                            has0 = mono_q(mono, node, 0) != (R)0;
-                            has1 = mono_q(mono, node, 1) != (R)0;
-                        }
-                        const int nq = (has0 ? 1 : 0) + (has1 ? 1 : 0);
-                        if (sizeof(R) == 4 && inside && MODE != JTP_SPLIT_POLY4) {
-                            act = 4;
-                            napp = nq;
-                            nmono_p = nq;
-                        } else {
-                            act = 1;
-                            napm = nq;
-                        }
-                    } else if (cnt <= a.leaf) {
-                        act = 2;
-                        napp = cnt;
-                    } else {
-                        act = 3;
-                    }
+                    ext = fmax(ext, (R)e * scale);
                 }
             }
-            // appends: one scan over (particles | monopoles << 16)
-            const int packed = napp | (napm << 16);
+            const bool reach = live && gap2 < rcut2;
+            const bool inside = far2 < rcut2;
+            // accept the two monopoles / take the bucket's particles / open
+            const bool acc = reach && isnode && ext * ext < (inside ? theta2 : thetac2) * gap2;
+            const bool bucket = reach && !acc && cnt <= a.leaf;
+            const bool open = reach && !acc && !bucket;
+            // The monopole records are read by every lane (lanes that do not accept read node 0's,
+            // one shared cache line), so the entries need no per-lane branch.
+            const int32_t mnode = acc ? node : 0;
+            const E m0 = mono_entry(mono, mnode, 0, gc, scale), m1 = mono_entry(mono, mnode, 1, gc, scale);
+            const bool h0 = acc && m0.w != (R)0, h1 = acc && m1.w != (R)0;
+            const int nq = (h0 ? 1 : 0) + (h1 ? 1 : 0);
+            // FP32: monopoles of a node entirely within r_cut take the (exact there) cut pair law and sit
+            // on the particle side; the others take the tail law from the top of the list
+            const bool cutlaw = sizeof(R) == 4 && inside && MODE != JTP_SPLIT_POLY4;
+            const int nb = bucket ? cnt : 0;
+            const int n4 = cutlaw ? nq : 0;
+            const int nt = cutlaw ? 0 : nq;
+            // one scan over (bucket particles | cut-law monopoles << 11 | tail monopoles << 19):
+            // per wave at most 64 x 16, 128 and 128 entries
+            const int packed = nb | (n4 << 11) | (nt << 19);
             const int incl = wave_incl_scan_dpp(packed);
-            const int offp = lp + (incl & 0xffff) - napp;
-            const int offm = lm + (incl >> 16) - napm;
-            const int totl = __shfl(incl, 63, 64);
-            if (act == 1 || act == 4) {  // tail-form monopoles grow down from the top, cut-law ones up
-                int o = act == 1 ? kListCap - 1 - offm : offp;
-                const int d = act == 1 ? -1 : 1;
-                if (sizeof(R) == 4) {
-                    if (has0) {
-                        L[o] = pre0;
-                        o += d;
-                    }
-                    if (has1) L[o] = pre1;
-                } else {
-                    if (has0) {
-                        L[o] = mono_entry(mono, node, 0, gc, scale);
-                        o += d;
-                    }
-                    if (has1) L[o] = mono_entry(mono, node, 1, gc, scale);
+            const int totl = __builtin_amdgcn_readlane(incl, 63);
+            const int btot = totl & 0x7ff, ptot = btot + ((totl >> 11) & 0xff), ttot = totl >> 19;
+            const int bexcl = (incl & 0x7ff) - nb;
+            if (nq) {  // particle side after the bucket particles, or top down
+                const bool top = !cutlaw;
+                int o = top ? kListCap - 1 - (lm + (incl >> 19) - nt) : lp + btot + ((incl >> 11) & 0xff) - n4;
+                const int d = top ? -1 : 1;
+                if (h0) {
+                    L[o] = m0;
+                    o += d;
                 }
-            } else if (act == 2) {
-                for (int t = 0; t < napp; ++t) {
-                    const uint4 pr = a.parts[first + t];
+                if (h1) L[o] = m1;
+            }
+            // bucket particles, one lane per particle: item w of the step belongs to the bucket lane
+            // whose run [bexcl, bexcl + nb) holds it, found by a running maximum over lane-indexed
+            // run starts (64 items per round)
+            for (int r0 = 0; r0 < btot; r0 += 64) {
+                O[lane] = 0u;
+                wave_lds_fence();
+                if (nb && bexcl < r0 + 64 && bexcl + nb > r0) O[bexcl > r0 ? bexcl - r0 : 0] = (uint32_t)lane + 1u;
+                wave_lds_fence();
+                // every lane takes part in the lane reads (a read from an inactive lane returns 0)
+                const int ow = max((int)wave_incl_max_dpp(O[lane]) - 1, 0);
+                const int fb = __shfl(first, ow, 64), eb = __shfl(bexcl, ow, 64);
+                const int w = r0 + lane;
+                if (w < btot) {
+                    const uint4 pr = a.parts[fb + (w - eb)];
                     E e;
                     e.x = fixed_rel<R>(pr.x, gc[0], scale);
                     e.y = fixed_rel<R>(pr.y, gc[1], scale);
                     e.z = fixed_rel<R>(pr.z, gc[2], scale);
                     e.w = qpart[(int32_t)pr.w > 0 ? 1 : 0];
-                    L[offp + t] = e;
+                    L[lp + w] = e;
                 }
             }
             // pushes
-            const uint64_t om = __ballot(act == 3);
+            const uint64_t om = __ballot(open);
             const int nopen = __popcll(om);
             if (sp + 2 * nopen > kStackPhys) {
                 if (lane == 0) atomicExch(&a.counters[kCtrErr], 1);
-            } else if (act == 3) {
+            } else if (open) {
                 const int rank = __popcll(om & lt);
                 const int2 kd = a.kids[node];
                 S[sp + 2 * rank] = kd.y;
@@ -472,10 +454,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             }
             if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
             // the counts are wave-uniform: keep them in scalar registers
-            lp = __builtin_amdgcn_readfirstlane(lp + (totl & 0xffff));
-            lm = __builtin_amdgcn_readfirstlane(lm + (totl >> 16));
+            lp = __builtin_amdgcn_readfirstlane(lp + ptot);
+            lm = __builtin_amdgcn_readfirstlane(lm + ttot);
             sp = __builtin_amdgcn_readfirstlane(sp);
-            mono_inside += __popcll(__ballot(nmono_p > 0)) + __popcll(__ballot(nmono_p > 1));
+            mono_inside += ptot - btot;
             wave_lds_fence();
             if (lp + lm >= kFlushAt) {
                 flush(lp, lm);
