@@ -917,9 +917,13 @@ int jtp_stage_copy(jtp_ctx* h, int32_t stage, void* host_dst, size_t bytes) {
                 }
             } else {
                 uint32_t* o = (uint32_t*)host_dst;
-                for (int64_t i = 0; i < nI; ++i) {
-                    o[6 * i] = geo[2 * i].x; o[6 * i + 1] = geo[2 * i].y; o[6 * i + 2] = geo[2 * i].z;
-                    o[6 * i + 3] = geo[2 * i + 1].x; o[6 * i + 4] = geo[2 * i + 1].y; o[6 * i + 5] = geo[2 * i + 1].z;
+                for (int64_t i = 0; i < nI; ++i) {  // stored as (centre, extent): min = c - e / 2, max = min + e
+                    const uint32_t c3[3] = {geo[2 * i].x, geo[2 * i].y, geo[2 * i].z};
+                    const uint32_t e3[3] = {geo[2 * i + 1].x, geo[2 * i + 1].y, geo[2 * i + 1].z};
+                    for (int a = 0; a < 3; ++a) {
+                        o[6 * i + a] = c3[a] - e3[a] / 2;
+                        o[6 * i + 3 + a] = c3[a] - e3[a] / 2 + e3[a];
+                    }
                 }
             }
             break;

@@ -120,7 +120,7 @@ struct Ctx {
     uint32_t *hist_blk = nullptr;    // scan partials
     uint4 *parts = nullptr;          // sorted records: fixed xyz + species
     // tree
-    uint4 *node_geo = nullptr;       // 2 per internal node
+    uint4 *node_geo = nullptr;       // 2 per internal node: (centre xyz, first leaf), (extent xyz, last leaf)
     int2 *node_kids = nullptr;
     int2 *up = nullptr;              // internal node: (parent, sibling); the root's parent is -1
     int32_t *flags = nullptr;        // bottom-up arrival counters

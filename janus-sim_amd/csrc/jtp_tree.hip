@@ -100,8 +100,12 @@ __device__ inline void load_child(int32_t c, const uint4* parts, uint4* geo, Nod
         const uint4 g0 = geo[2 * c], g1 = geo[2 * c + 1];
         o.first = g0.w;
         o.last = g1.w;
-        o.mn[0] = g0.x; o.mn[1] = g0.y; o.mn[2] = g0.z;
-        o.mx[0] = g1.x; o.mx[1] = g1.y; o.mx[2] = g1.z;
+        const uint32_t nc[3] = {g0.x, g0.y, g0.z}, e[3] = {g1.x, g1.y, g1.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            o.mn[a] = nc[a] - e[a] / 2;
+            o.mx[a] = o.mn[a] + e[a];
+        }
         const NodeSum s = sums[c];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -186,8 +190,16 @@ __device__ inline ChildData combine_write(int32_t p, const ChildData& a, const C
     o.first = a.first;
     o.last = b.last;
     sums[p] = s;
-    geo[2 * p] = make_uint4(o.mn[0], o.mn[1], o.mn[2], o.first);
-    geo[2 * p + 1] = make_uint4(o.mx[0], o.mx[1], o.mx[2], o.last);
+    // box as (centre, first) and (extent, last): the walk's box test needs no further arithmetic,
+    // and min = centre - extent / 2, max = min + extent recover the corners exactly
+    uint32_t e[3], nc[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        e[ax] = o.mx[ax] - o.mn[ax];
+        nc[ax] = o.mn[ax] + e[ax] / 2;
+    }
+    geo[2 * p] = make_uint4(nc[0], nc[1], nc[2], o.first);
+    geo[2 * p + 1] = make_uint4(e[0], e[1], e[2], o.last);
     write_mono<PREC>(p, s, o.mn, o.mx, mono, qp, qm);
     return o;
 }

@@ -64,6 +64,7 @@ struct WalkArgs {
     float tscale;    // particles: t = rho^2 * tscale - 1 (erfc + Gaussian) or rho * tscale - 1
     float tscale_m;  // monopoles
     float escale;    // exp(-x^2) = exp2(r2s * escale)
+    float theta2_f, theta_cut2_f;
 };
 
 __device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
@@ -160,9 +161,8 @@ __device__ inline R fixed_rel(uint32_t u, uint32_t c, R scale) {
     return (R)(int32_t)(u - c) * scale;
 }
 
-// Monopole entry (relative to the group centre gc).
-__device__ inline float4 mono_entry(const uint4* mono, int32_t node, int s, const uint32_t gc[3], float scale) {
-    const uint4 m = mono[2 * node + s];
+// Monopole entry (relative to the group centre gc) from its record.
+__device__ inline float4 mono_entry(const uint4& m, const uint32_t gc[3], float scale) {
     return make_float4(fixed_rel<float>(m.x, gc[0], scale), fixed_rel<float>(m.y, gc[1], scale),
                        fixed_rel<float>(m.z, gc[2], scale), __uint_as_float(m.w));
 }
@@ -170,8 +170,7 @@ __device__ inline double wrap_fixed(double d) {
     // d in fixed units, bring into [-2^31, 2^31)
     return d - 4294967296.0 * rint(d * (1.0 / 4294967296.0));
 }
-__device__ inline double4 mono_entry(const double4* mono, int32_t node, int s, const uint32_t gc[3], double scale) {
-    const double4 m = mono[2 * node + s];
+__device__ inline double4 mono_entry(const double4& m, const uint32_t gc[3], double scale) {
     return make_double4(wrap_fixed(m.x - (double)gc[0]) * scale, wrap_fixed(m.y - (double)gc[1]) * scale,
                         wrap_fixed(m.z - (double)gc[2]) * scale, m.w);
 }
@@ -194,8 +193,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     uint32_t* O = owner_s[w];
     const M* mono = (const M*)a.mono;
     const R scale = (R)a.scale;
-    const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2, theta2 = (R)a.theta2, thetac2 = (R)a.theta_cut2;
+    const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2;
+    // the opening angles as kernel arguments of the arithmetic type (no per-step conversion)
+    const R theta2 = sizeof(R) == 4 ? (R)a.theta2_f : (R)a.theta2;
+    const R thetac2 = sizeof(R) == 4 ? (R)a.theta_cut2_f : (R)a.theta_cut2;
     const R rcut2s = (R)(a.rcut2 + a.eps2);
+    const R rcut2u = (R)(a.rcut2 / (a.scale * a.scale));  // r_cut^2 in fixed-point units
     const R qpart[2] = {(R)a.qm, (R)a.qp};
     // entries one popped node can append, rounded up to a power of two so the pop bound is a shift
     int app_shift = 1;
@@ -246,6 +249,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         }
         const R tx = fixed_rel<R>(rec.x, gc[0], scale), ty = fixed_rel<R>(rec.y, gc[1], scale),
                 tz = fixed_rel<R>(rec.z, gc[2], scale);
+        const R ghu[3] = {(R)gh[0] + (R)0.5, (R)gh[1] + (R)0.5, (R)gh[2] + (R)0.5};
         R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
         double spot = 0;  // POT: short-range potential sum
 
@@ -356,44 +360,44 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const int32_t node = live ? popped : a.root;
 
             // Classification, free of branches except the memory traffic: an internal node's record is
-            // (min corner, first) and (max corner, last); a particle child's is its own record twice.
+            // (box centre, first) and (box extent, last); a particle child's is its own record and a
+            // zero extent.
             const bool isnode = node >= 0;
             const uint4* rec0 = isnode ? a.geo + 2 * node : a.parts + ~node;
             const uint4 g0 = rec0[0];
-            const uint4 g1 = (isnode ? rec0 + 1 : rec0)[0];
+            const uint4 g1r = (isnode ? rec0 + 1 : rec0)[0];
+            // an internal node's monopoles and children are read with its box (one memory round
+            // trip per step for all three; particle children read node 0's, a shared cache line)
+            const int32_t snode = isnode ? node : 0;
+            const M mr0 = mono[2 * snode], mr1 = mono[2 * snode + 1];
+            const int2 kd = a.kids[snode];
+            const uint4 g1 = isnode ? g1r : make_uint4(0u, 0u, 0u, 0u);
             const int32_t first = isnode ? (int32_t)g0.w : ~node;
             const int32_t cnt = isnode ? (int32_t)g1.w - first + 1 : 1;
-            // 32-bit integer box arithmetic: |centre offset| <= 2^31, group half-extent <= 2^27 (a
-            // group lies in one level-4 cell) and node half-extent <= 2^31, so adc, t and gap fit
-            R gap2 = 0, far2 = 0, ext = 0;
+            // box-to-box gap and farthest-corner distance in fixed-point units: the node spans
+            // centre -+ (extent / 2 + 1/2), the group centre -+ (gh + 1/2) (ghu); the periodic minimum
+            // image comes from the 32-bit wrap of the centre offset
+            R gap2 = 0, far2 = 0;
             {
-                const uint32_t nmn[3] = {g0.x, g0.y, g0.z}, nmx[3] = {g1.x, g1.y, g1.z};
+                const uint32_t nc[3] = {g0.x, g0.y, g0.z}, ne[3] = {g1.x, g1.y, g1.z};
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) {
-                    const uint32_t e = nmx[ax] - nmn[ax];
-                    const uint32_t nc = nmn[ax] + e / 2;
-                    const uint32_t nh = e / 2 + (e & 1u);
-                    const int32_t dc = (int32_t)(nc - gc[ax]);
-                    const uint32_t adc = dc < 0 ? 0u - (uint32_t)dc : (uint32_t)dc;
-                    const uint32_t t = gh[ax] + nh;
-                    const uint32_t gap = adc > t ? adc - t : 0u;
-                    const R gr_ = (R)gap * scale;
-                    gap2 += gr_ * gr_;
-                    const R fr_ = ((R)adc + (R)t) * scale;
-                    far2 += fr_ * fr_;
-                    ext = fmax(ext, (R)e * scale);
+                    const R adc = fabs((R)(int32_t)(nc[ax] - gc[ax]));
+                    const R t = (R)ne[ax] * (R)0.5 + ghu[ax];
+                    const R gp = fmax(adc - t, (R)0);
+                    gap2 += gp * gp;
+                    const R fr = adc + t;
+                    far2 += fr * fr;
                 }
             }
-            const bool reach = live && gap2 < rcut2;
-            const bool inside = far2 < rcut2;
+            const R ext = (R)max(max(g1.x, g1.y), g1.z);
+            const bool reach = live && gap2 < rcut2u;
+            const bool inside = far2 < rcut2u;
             // accept the two monopoles / take the bucket's particles / open
             const bool acc = reach && isnode && ext * ext < (inside ? theta2 : thetac2) * gap2;
             const bool bucket = reach && !acc && cnt <= a.leaf;
             const bool open = reach && !acc && !bucket;
-            // The monopole records are read by every lane (lanes that do not accept read node 0's,
-            // one shared cache line), so the entries need no per-lane branch.
-            const int32_t mnode = acc ? node : 0;
-            const E m0 = mono_entry(mono, mnode, 0, gc, scale), m1 = mono_entry(mono, mnode, 1, gc, scale);
+            const E m0 = mono_entry(mr0, gc, scale), m1 = mono_entry(mr1, gc, scale);
             const bool h0 = acc && m0.w != (R)0, h1 = acc && m1.w != (R)0;
             const int nq = (h0 ? 1 : 0) + (h1 ? 1 : 0);
             // FP32: monopoles of a node entirely within r_cut take the (exact there) cut pair law and sit
@@ -448,7 +452,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 if (lane == 0) atomicExch(&a.counters[kCtrErr], 1);
             } else if (open) {
                 const int rank = __popcll(om & lt);
-                const int2 kd = a.kids[node];
                 S[sp + 2 * rank] = kd.y;
                 S[sp + 2 * rank + 1] = kd.x;
             }
@@ -557,13 +560,15 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
     a.leaf = c.p.leaf_size;
     a.cellnode = n >= 2 ? c.cellnode : nullptr;
     a.start_level = c.start_level;
-    a.max_steps = 4 * n + 1024;
+    a.max_steps = 4 * n + 1024;  // each step pops >= 1 of the <= 2n-1 nodes
     const double tc = c.p.theta_cut < c.p.theta ? c.p.theta_cut : c.p.theta;
-    a.theta_cut2 = tc * tc;  // each step pops >= 1 of the <= 2n-1 nodes
+    a.theta_cut2 = tc * tc;
     a.scale = c.p.box_size / 4294967296.0;
     a.eps2 = c.p.softening * c.p.softening;
     a.rcut2 = c.p.r_cut * c.p.r_cut;
     a.theta2 = c.p.theta * c.p.theta;
+    a.theta2_f = (float)a.theta2;
+    a.theta_cut2_f = (float)a.theta_cut2;
     a.inv2rs = c.p.r_s > 0 ? 0.5 / c.p.r_s : 0.0;
     a.qp = c.p.g_constant * c.m_plus;
     a.qm = -c.p.g_constant * c.m_minus;
@@ -581,8 +586,8 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
         // short of the node's mean potential by ~ <d^2>/6 times that Laplacian: a bias of one sign that
         // the forces do not show but the potential sums up (about 2 % of the short-range potential
         // at theta = 0.5 on a uniform lattice).  The potential therefore opens every node (theta = 0).
-        a.theta2 = 0.0;
-        a.theta_cut2 = 0.0;
+        a.theta2 = a.theta_cut2 = 0.0;
+        a.theta2_f = a.theta_cut2_f = 0.0f;
         a.pot = pot;
         if (lpt == 4) return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2, 4, true>(a, gmax, s);
         if (lpt == 2) return launch_walk<double, JTP_SPLIT_ERFC_GAUSS, 2, 2, true>(a, gmax, s);
