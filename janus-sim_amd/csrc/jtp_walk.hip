@@ -161,6 +161,17 @@ __device__ inline R fixed_rel(uint32_t u, uint32_t c, R scale) {
     return (R)(int32_t)(u - c) * scale;
 }
 
+// Particle entry (relative to the group centre gc, minimum image from the 32-bit wrap).
+template <typename R, typename E>
+__device__ inline E part_entry(const uint4& pr, const uint32_t gc[3], R scale, const R qpart[2]) {
+    E e;
+    e.x = fixed_rel<R>(pr.x, gc[0], scale);
+    e.y = fixed_rel<R>(pr.y, gc[1], scale);
+    e.z = fixed_rel<R>(pr.z, gc[2], scale);
+    e.w = qpart[(int32_t)pr.w > 0 ? 1 : 0];
+    return e;
+}
+
 // Monopole entry (relative to the group centre gc) from its record.
 __device__ inline float4 mono_entry(const uint4& m, const uint32_t gc[3], float scale) {
     return make_float4(fixed_rel<float>(m.x, gc[0], scale), fixed_rel<float>(m.y, gc[1], scale),
@@ -254,6 +265,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         double spot = 0;  // POT: short-range potential sum
 
         int sp = 1, lp = 0, lm = 0;
+        int pidx = -1, pdst = 0;  // a bucket particle to list in the next step (index, list slot)
         int mono_inside = 0;  // monopole entries placed on the particle side since the last flush
         int64_t steps = 0;
         if (a.cellnode) {
@@ -278,6 +290,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
         wave_lds_fence();
 
         // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
+        auto drain = [&]() {  // list the pending bucket particles before the list is read
+            if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);
+            pidx = -1;
+            wave_lds_fence();
+        };
         auto flush = [&](int np_in, int nm_in) {
             // the counts are wave-uniform: keep the loop control in scalar registers
             const int np_ = __builtin_amdgcn_readfirstlane(np_in);
@@ -371,6 +388,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             const int32_t snode = isnode ? node : 0;
             const M mr0 = mono[2 * snode], mr1 = mono[2 * snode + 1];
             const int2 kd = a.kids[snode];
+            if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);  // previous step's
             const uint4 g1 = isnode ? g1r : make_uint4(0u, 0u, 0u, 0u);
             const int32_t first = isnode ? (int32_t)g0.w : ~node;
             const int32_t cnt = isnode ? (int32_t)g1.w - first + 1 : 1;
@@ -426,6 +444,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             // bucket particles, one lane per particle: item w of the step belongs to the bucket lane
             // whose run [bexcl, bexcl + nb) holds it, found by a running maximum over lane-indexed
             // run starts (64 items per round)
+            // The first round's particles are read and listed during the next step (their loads
+            // overlap that step's node reads); later rounds are listed now.
+            int nidx = -1, ndst = 0;
             for (int r0 = 0; r0 < btot; r0 += 64) {
                 O[lane] = 0u;
                 wave_lds_fence();
@@ -436,13 +457,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
                 const int fb = __shfl(first, ow, 64), eb = __shfl(bexcl, ow, 64);
                 const int w = r0 + lane;
                 if (w < btot) {
-                    const uint4 pr = a.parts[fb + (w - eb)];
-                    E e;
-                    e.x = fixed_rel<R>(pr.x, gc[0], scale);
-                    e.y = fixed_rel<R>(pr.y, gc[1], scale);
-                    e.z = fixed_rel<R>(pr.z, gc[2], scale);
-                    e.w = qpart[(int32_t)pr.w > 0 ? 1 : 0];
-                    L[lp + w] = e;
+                    if (r0 == 0) {
+                        nidx = fb + (w - eb);
+                        ndst = lp + w;
+                    } else {
+                        L[lp + w] = part_entry<R, E>(a.parts[fb + (w - eb)], gc, scale, qpart);
+                    }
                 }
             }
             // pushes
@@ -457,18 +477,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
             }
             if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
             // the counts are wave-uniform: keep them in scalar registers
+            pidx = nidx;
+            pdst = ndst;
             lp = __builtin_amdgcn_readfirstlane(lp + ptot);
             lm = __builtin_amdgcn_readfirstlane(lm + ttot);
             sp = __builtin_amdgcn_readfirstlane(sp);
             mono_inside += ptot - btot;
             wave_lds_fence();
             if (lp + lm >= kFlushAt) {
+                drain();
                 flush(lp, lm);
                 lp = 0;
                 lm = 0;
                 wave_lds_fence();
             }
         }
+        drain();
         if (lp + lm > 0) flush(lp, lm);
         wave_lds_fence();
 
