@@ -177,8 +177,9 @@ def accuracy(w, name, d_pos, d_sp, pos_fixed):
 
 def run_ensemble(args):
     """Uneven ensemble of 48 boxes (synth.ensemble_c3): total particle-force-updates/s of the FP32 form
-    with the boxes run sequentially on one stream, concurrently over --ensemble-streams streams (default
-    4), and as one replayed graph."""
+    with the boxes prepared sequentially on one stream, concurrently over --ensemble-streams streams
+    (default 4) or as one replayed graph, and walked either by one launch over all boxes' groups (the
+    default) or box by box ("per_box" modes)."""
     t0 = time.perf_counter()
     ws = synth.ensemble_c3()
     log(f"ensemble generated in {time.perf_counter() - t0:.1f} s ({len(ws)} boxes, N={sum(w.n for w in ws)})")
@@ -211,8 +212,9 @@ def run_ensemble(args):
     log(f"  separate calls: {dt * 1e3:.2f} ms per evaluation of all boxes")
     for ctx in ctxs:
         ctx.close()
-    for mode in ("sequential", "concurrent", "graph"):
-        kw = dict(sequential=(mode == "sequential"), graph=(mode == "graph"))
+    for mode in ("per_box_sequential", "per_box_concurrent", "per_box_graph", "sequential", "concurrent", "graph"):
+        kw = dict(sequential=mode.endswith("sequential"), graph=mode.endswith("graph"),
+                  per_box_walk=mode.startswith("per_box"))
         for _ in range(args.warmup):
             ens.compute_forces_device(n, dpos, dsp, dacc, **kw)
         j.load_library().jtp_device_synchronize()

@@ -217,15 +217,19 @@ int  jtp_segregation(jtp_ctx* ctx, int64_t n, const uint32_t* d_pos_fixed, const
 /* ---- ensembles of independent boxes (no counterpart in the reference, which runs one box per
  * process; the batch replaces a loop of step_treepm_gpu / TreePMForce calls over boxes).
  * Each box has its own parameters (L, G, r_s, ...) and species masses and its own buffers.
- * jtp_ensemble_compute_forces evaluates the force of every box: boxes are spread over n_streams
- * internal streams that fork from and join back into the caller's stream, so small boxes run
- * concurrently; JTP_ENS_SEQUENTIAL runs them one after another on the caller's stream;
- * JTP_ENS_GRAPH captures the whole batch (all streams) into one hipGraph and replays it
- * (re-captured when a count, a pointer or the stream changes).  Results per box are identical in
- * every mode and to jtp_compute_forces on that box.  Arrays are indexed by box. */
+ * jtp_ensemble_compute_forces evaluates the force of every box: each box's preparation (keys,
+ * sort, tree build, PM) is spread over n_streams internal streams that fork from and join back into
+ * the caller's stream, so small boxes run concurrently, and then the short-range walk of all boxes
+ * runs as one persistent launch pulling (box, particle-group) work items (one launch per set of
+ * boxes sharing precision, split law and group size).  JTP_ENS_WALK_PER_BOX launches each box's walk
+ * on its own stream after its preparation instead; JTP_ENS_SEQUENTIAL runs the boxes one after
+ * another on the caller's stream; JTP_ENS_GRAPH captures the whole batch (all streams) into one
+ * hipGraph and replays it (re-captured when a count, a pointer or the stream changes).  Results per
+ * box are identical in every mode and to jtp_compute_forces on that box.  Arrays are indexed by box. */
 typedef struct jtp_ensemble jtp_ensemble;
-#define JTP_ENS_SEQUENTIAL 0x1
-#define JTP_ENS_GRAPH      0x100
+#define JTP_ENS_SEQUENTIAL   0x1
+#define JTP_ENS_WALK_PER_BOX 0x2
+#define JTP_ENS_GRAPH        0x100
 int  jtp_ensemble_create(int32_t n_boxes, const jtp_params* params, const double* m_plus,
                          const double* m_minus_abs, int32_t n_streams, jtp_ensemble** out);
 int  jtp_ensemble_destroy(jtp_ensemble* ens);

@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -413,8 +414,10 @@ int jtp_set_species_mass(jtp_ctx* h, double m_plus, double m_minus_abs) {
 
 // Enqueues one force evaluation and the copies of its status words to pinned host memory; no
 // synchronisation, so it can be captured into a graph.  `timed` brackets the phases with events.
-static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t* d_species, void* d_acc,
-                          hipStream_t s, bool timed, double* pot = nullptr) {
+// Everything before the walk: keys, sort, gather, tree build (side stream) and PM phases, joined
+// back into `s`.
+static int enqueue_prep(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t* d_species, hipStream_t s,
+                        bool timed) {
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[0], s));
     JTP_HIP(&c, launch_fill_i32((int32_t*)c.dev_inter, 4, 0, s));
     // the packed records live in the node-sum buffer, which the tree build (after the gather) overwrites
@@ -448,11 +451,11 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
     if (c.p.pm_enabled) JTP_HIP(&c, pm_interp(c, n, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[5], s));
     if (fork) JTP_HIP(&c, hipStreamWaitEvent(s, c.join_ev, 0));
-    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[7], s));
-    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s, pot));
-    else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
-    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[6], s));
-    // status: interaction counts, group count, walk error flag, root flag
+    return JTP_OK;
+}
+
+// Status words of an evaluation: interaction counts, group count, walk error flag, root flag.
+static int enqueue_status(Ctx& c, int64_t n, hipStream_t s) {
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned, c.dev_inter, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 3, c.dev_inter + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     JTP_HIP(&c, hipMemcpyAsync(c.host_pinned + 1, c.dev_counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -460,6 +463,17 @@ static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t
         JTP_HIP(&c, hipMemcpyAsync((int32_t*)(c.host_pinned + 2), c.dev_counters + kCtrRoot, sizeof(int32_t),
                                    hipMemcpyDeviceToHost, s));
     return JTP_OK;
+}
+
+static int enqueue_forces(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t* d_species, void* d_acc,
+                          hipStream_t s, bool timed, double* pot = nullptr) {
+    int rc = enqueue_prep(c, n, d_pos, d_species, s, timed);
+    if (rc != JTP_OK) return rc;
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[7], s));
+    if (c.p.tree_enabled) JTP_HIP(&c, walk(c, n, d_acc, s, pot));
+    else JTP_HIP(&c, write_pm_only(c, n, d_acc, s));
+    if (timed) JTP_HIP(&c, hipEventRecord(c.ev[6], s));
+    return enqueue_status(c, n, s);
 }
 
 // Interprets the status words of the last force evaluation (after the stream synchronised).
@@ -1039,6 +1053,11 @@ struct jtp_ensemble {
     hipStream_t g_stream = nullptr;
     int last_mode = 0;  // 0 concurrent, 1 sequential, 2 graph
     bool have_result = false;
+    // one walk launch over all boxes: per-box walk arguments (pinned host copy, device copy) and the
+    // launch's group offsets and queue heads
+    unsigned char* h_args = nullptr;
+    void* d_args = nullptr;
+    int32_t* d_ctrl = nullptr;
 };
 
 static int ens_fail(jtp_ensemble* e, int code, const std::string& msg) {
@@ -1064,6 +1083,9 @@ int jtp_ensemble_destroy(jtp_ensemble* e) {
     for (int k = 0; k < kMaxEnsStreams; ++k)
         if (e->join[k]) (void)hipEventDestroy(e->join[k]);
     for (jtp_ctx* b : e->boxes) jtp_destroy(b);
+    if (e->h_args) (void)hipHostFree(e->h_args);
+    if (e->d_args) (void)hipFree(e->d_args);
+    if (e->d_ctrl) (void)hipFree(e->d_ctrl);
     delete e;
     return JTP_OK;
 }
@@ -1098,6 +1120,10 @@ int jtp_ensemble_create(int32_t n_boxes, const jtp_params* params, const double*
     if (er == hipSuccess) er = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
     if (er == hipSuccess) er = hipEventCreate(&e->t0);
     if (er == hipSuccess) er = hipEventCreate(&e->t1);
+    const size_t ab = walk_args_bytes() * (size_t)n_boxes;
+    if (er == hipSuccess) er = hipHostMalloc((void**)&e->h_args, ab, hipHostMallocDefault);
+    if (er == hipSuccess) er = hipMalloc(&e->d_args, ab);
+    if (er == hipSuccess) er = hipMalloc((void**)&e->d_ctrl, sizeof(int32_t) * (size_t)n_boxes * (2 + kXcds));
     if (er != hipSuccess) {
         (void)hipGetLastError();
         jtp_ensemble_destroy(e);
@@ -1110,18 +1136,27 @@ int jtp_ensemble_create(int32_t n_boxes, const jtp_params* params, const double*
 const char* jtp_ensemble_last_error(const jtp_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
 
 // Enqueues every box.  concurrent: fork from `s`, box b on streams[b % K], join back into `s`.
+// one_walk: the boxes' preparation (keys, sort, build, PM) as above, then one walk launch per
+// group of boxes with equal walk_batch_key over all their groups, on `s`.
 static int ens_enqueue(jtp_ensemble* e, const int64_t* n, const uint32_t* const* pos, const int8_t* const* sp,
-                       void* const* acc, bool concurrent, hipStream_t s, bool timed) {
+                       void* const* acc, bool concurrent, bool one_walk, hipStream_t s, bool timed) {
     const int K = (int)e->streams.size();
     if (concurrent) {
         ENS_HIP(e, hipEventRecord(e->fork, s));
         for (int k = 0; k < K; ++k) ENS_HIP(e, hipStreamWaitEvent(e->streams[k], e->fork, 0));
     }
+    std::vector<int> batched;  // boxes whose walk joins the shared launch
     for (size_t b = 0; b < e->boxes.size(); ++b) {
         if (n[b] == 0) continue;
         Ctx& c = e->boxes[b]->c;
         hipStream_t bs = concurrent ? e->streams[b % K] : s;
-        int rc = enqueue_forces(c, n[b], pos[b], sp[b], acc[b], bs, timed);
+        int rc;
+        if (one_walk && c.p.tree_enabled) {
+            rc = enqueue_prep(c, n[b], pos[b], sp[b], bs, false);
+            batched.push_back((int)b);
+        } else {
+            rc = enqueue_forces(c, n[b], pos[b], sp[b], acc[b], bs, timed);
+        }
         if (rc != JTP_OK) return ens_fail(e, rc, "box " + std::to_string(b) + ": " + c.err);
     }
     if (concurrent) {
@@ -1130,6 +1165,40 @@ static int ens_enqueue(jtp_ensemble* e, const int64_t* n, const uint32_t* const*
             ENS_HIP(e, hipStreamWaitEvent(s, e->join[k], 0));
         }
     }
+    if (batched.empty()) return JTP_OK;
+    // the walk arguments of the batched boxes, grouped by key (each group contiguous)
+    const size_t ab = walk_args_bytes();
+    std::vector<int> keys;
+    for (int b : batched) {
+        const int k = walk_batch_key(e->boxes[b]->c);
+        if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
+    }
+    size_t slot = 0, ctrl = 0;
+    std::vector<std::pair<size_t, int>> launches;  // first slot, box count
+    std::vector<int64_t> items;
+    for (int k : keys) {
+        const size_t first = slot;
+        int64_t gmax = 0;
+        for (int b : batched) {
+            Ctx& c = e->boxes[b]->c;
+            if (walk_batch_key(c) != k) continue;
+            walk_fill_args(c, n[b], acc[b], e->h_args + ab * slot++);
+            gmax += c.n_groups_max;
+        }
+        launches.emplace_back(first, (int)(slot - first));
+        items.push_back(gmax);
+    }
+    ENS_HIP(e, hipMemcpyAsync(e->d_args, e->h_args, ab * slot, hipMemcpyHostToDevice, s));
+    for (size_t i = 0; i < launches.size(); ++i) {
+        const int m = launches[i].second;
+        ENS_HIP(e, walk_boxes((const unsigned char*)e->d_args + ab * launches[i].first, m, e->d_ctrl + ctrl,
+                              items[i], keys[i], s));
+        ctrl += (size_t)m + 1 + kXcds;
+    }
+    for (int b : batched) {
+        int rc = enqueue_status(e->boxes[b]->c, n[b], s);
+        if (rc != JTP_OK) return ens_fail(e, rc, "box " + std::to_string(b) + ": " + e->boxes[b]->c.err);
+    }
     return JTP_OK;
 }
 
@@ -1137,9 +1206,10 @@ int jtp_ensemble_compute_forces(jtp_ensemble* e, const int64_t* n, const uint32_
                                 const int8_t* const* sp, void* const* acc, int32_t flags, void* stream) {
     if (!e) return JTP_ERR_ARG;
     if (!n || !pos || !sp || !acc) return ens_fail(e, JTP_ERR_ARG, "null array");
-    if (flags & ~(JTP_ENS_SEQUENTIAL | JTP_ENS_GRAPH)) return ens_fail(e, JTP_ERR_ARG, "unknown flags");
+    if (flags & ~(JTP_ENS_SEQUENTIAL | JTP_ENS_WALK_PER_BOX | JTP_ENS_GRAPH)) return ens_fail(e, JTP_ERR_ARG, "unknown flags");
     const bool graph = (flags & JTP_ENS_GRAPH) != 0;
     const bool concurrent = !(flags & JTP_ENS_SEQUENTIAL);
+    const bool one_walk = !(flags & JTP_ENS_WALK_PER_BOX);
     hipStream_t s = (hipStream_t)stream;
     const size_t B = e->boxes.size();
     e->have_result = false;
@@ -1150,7 +1220,7 @@ int jtp_ensemble_compute_forces(jtp_ensemble* e, const int64_t* n, const uint32_
         Ctx& c = e->boxes[b]->c;
         c.have_result = false;
         c.last_n = n[b];
-        c.last_kind = graph ? 3 : 0;
+        c.last_kind = (graph || (one_walk && c.p.tree_enabled)) ? 3 : 0;  // 3: phases not timed per box
         c.last_stream = s;
         if (n[b] > 0) {
             int rc = prepare(c, n[b]);  // allocations happen here, outside any capture
@@ -1167,13 +1237,13 @@ int jtp_ensemble_compute_forces(jtp_ensemble* e, const int64_t* n, const uint32_
             ptrs.push_back(acc[b]);
         }
         std::vector<int64_t> nv(n, n + B);
-        ptrs.push_back((const void*)(intptr_t)(concurrent ? 1 : 0));
+        ptrs.push_back((const void*)(intptr_t)((concurrent ? 1 : 0) | (one_walk ? 2 : 0)));
         if (!e->exec || dirty || nv != e->g_n || ptrs != e->g_ptrs || s != e->g_stream) {
             if (e->exec) (void)hipGraphExecDestroy(e->exec);
             e->exec = nullptr;
             if (!e->capture_stream) ENS_HIP(e, hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
             ENS_HIP(e, hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal));
-            int rc = ens_enqueue(e, n, pos, sp, acc, concurrent, e->capture_stream, false);
+            int rc = ens_enqueue(e, n, pos, sp, acc, concurrent, one_walk, e->capture_stream, false);
             hipGraph_t g = nullptr;
             hipError_t er = hipStreamEndCapture(e->capture_stream, &g);
             if (rc != JTP_OK) {
@@ -1197,7 +1267,7 @@ int jtp_ensemble_compute_forces(jtp_ensemble* e, const int64_t* n, const uint32_
         ENS_HIP(e, hipGraphLaunch(e->exec, s));
     } else {
         ENS_HIP(e, hipEventRecord(e->t0, s));
-        int rc = ens_enqueue(e, n, pos, sp, acc, concurrent, s, true);
+        int rc = ens_enqueue(e, n, pos, sp, acc, concurrent, one_walk, s, true);
         if (rc != JTP_OK) return rc;
     }
     ENS_HIP(e, hipEventRecord(e->t1, s));

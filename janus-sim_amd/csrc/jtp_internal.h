@@ -233,6 +233,13 @@ hipError_t pm_tables(Ctx& c);                                      // twiddles +
 
 // pot != nullptr (FP64 contexts, erfc + Gaussian law): also writes the short-range potential
 hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot = nullptr);
+// One walk launch over several boxes: each box's arguments (walk_args_bytes() each, filled on the
+// host by walk_fill_args after its build is enqueued) in device memory; boxes with equal
+// walk_batch_key share a launch.  d_ctrl holds nbox + 1 + kXcds int32 (group offsets, queue heads).
+size_t walk_args_bytes();
+void walk_fill_args(Ctx& c, int64_t n, void* acc_out, void* dst);
+int walk_batch_key(const Ctx& c);
+hipError_t walk_boxes(const void* d_args, int32_t nbox, int32_t* d_ctrl, int64_t items_max, int key, hipStream_t s);
 hipError_t launch_kick(Ctx& c, void* vel, const void* acc, const int8_t* species, int64_t n, hipStream_t s);
 hipError_t launch_drift(Ctx& c, uint32_t* pos, const void* vel, int64_t n, hipStream_t s);
 hipError_t launch_diagnostics(Ctx& c, int64_t n, const void* vel, const int8_t* species, double* scratch,

@@ -188,24 +188,30 @@ __device__ inline double4 mono_entry(const double4& m, const uint32_t gc[3], dou
 
 // LPT lanes share one target: a group holds up to 64 / LPT targets and each target's
 // interaction list is split over its LPT lanes (entries j = sub, sub + LPT, ...).
-template <typename R, int MODE, int WAVES, int LPT, bool POT = false>
-__global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
+template <typename R, int LPT>
+struct WalkLds {
+    using E = typename EntryT<R>::T;
+    E list[kListCap];
+    int32_t stack[kStackPhys];
+    uint32_t owner[64];  // bucket-particle owner lanes, one round at a time
+};
+
+// One group g of one box: traversal, list evaluation and the output write.  Adds the group's
+// interaction counts to my_inter / my_mono (lane 0's copies are the wavefront's).
+template <typename R, int MODE, int LPT, bool POT>
+__device__ __forceinline__ void walk_group(const WalkArgs& a, int g, WalkLds<R, LPT>& lds, unsigned long long& my_inter,
+                                           unsigned long long& my_mono) {
     constexpr int GL = 64 / LPT;  // targets per wavefront
     using E = typename EntryT<R>::T;
     using M = typename std::conditional<sizeof(R) == 4, uint4, double4>::type;
-    __shared__ E list_s[WAVES][kListCap];
-    __shared__ int32_t stack_s[WAVES][kStackPhys];
-    __shared__ uint32_t owner_s[WAVES][64];  // bucket-particle owner lanes, one round at a time
-
-    const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    E* L = list_s[w];
-    int32_t* S = stack_s[w];
-    uint32_t* O = owner_s[w];
+    E* L = lds.list;
+    int32_t* S = lds.stack;
+    uint32_t* O = lds.owner;
     const M* mono = (const M*)a.mono;
     const R scale = (R)a.scale;
     const R eps2 = (R)a.eps2, rcut2 = (R)a.rcut2;
-    // the opening angles as kernel arguments of the arithmetic type (no per-step conversion)
+    // the opening angles as arguments of the arithmetic type (no per-step conversion)
     const R theta2 = sizeof(R) == 4 ? (R)a.theta2_f : (R)a.theta2;
     const R thetac2 = sizeof(R) == 4 ? (R)a.theta_cut2_f : (R)a.theta_cut2;
     const R rcut2s = (R)(a.rcut2 + a.eps2);
@@ -215,307 +221,344 @@ __global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
     int app_shift = 1;
     while ((1 << app_shift) < a.leaf) ++app_shift;
     const uint64_t lt = lanemask_lt();
-    unsigned long long my_inter = 0, my_mono = 0;
-    const int32_t n_groups = a.counters[kCtrNGroups];
-    // XCD-local queues: the groups (in Morton order) are cut into kXcds contiguous ranges and a
-    // workgroup first drains the range of the XCD it runs on (workgroup i -> XCD i % 8), so
-    // concurrent groups on one XCD share tree nodes in that XCD's L2; then it steals from the
-    // following ranges.  Each group's result does not depend on who computes it.
-    const int xcd = blockIdx.x % kXcds;
-    int steal = 0;
 
-    while (true) {
-        int g = -1;
-        while (steal < kXcds) {
-            const int q = (xcd + steal) % kXcds;
-            const int32_t start = (int32_t)((int64_t)n_groups * q / kXcds);
-            const int32_t end = (int32_t)((int64_t)n_groups * (q + 1) / kXcds);
-            int t = 0;
-            if (lane == 0) t = atomicAdd(&a.counters[kCtrXcd + q], 1);
-            t = __shfl(t, 0, 64);
-            if (start + t < end) {
-                g = start + t;
-                break;
-            }
-            ++steal;
-        }
-        if (g < 0) break;
-        const int2 gr = a.groups[g];
-        const int tgt = lane % GL, sub = lane / GL;
-        const bool valid = tgt < gr.y;
-        const int64_t ti = (int64_t)gr.x + (valid ? tgt : 0);
-        const uint4 rec = a.parts[ti];
-        const int nvalid = gr.y < GL ? gr.y : GL;
+    const int2 gr = a.groups[g];
+    const int tgt = lane % GL, sub = lane / GL;
+    const bool valid = tgt < gr.y;
+    const int64_t ti = (int64_t)gr.x + (valid ? tgt : 0);
+    const uint4 rec = a.parts[ti];
+    const int nvalid = gr.y < GL ? gr.y : GL;
 
-        uint32_t gmn[3], gmx[3], gc[3], gh[3];
-        {
-            const uint32_t p[3] = {rec.x, rec.y, rec.z};
+    uint32_t gmn[3], gmx[3], gc[3], gh[3];
+    {
+        const uint32_t p[3] = {rec.x, rec.y, rec.z};
 #pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                gmn[ax] = wave_min_u32(valid ? p[ax] : 0xffffffffu);
-                gmx[ax] = wave_max_u32(valid ? p[ax] : 0u);
-                gc[ax] = gmn[ax] + (gmx[ax] - gmn[ax]) / 2;
-                gh[ax] = (gmx[ax] - gmn[ax]) / 2 + ((gmx[ax] - gmn[ax]) & 1u);
-            }
-        }
-        const R tx = fixed_rel<R>(rec.x, gc[0], scale), ty = fixed_rel<R>(rec.y, gc[1], scale),
-                tz = fixed_rel<R>(rec.z, gc[2], scale);
-        const R ghu[3] = {(R)gh[0] + (R)0.5, (R)gh[1] + (R)0.5, (R)gh[2] + (R)0.5};
-        R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
-        double spot = 0;  // POT: short-range potential sum
-
-        int sp = 1, lp = 0, lm = 0;
-        int pidx = -1, pdst = 0;  // a bucket particle to list in the next step (index, list slot)
-        int mono_inside = 0;  // monopole entries placed on the particle side since the last flush
-        int64_t steps = 0;
-        if (a.cellnode) {
-            // seed with the 27 cells around the group's cell: a cell's edge is >= r_cut, so every
-            // source within r_cut of the group lies in them (the group lies inside one cell)
-            const int lv = a.start_level, sh = 32 - lv, msk = (1 << lv) - 1;
-            int32_t nd = kNoNodeW;
-            if (lane < 27) {
-                const int cx = (int)((gc[0] >> sh) + (lane % 3) - 1) & msk;
-                const int cy = (int)((gc[1] >> sh) + ((lane / 3) % 3) - 1) & msk;
-                const int cz = (int)((gc[2] >> sh) + (lane / 9) - 1) & msk;
-                const uint64_t cm = spread3_21((uint32_t)cx) | (spread3_21((uint32_t)cy) << 1) |
-                                    (spread3_21((uint32_t)cz) << 2);
-                nd = a.cellnode[cm];
-            }
-            const uint64_t have = __ballot(nd != kNoNodeW);
-            if (nd != kNoNodeW) S[__popcll(have & lt)] = nd;
-            sp = __popcll(have);
-        } else if (lane == 0) {
-            S[0] = a.root;
-        }
-        wave_lds_fence();
-
-        // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
-        auto drain = [&]() {  // list the pending bucket particles before the list is read
-            if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);
-            pidx = -1;
-            wave_lds_fence();
-        };
-        auto flush = [&](int np_in, int nm_in) {
-            // the counts are wave-uniform: keep the loop control in scalar registers
-            const int np_ = __builtin_amdgcn_readfirstlane(np_in);
-            const int nm_ = __builtin_amdgcn_readfirstlane(nm_in);
-            const int itp = (np_ + LPT - 1) / LPT, itm = (nm_ + LPT - 1) / LPT;
-            if (LPT > 1) {  // zero-charge padding so every lane reads a finite entry
-                if (lane < LPT - 1) {
-                    E z;
-                    z.x = z.y = z.z = (R)0;
-                    z.w = (R)0;
-                    L[np_ + lane] = z;
-                    L[kListCap - nm_ - 1 - lane] = z;
-                }
-                wave_lds_fence();
-            }
-            R px = 0, py = 0, pz = 0;
-            double ppot = 0;
-#pragma unroll 4
-            for (int jj = 0; jj < itp; ++jj) {
-                const E e = L[jj * LPT + sub];
-                const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
-                const R f = Pair<R, MODE, true>::f(e.w, dx, dy, dz, a, eps2, rcut2, rcut2s);
-                px += f * dx;
-                py += f * dy;
-                pz += f * dz;
-                if (POT) ppot += pair_potential<true>((double)e.w, dx, dy, dz, a, (double)eps2, (double)rcut2);
-            }
-            const int m0 = kListCap - itm * LPT + sub;
-#pragma unroll 4
-            for (int jj = 0; jj < itm; ++jj) {
-                const E e = L[m0 + jj * LPT];
-                const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
-                const R f = Pair<R, MODE, MODE == JTP_SPLIT_POLY4>::f(e.w, dx, dy, dz, a, eps2, rcut2, rcut2s);
-                px += f * dx;
-                py += f * dy;
-                pz += f * dz;
-                if (POT) ppot += pair_potential<false>((double)e.w, dx, dy, dz, a, (double)eps2, (double)rcut2);
-            }
-#pragma unroll
-            for (int off = GL; off < 64; off <<= 1) {  // combine the LPT partial sums of a target
-                px += __shfl_xor(px, off, 64);
-                py += __shfl_xor(py, off, 64);
-                pz += __shfl_xor(pz, off, 64);
-                if (POT) ppot += __shfl_xor(ppot, off, 64);
-            }
-            if (POT) spot += ppot;
-            if (sizeof(R) == 4) {  // compensated accumulation of the chunk partials
-                R y = px - cx, t = sx + y;
-                cx = (t - sx) - y;
-                sx = t;
-                y = py - cy; t = sy + y; cy = (t - sy) - y; sy = t;
-                y = pz - cz; t = sz + y; cz = (t - sz) - y; sz = t;
-            } else {
-                sx += px;
-                sy += py;
-                sz += pz;
-            }
-            my_inter += (unsigned long long)(np_ + nm_) * (unsigned long long)nvalid;
-            my_mono += (unsigned long long)(nm_ + mono_inside) * (unsigned long long)nvalid;
-            mono_inside = 0;
-        };
-
-        while (sp > 0) {
-            if (++steps > a.max_steps) {  // a finite hierarchy never gets here; report instead of spinning
-                if (lane == 0) atomicExch(&a.counters[kCtrErr], 2);
-                break;
-            }
-            int k = sp < 64 ? sp : 64;
-            const int kl = (kListCap - 2 * kPad - lp - lm) >> app_shift;
-            k = k < kl ? k : kl;
-            const int ks = kStackCap - sp > 1 ? kStackCap - sp : 1;
-            k = k < ks ? k : ks;
-            sp -= k;
-            const bool has = lane < k;
-            const int32_t popped = has ? S[sp + lane] : a.root;
-            wave_lds_fence();
-            const bool bad = has && (popped >= (int32_t)(a.n - 1) || ~popped >= (int32_t)a.n);
-            if (bad) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skipped
-            const bool live = has && !bad;
-            const int32_t node = live ? popped : a.root;
-
-            // Classification, free of branches except the memory traffic: an internal node's record is
-            // (box centre, first) and (box extent, last); a particle child's is its own record and a
-            // zero extent.
-            const bool isnode = node >= 0;
-            const uint4* rec0 = isnode ? a.geo + 2 * node : a.parts + ~node;
-            const uint4 g0 = rec0[0];
-            const uint4 g1r = (isnode ? rec0 + 1 : rec0)[0];
-            // an internal node's monopoles and children are read with its box (one memory round
-            // trip per step for all three; particle children read node 0's, a shared cache line)
-            const int32_t snode = isnode ? node : 0;
-            const M mr0 = mono[2 * snode], mr1 = mono[2 * snode + 1];
-            const int2 kd = a.kids[snode];
-            if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);  // previous step's
-            const uint4 g1 = isnode ? g1r : make_uint4(0u, 0u, 0u, 0u);
-            const int32_t first = isnode ? (int32_t)g0.w : ~node;
-            const int32_t cnt = isnode ? (int32_t)g1.w - first + 1 : 1;
-            // box-to-box gap and farthest-corner distance in fixed-point units: the node spans
-            // centre -+ (extent / 2 + 1/2), the group centre -+ (gh + 1/2) (ghu); the periodic minimum
-            // image comes from the 32-bit wrap of the centre offset
-            R gap2 = 0, far2 = 0;
-            {
-                const uint32_t nc[3] = {g0.x, g0.y, g0.z}, ne[3] = {g1.x, g1.y, g1.z};
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    const R adc = fabs((R)(int32_t)(nc[ax] - gc[ax]));
-                    const R t = (R)ne[ax] * (R)0.5 + ghu[ax];
-                    const R gp = fmax(adc - t, (R)0);
-                    gap2 += gp * gp;
-                    const R fr = adc + t;
-                    far2 += fr * fr;
-                }
-            }
-            const R ext = (R)max(max(g1.x, g1.y), g1.z);
-            const bool reach = live && gap2 < rcut2u;
-            const bool inside = far2 < rcut2u;
-            // accept the two monopoles / take the bucket's particles / open
-            const bool acc = reach && isnode && ext * ext < (inside ? theta2 : thetac2) * gap2;
-            const bool bucket = reach && !acc && cnt <= a.leaf;
-            const bool open = reach && !acc && !bucket;
-            const E m0 = mono_entry(mr0, gc, scale), m1 = mono_entry(mr1, gc, scale);
-            const bool h0 = acc && m0.w != (R)0, h1 = acc && m1.w != (R)0;
-            const int nq = (h0 ? 1 : 0) + (h1 ? 1 : 0);
-            // FP32: monopoles of a node entirely within r_cut take the (exact there) cut pair law and sit
-            // on the particle side; the others take the tail law from the top of the list
-            const bool cutlaw = sizeof(R) == 4 && inside && MODE != JTP_SPLIT_POLY4;
-            const int nb = bucket ? cnt : 0;
-            const int n4 = cutlaw ? nq : 0;
-            const int nt = cutlaw ? 0 : nq;
-            // one scan over (bucket particles | cut-law monopoles << 11 | tail monopoles << 19):
-            // per wave at most 64 x 16, 128 and 128 entries
-            const int packed = nb | (n4 << 11) | (nt << 19);
-            const int incl = wave_incl_scan_dpp(packed);
-            const int totl = __builtin_amdgcn_readlane(incl, 63);
-            const int btot = totl & 0x7ff, ptot = btot + ((totl >> 11) & 0xff), ttot = totl >> 19;
-            const int bexcl = (incl & 0x7ff) - nb;
-            if (nq) {  // particle side after the bucket particles, or top down
-                const bool top = !cutlaw;
-                int o = top ? kListCap - 1 - (lm + (incl >> 19) - nt) : lp + btot + ((incl >> 11) & 0xff) - n4;
-                const int d = top ? -1 : 1;
-                if (h0) {
-                    L[o] = m0;
-                    o += d;
-                }
-                if (h1) L[o] = m1;
-            }
-            // bucket particles, one lane per particle: item w of the step belongs to the bucket lane
-            // whose run [bexcl, bexcl + nb) holds it, found by a running maximum over lane-indexed
-            // run starts (64 items per round)
-            // The first round's particles are read and listed during the next step (their loads
-            // overlap that step's node reads); later rounds are listed now.
-            int nidx = -1, ndst = 0;
-            for (int r0 = 0; r0 < btot; r0 += 64) {
-                O[lane] = 0u;
-                wave_lds_fence();
-                if (nb && bexcl < r0 + 64 && bexcl + nb > r0) O[bexcl > r0 ? bexcl - r0 : 0] = (uint32_t)lane + 1u;
-                wave_lds_fence();
-                // every lane takes part in the lane reads (a read from an inactive lane returns 0)
-                const int ow = max((int)wave_incl_max_dpp(O[lane]) - 1, 0);
-                const int fb = __shfl(first, ow, 64), eb = __shfl(bexcl, ow, 64);
-                const int w = r0 + lane;
-                if (w < btot) {
-                    if (r0 == 0) {
-                        nidx = fb + (w - eb);
-                        ndst = lp + w;
-                    } else {
-                        L[lp + w] = part_entry<R, E>(a.parts[fb + (w - eb)], gc, scale, qpart);
-                    }
-                }
-            }
-            // pushes
-            const uint64_t om = __ballot(open);
-            const int nopen = __popcll(om);
-            if (sp + 2 * nopen > kStackPhys) {
-                if (lane == 0) atomicExch(&a.counters[kCtrErr], 1);
-            } else if (open) {
-                const int rank = __popcll(om & lt);
-                S[sp + 2 * rank] = kd.y;
-                S[sp + 2 * rank + 1] = kd.x;
-            }
-            if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
-            // the counts are wave-uniform: keep them in scalar registers
-            pidx = nidx;
-            pdst = ndst;
-            lp = __builtin_amdgcn_readfirstlane(lp + ptot);
-            lm = __builtin_amdgcn_readfirstlane(lm + ttot);
-            sp = __builtin_amdgcn_readfirstlane(sp);
-            mono_inside += ptot - btot;
-            wave_lds_fence();
-            if (lp + lm >= kFlushAt) {
-                drain();
-                flush(lp, lm);
-                lp = 0;
-                lm = 0;
-                wave_lds_fence();
-            }
-        }
-        drain();
-        if (lp + lm > 0) flush(lp, lm);
-        wave_lds_fence();
-
-        if (valid && sub == 0) {
-            const R sg = (int32_t)rec.w > 0 ? (R)1 : (R)-1;
-            R ox = sg * sx, oy = sg * sy, oz = sg * sz;
-            if (a.pm_acc) {
-                const R* pm = (const R*)a.pm_acc;
-                ox += pm[3 * ti];
-                oy += pm[3 * ti + 1];
-                oz += pm[3 * ti + 2];
-            }
-            const int64_t o = (int64_t)a.perm[ti];
-            R* out = (R*)a.out;
-            out[3 * o] = ox;
-            out[3 * o + 1] = oy;
-            out[3 * o + 2] = oz;
-            if (POT) a.pot[o] = -spot;  // Phi_short = -sum_j q_j erfc(rho / 2 r_s) / rho
+        for (int ax = 0; ax < 3; ++ax) {
+            gmn[ax] = wave_min_u32(valid ? p[ax] : 0xffffffffu);
+            gmx[ax] = wave_max_u32(valid ? p[ax] : 0u);
+            gc[ax] = gmn[ax] + (gmx[ax] - gmn[ax]) / 2;
+            gh[ax] = (gmx[ax] - gmn[ax]) / 2 + ((gmx[ax] - gmn[ax]) & 1u);
         }
     }
-    if (lane == 0 && my_inter) {
+    const R tx = fixed_rel<R>(rec.x, gc[0], scale), ty = fixed_rel<R>(rec.y, gc[1], scale),
+            tz = fixed_rel<R>(rec.z, gc[2], scale);
+    const R ghu[3] = {(R)gh[0] + (R)0.5, (R)gh[1] + (R)0.5, (R)gh[2] + (R)0.5};
+    R sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
+    double spot = 0;  // POT: short-range potential sum
+
+    int sp = 1, lp = 0, lm = 0;
+    int pidx = -1, pdst = 0;  // a bucket particle to list in the next step (index, list slot)
+    int mono_inside = 0;  // monopole entries placed on the particle side since the last flush
+    int64_t steps = 0;
+    if (a.cellnode) {
+        // seed with the 27 cells around the group's cell: a cell's edge is >= r_cut, so every
+        // source within r_cut of the group lies in them (the group lies inside one cell)
+        const int lv = a.start_level, sh = 32 - lv, msk = (1 << lv) - 1;
+        int32_t nd = kNoNodeW;
+        if (lane < 27) {
+            const int cx = (int)((gc[0] >> sh) + (lane % 3) - 1) & msk;
+            const int cy = (int)((gc[1] >> sh) + ((lane / 3) % 3) - 1) & msk;
+            const int cz = (int)((gc[2] >> sh) + (lane / 9) - 1) & msk;
+            const uint64_t cm = spread3_21((uint32_t)cx) | (spread3_21((uint32_t)cy) << 1) |
+                                (spread3_21((uint32_t)cz) << 2);
+            nd = a.cellnode[cm];
+        }
+        const uint64_t have = __ballot(nd != kNoNodeW);
+        if (nd != kNoNodeW) S[__popcll(have & lt)] = nd;
+        sp = __popcll(have);
+    } else if (lane == 0) {
+        S[0] = a.root;
+    }
+    wave_lds_fence();
+
+    // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
+    auto drain = [&]() {  // list the pending bucket particles before the list is read
+        if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);
+        pidx = -1;
+        wave_lds_fence();
+    };
+    auto flush = [&](int np_in, int nm_in) {
+        // the counts are wave-uniform: keep the loop control in scalar registers
+        const int np_ = __builtin_amdgcn_readfirstlane(np_in);
+        const int nm_ = __builtin_amdgcn_readfirstlane(nm_in);
+        const int itp = (np_ + LPT - 1) / LPT, itm = (nm_ + LPT - 1) / LPT;
+        if (LPT > 1) {  // zero-charge padding so every lane reads a finite entry
+            if (lane < LPT - 1) {
+                E z;
+                z.x = z.y = z.z = (R)0;
+                z.w = (R)0;
+                L[np_ + lane] = z;
+                L[kListCap - nm_ - 1 - lane] = z;
+            }
+            wave_lds_fence();
+        }
+        R px = 0, py = 0, pz = 0;
+        double ppot = 0;
+#pragma unroll 4
+        for (int jj = 0; jj < itp; ++jj) {
+            const E e = L[jj * LPT + sub];
+            const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+            const R f = Pair<R, MODE, true>::f(e.w, dx, dy, dz, a, eps2, rcut2, rcut2s);
+            px += f * dx;
+            py += f * dy;
+            pz += f * dz;
+            if (POT) ppot += pair_potential<true>((double)e.w, dx, dy, dz, a, (double)eps2, (double)rcut2);
+        }
+        const int m0 = kListCap - itm * LPT + sub;
+#pragma unroll 4
+        for (int jj = 0; jj < itm; ++jj) {
+            const E e = L[m0 + jj * LPT];
+            const R dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+            const R f = Pair<R, MODE, MODE == JTP_SPLIT_POLY4>::f(e.w, dx, dy, dz, a, eps2, rcut2, rcut2s);
+            px += f * dx;
+            py += f * dy;
+            pz += f * dz;
+            if (POT) ppot += pair_potential<false>((double)e.w, dx, dy, dz, a, (double)eps2, (double)rcut2);
+        }
+#pragma unroll
+        for (int off = GL; off < 64; off <<= 1) {  // combine the LPT partial sums of a target
+            px += __shfl_xor(px, off, 64);
+            py += __shfl_xor(py, off, 64);
+            pz += __shfl_xor(pz, off, 64);
+            if (POT) ppot += __shfl_xor(ppot, off, 64);
+        }
+        if (POT) spot += ppot;
+        if (sizeof(R) == 4) {  // compensated accumulation of the chunk partials
+            R y = px - cx, t = sx + y;
+            cx = (t - sx) - y;
+            sx = t;
+            y = py - cy; t = sy + y; cy = (t - sy) - y; sy = t;
+            y = pz - cz; t = sz + y; cz = (t - sz) - y; sz = t;
+        } else {
+            sx += px;
+            sy += py;
+            sz += pz;
+        }
+        my_inter += (unsigned long long)(np_ + nm_) * (unsigned long long)nvalid;
+        my_mono += (unsigned long long)(nm_ + mono_inside) * (unsigned long long)nvalid;
+        mono_inside = 0;
+    };
+
+    while (sp > 0) {
+        if (++steps > a.max_steps) {  // a finite hierarchy never gets here; report instead of spinning
+            if (lane == 0) atomicExch(&a.counters[kCtrErr], 2);
+            break;
+        }
+        int k = sp < 64 ? sp : 64;
+        const int kl = (kListCap - 2 * kPad - lp - lm) >> app_shift;
+        k = k < kl ? k : kl;
+        const int ks = kStackCap - sp > 1 ? kStackCap - sp : 1;
+        k = k < ks ? k : ks;
+        sp -= k;
+        const bool has = lane < k;
+        const int32_t popped = has ? S[sp + lane] : a.root;
+        wave_lds_fence();
+        const bool bad = has && (popped >= (int32_t)(a.n - 1) || ~popped >= (int32_t)a.n);
+        if (bad) atomicExch(&a.counters[kCtrErr], 3);  // malformed child index: skipped
+        const bool live = has && !bad;
+        const int32_t node = live ? popped : a.root;
+
+        // Classification, free of branches except the memory traffic: an internal node's record is
+        // (box centre, first) and (box extent, last); a particle child's is its own record and a
+        // zero extent.
+        const bool isnode = node >= 0;
+        const uint4* rec0 = isnode ? a.geo + 2 * node : a.parts + ~node;
+        const uint4 g0 = rec0[0];
+        const uint4 g1r = (isnode ? rec0 + 1 : rec0)[0];
+        // an internal node's monopoles and children are read with its box (one memory round
+        // trip per step for all three; particle children read node 0's, a shared cache line)
+        const int32_t snode = isnode ? node : 0;
+        const M mr0 = mono[2 * snode], mr1 = mono[2 * snode + 1];
+        const int2 kd = a.kids[snode];
+        if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);  // previous step's
+        const uint4 g1 = isnode ? g1r : make_uint4(0u, 0u, 0u, 0u);
+        const int32_t first = isnode ? (int32_t)g0.w : ~node;
+        const int32_t cnt = isnode ? (int32_t)g1.w - first + 1 : 1;
+        // box-to-box gap and farthest-corner distance in fixed-point units: the node spans
+        // centre -+ (extent / 2 + 1/2), the group centre -+ (gh + 1/2) (ghu); the periodic minimum
+        // image comes from the 32-bit wrap of the centre offset
+        R gap2 = 0, far2 = 0;
+        {
+            const uint32_t nc[3] = {g0.x, g0.y, g0.z}, ne[3] = {g1.x, g1.y, g1.z};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const R adc = fabs((R)(int32_t)(nc[ax] - gc[ax]));
+                const R t = (R)ne[ax] * (R)0.5 + ghu[ax];
+                const R gp = fmax(adc - t, (R)0);
+                gap2 += gp * gp;
+                const R fr = adc + t;
+                far2 += fr * fr;
+            }
+        }
+        const R ext = (R)max(max(g1.x, g1.y), g1.z);
+        const bool reach = live && gap2 < rcut2u;
+        const bool inside = far2 < rcut2u;
+        // accept the two monopoles / take the bucket's particles / open
+        const bool acc = reach && isnode && ext * ext < (inside ? theta2 : thetac2) * gap2;
+        const bool bucket = reach && !acc && cnt <= a.leaf;
+        const bool open = reach && !acc && !bucket;
+        const E m0 = mono_entry(mr0, gc, scale), m1 = mono_entry(mr1, gc, scale);
+        const bool h0 = acc && m0.w != (R)0, h1 = acc && m1.w != (R)0;
+        const int nq = (h0 ? 1 : 0) + (h1 ? 1 : 0);
+        // FP32: monopoles of a node entirely within r_cut take the (exact there) cut pair law and sit
+        // on the particle side; the others take the tail law from the top of the list
+        const bool cutlaw = sizeof(R) == 4 && inside && MODE != JTP_SPLIT_POLY4;
+        const int nb = bucket ? cnt : 0;
+        const int n4 = cutlaw ? nq : 0;
+        const int nt = cutlaw ? 0 : nq;
+        // one scan over (bucket particles | cut-law monopoles << 11 | tail monopoles << 19):
+        // per wave at most 64 x 16, 128 and 128 entries
+        const int packed = nb | (n4 << 11) | (nt << 19);
+        const int incl = wave_incl_scan_dpp(packed);
+        const int totl = __builtin_amdgcn_readlane(incl, 63);
+        const int btot = totl & 0x7ff, ptot = btot + ((totl >> 11) & 0xff), ttot = totl >> 19;
+        const int bexcl = (incl & 0x7ff) - nb;
+        if (nq) {  // particle side after the bucket particles, or top down
+            const bool top = !cutlaw;
+            int o = top ? kListCap - 1 - (lm + (incl >> 19) - nt) : lp + btot + ((incl >> 11) & 0xff) - n4;
+            const int d = top ? -1 : 1;
+            if (h0) {
+                L[o] = m0;
+                o += d;
+            }
+            if (h1) L[o] = m1;
+        }
+        // bucket particles, one lane per particle: item w of the step belongs to the bucket lane
+        // whose run [bexcl, bexcl + nb) holds it, found by a running maximum over lane-indexed
+        // run starts (64 items per round)
+        // The first round's particles are read and listed during the next step (their loads
+        // overlap that step's node reads); later rounds are listed now.
+        int nidx = -1, ndst = 0;
+        for (int r0 = 0; r0 < btot; r0 += 64) {
+            O[lane] = 0u;
+            wave_lds_fence();
+            if (nb && bexcl < r0 + 64 && bexcl + nb > r0) O[bexcl > r0 ? bexcl - r0 : 0] = (uint32_t)lane + 1u;
+            wave_lds_fence();
+            // every lane takes part in the lane reads (a read from an inactive lane returns 0)
+            const int ow = max((int)wave_incl_max_dpp(O[lane]) - 1, 0);
+            const int fb = __shfl(first, ow, 64), eb = __shfl(bexcl, ow, 64);
+            const int w = r0 + lane;
+            if (w < btot) {
+                if (r0 == 0) {
+                    nidx = fb + (w - eb);
+                    ndst = lp + w;
+                } else {
+                    L[lp + w] = part_entry<R, E>(a.parts[fb + (w - eb)], gc, scale, qpart);
+                }
+            }
+        }
+        // pushes
+        const uint64_t om = __ballot(open);
+        const int nopen = __popcll(om);
+        if (sp + 2 * nopen > kStackPhys) {
+            if (lane == 0) atomicExch(&a.counters[kCtrErr], 1);
+        } else if (open) {
+            const int rank = __popcll(om & lt);
+            S[sp + 2 * rank] = kd.y;
+            S[sp + 2 * rank + 1] = kd.x;
+        }
+        if (sp + 2 * nopen <= kStackPhys) sp += 2 * nopen;
+        // the counts are wave-uniform: keep them in scalar registers
+        pidx = nidx;
+        pdst = ndst;
+        lp = __builtin_amdgcn_readfirstlane(lp + ptot);
+        lm = __builtin_amdgcn_readfirstlane(lm + ttot);
+        sp = __builtin_amdgcn_readfirstlane(sp);
+        mono_inside += ptot - btot;
+        wave_lds_fence();
+        if (lp + lm >= kFlushAt) {
+            drain();
+            flush(lp, lm);
+            lp = 0;
+            lm = 0;
+            wave_lds_fence();
+        }
+    }
+    drain();
+    if (lp + lm > 0) flush(lp, lm);
+    wave_lds_fence();
+
+    if (valid && sub == 0) {
+        const R sg = (int32_t)rec.w > 0 ? (R)1 : (R)-1;
+        R ox = sg * sx, oy = sg * sy, oz = sg * sz;
+        if (a.pm_acc) {
+            const R* pm = (const R*)a.pm_acc;
+            ox += pm[3 * ti];
+            oy += pm[3 * ti + 1];
+            oz += pm[3 * ti + 2];
+        }
+        const int64_t o = (int64_t)a.perm[ti];
+        R* out = (R*)a.out;
+        out[3 * o] = ox;
+        out[3 * o + 1] = oy;
+        out[3 * o + 2] = oz;
+        if (POT) a.pot[o] = -spot;  // Phi_short = -sum_j q_j erfc(rho / 2 r_s) / rho
+    }
+}
+
+// XCD-local queues: the work items (in Morton order) are cut into kXcds contiguous ranges and a
+// workgroup first drains the range of the XCD it runs on (workgroup i -> XCD i % 8), so
+// concurrent groups on one XCD share tree nodes in that XCD's L2; then it steals from the
+// following ranges.  Returns the next item or -1.  Each item's result does not depend on who
+// computes it.
+__device__ inline int next_item(int32_t n_items, int32_t* heads, int xcd, int& steal) {
+    const int lane = threadIdx.x & 63;
+    while (steal < kXcds) {
+        const int q = (xcd + steal) % kXcds;
+        const int32_t start = (int32_t)((int64_t)n_items * q / kXcds);
+        const int32_t end = (int32_t)((int64_t)n_items * (q + 1) / kXcds);
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&heads[q], 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+        if (start + t < end) return start + t;
+        ++steal;
+    }
+    return -1;
+}
+
+// One wavefront per group of one box.
+template <typename R, int MODE, int WAVES, int LPT, bool POT = false>
+__global__ __launch_bounds__(64 * WAVES) void k_walk(WalkArgs a) {
+    __shared__ WalkLds<R, LPT> lds_s[WAVES];
+    WalkLds<R, LPT>& lds = lds_s[threadIdx.x >> 6];
+    unsigned long long my_inter = 0, my_mono = 0;
+    const int32_t n_groups = a.counters[kCtrNGroups];
+    const int xcd = blockIdx.x % kXcds;
+    int steal = 0;
+    for (int g; (g = next_item(n_groups, a.counters + kCtrXcd, xcd, steal)) >= 0;)
+        walk_group<R, MODE, LPT, POT>(a, g, lds, my_inter, my_mono);
+    if ((threadIdx.x & 63) == 0 && my_inter) {
         atomicAdd(a.inter, my_inter);
         atomicAdd(a.inter + 1, my_mono);
+    }
+}
+
+// One persistent launch over the groups of several boxes: work item t is group t - goff[b] of box
+// b (goff[b] <= t < goff[b + 1], boxes in order); each box's arguments, counters and outputs are
+// its own, so every box's result is the one its own launch gives.
+template <typename R, int MODE, int WAVES, int LPT>
+__global__ __launch_bounds__(64 * WAVES) void k_walk_boxes(const WalkArgs* __restrict__ boxes,
+                                                         const int32_t* __restrict__ goff, int32_t nbox,
+                                                         int32_t* heads) {
+    __shared__ WalkLds<R, LPT> lds_s[WAVES];
+    WalkLds<R, LPT>& lds = lds_s[threadIdx.x >> 6];
+    const int32_t n_items = goff[nbox];
+    const int xcd = blockIdx.x % kXcds;
+    int steal = 0;
+    for (int t; (t = next_item(n_items, heads, xcd, steal)) >= 0;) {
+        int lo = 0, hi = nbox;  // largest b with goff[b] <= t
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (goff[mid] <= t) lo = mid; else hi = mid;
+        }
+        const int b = __builtin_amdgcn_readfirstlane(lo);
+        const WalkArgs& a = boxes[b];
+        unsigned long long my_inter = 0, my_mono = 0;
+        walk_group<R, MODE, LPT, false>(a, t - goff[b], lds, my_inter, my_mono);
+        if ((threadIdx.x & 63) == 0 && my_inter) {
+            atomicAdd(a.inter, my_inter);
+            atomicAdd(a.inter + 1, my_mono);
+        }
     }
 }
 
@@ -547,16 +590,22 @@ hipError_t write_pm_only(Ctx& c, int64_t n, void* acc_out, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int MODE, int WAVES, int LPT, bool POT = false>
-static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
+// Persistent grid: enough workgroups for every work item, at most 16 per CU (what fits).
+template <int WAVES>
+static unsigned walk_blocks(int64_t items) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int64_t blocks = (gmax + WAVES - 1) / WAVES;
+    int64_t blocks = (items + WAVES - 1) / WAVES;
     const int64_t cap = (int64_t)cus * 16;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    k_walk<R, MODE, WAVES, LPT, POT><<<dim3((unsigned)blocks), dim3(64 * WAVES), 0, s>>>(a);
+    return (unsigned)blocks;
+}
+
+template <typename R, int MODE, int WAVES, int LPT, bool POT = false>
+static hipError_t launch_walk(const WalkArgs& a, int64_t gmax, hipStream_t s) {
+    k_walk<R, MODE, WAVES, LPT, POT><<<dim3(walk_blocks<WAVES>(gmax)), dim3(64 * WAVES), 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -567,8 +616,8 @@ static hipError_t launch_walk_mode(int mode, const WalkArgs& a, int64_t gmax, hi
     return launch_walk<R, JTP_SPLIT_POLY4, WAVES, LPT>(a, gmax, s);
 }
 
-hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
-    WalkArgs a{};
+static void fill_args(Ctx& c, int64_t n, void* acc_out, WalkArgs& a) {
+    a = WalkArgs{};
     a.parts = c.parts;
     a.geo = c.node_geo;
     a.kids = c.node_kids;
@@ -601,6 +650,11 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
     a.tscale = c.poly_tscale;
     a.tscale_m = c.poly_tscale_m;
     a.escale = (float)(c.p.r_s > 0 ? -1.4426950408889634 / (4.0 * c.p.r_s * c.p.r_s) : 0.0);
+}
+
+hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
+    WalkArgs a;
+    fill_args(c, n, acc_out, a);
     const int64_t gmax = c.n_groups_max;
     const int mode = c.p.split_mode;
     const int lpt = lanes_per_target(c.p.group_size);
@@ -625,6 +679,68 @@ hipError_t walk(Ctx& c, int64_t n, void* acc_out, hipStream_t s, double* pot) {
     if (lpt == 4) return launch_walk_mode<float, 4, 4>(mode, a, gmax, s);
     if (lpt == 2) return launch_walk_mode<float, 4, 2>(mode, a, gmax, s);
     return launch_walk_mode<float, 4, 1>(mode, a, gmax, s);
+}
+
+// ---- one walk launch over several boxes ----
+
+size_t walk_args_bytes() { return sizeof(WalkArgs); }
+
+void walk_fill_args(Ctx& c, int64_t n, void* acc_out, void* dst) { fill_args(c, n, acc_out, *(WalkArgs*)dst); }
+
+int walk_batch_key(const Ctx& c) {
+    return (c.p.precision * 4 + c.p.split_mode) * 8 + lanes_per_target(c.p.group_size);
+}
+
+// goff[b] = groups of the boxes before b (each box's count as its own build wrote it), goff[nbox] the
+// total; the kXcds queue heads after it start at 0.  One workgroup.
+__global__ __launch_bounds__(256) void k_walk_boxes_offsets(const WalkArgs* __restrict__ boxes, int32_t nbox,
+                                                            int32_t* __restrict__ ctrl) {
+    __shared__ int32_t cnt_s[256];
+    int32_t acc = 0;  // thread 0's running total
+    for (int b0 = 0; b0 < nbox; b0 += 256) {
+        const int b = b0 + (int)threadIdx.x;
+        cnt_s[threadIdx.x] = b < nbox ? boxes[b].counters[kCtrNGroups] : 0;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < 256 && b0 + i < nbox; ++i) {
+                ctrl[b0 + i] = acc;
+                acc += cnt_s[i];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctrl[nbox] = acc;
+    if (threadIdx.x < kXcds) ctrl[nbox + 1 + threadIdx.x] = 0;
+}
+
+template <typename R, int MODE, int WAVES, int LPT>
+static hipError_t launch_walk_boxes(const WalkArgs* d_args, int32_t nbox, int32_t* d_ctrl, int64_t items_max,
+                                    hipStream_t s) {
+    k_walk_boxes<R, MODE, WAVES, LPT><<<dim3(walk_blocks<WAVES>(items_max)), dim3(64 * WAVES), 0, s>>>(
+        d_args, d_ctrl, nbox, d_ctrl + nbox + 1);
+    return hipGetLastError();
+}
+
+template <typename R, int WAVES, int LPT>
+static hipError_t launch_walk_boxes_mode(int mode, const WalkArgs* d, int32_t nbox, int32_t* ctrl, int64_t items,
+                                         hipStream_t s) {
+    if (mode == JTP_SPLIT_ERFC_GAUSS) return launch_walk_boxes<R, JTP_SPLIT_ERFC_GAUSS, WAVES, LPT>(d, nbox, ctrl, items, s);
+    if (mode == JTP_SPLIT_ERFC) return launch_walk_boxes<R, JTP_SPLIT_ERFC, WAVES, LPT>(d, nbox, ctrl, items, s);
+    return launch_walk_boxes<R, JTP_SPLIT_POLY4, WAVES, LPT>(d, nbox, ctrl, items, s);
+}
+
+hipError_t walk_boxes(const void* d_args, int32_t nbox, int32_t* d_ctrl, int64_t items_max, int key, hipStream_t s) {
+    if (nbox < 1) return hipSuccess;
+    const WalkArgs* d = (const WalkArgs*)d_args;
+    k_walk_boxes_offsets<<<dim3(1), dim3(256), 0, s>>>(d, nbox, d_ctrl);
+    const int lpt = key % 8, mode = (key / 8) % 4, prec = key / 32;
+    if (prec == JTP_FP64) {
+        if (lpt == 4) return launch_walk_boxes_mode<double, 2, 4>(mode, d, nbox, d_ctrl, items_max, s);
+        if (lpt == 2) return launch_walk_boxes_mode<double, 2, 2>(mode, d, nbox, d_ctrl, items_max, s);
+        return launch_walk_boxes_mode<double, 2, 1>(mode, d, nbox, d_ctrl, items_max, s);
+    }
+    if (lpt == 4) return launch_walk_boxes_mode<float, 4, 4>(mode, d, nbox, d_ctrl, items_max, s);
+    if (lpt == 2) return launch_walk_boxes_mode<float, 4, 2>(mode, d, nbox, d_ctrl, items_max, s);
+    return launch_walk_boxes_mode<float, 4, 1>(mode, d, nbox, d_ctrl, items_max, s);
 }
 
 }  // namespace jtp

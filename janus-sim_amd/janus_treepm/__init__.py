@@ -33,7 +33,7 @@ STAGE_KEYS, STAGE_PERM, STAGE_NODES, STAGE_MONO, STAGE_RHO, STAGE_PHI, STAGE_ACC
     1, 2, 3, 4, 5, 6, 7, 8, 9)
 STAGE_RHO2, STAGE_PHI2 = 10, 11
 STEP_DKD, STEP_KDK, STEP_GRAPH = 0, 1, 0x100
-ENS_SEQUENTIAL, ENS_GRAPH = 0x1, 0x100
+ENS_SEQUENTIAL, ENS_WALK_PER_BOX, ENS_GRAPH = 0x1, 0x2, 0x100
 
 # every symbol declared in include/janus_treepm.h
 EXPORTED = [
@@ -468,14 +468,16 @@ class Ensemble:
         if rc != JTP_OK:
             raise JtpError(rc, self.lib.jtp_ensemble_last_error(self.h).decode())
 
-    def compute_forces_device(self, n, d_pos, d_species, d_acc, sequential=False, graph=False, stream=None):
+    def compute_forces_device(self, n, d_pos, d_species, d_acc, sequential=False, graph=False, per_box_walk=False,
+                              stream=None):
         nb = self.n_boxes
         ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
         nn = (C.c_int64 * nb)(*[int(x) for x in n])
         pp = (C.c_void_p * nb)(*[ptr(d) for d in d_pos])
         ss = (C.c_void_p * nb)(*[ptr(d) for d in d_species])
         aa = (C.c_void_p * nb)(*[ptr(d) for d in d_acc])
-        flags = (ENS_SEQUENTIAL if sequential else 0) | (ENS_GRAPH if graph else 0)
+        flags = ((ENS_SEQUENTIAL if sequential else 0) | (ENS_GRAPH if graph else 0)
+                 | (ENS_WALK_PER_BOX if per_box_walk else 0))
         self._check(self.lib.jtp_ensemble_compute_forces(self.h, nn, pp, ss, aa, flags, stream))
 
     def stats(self, box: int = -1) -> dict:

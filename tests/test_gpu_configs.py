@@ -3,7 +3,7 @@
 Ensemble: the 48 lattice + Zel'dovich boxes of synth.ensemble_c3 (N_b = 2 n^3 log2-uniform in
 [2 x 32^3, 2 x 128^3], 3.98e7 particles in total, PM 32^3 - 128^3, L ~ U[100, 500], |m-|/m+ ~ U[5, 40]):
 every box's FP32 accelerations from jtp_ensemble_compute_forces are bit-identical to a single-context
-jtp_compute_forces of that box, in every ensemble mode.
+jtp_compute_forces of that box, in every ensemble mode (one walk launch over all boxes or one per box).
 
 Large box: N = 2 x 256^3 + 7.2e6 m+ = 40,754,432 particles of the clustered recipe with 8000 halos,
 L = 300, PM 512^3, theta = 0.5: per-species deposited mass conservation, bit-identical reruns, the FP32
@@ -44,11 +44,12 @@ def test_ensemble_48_boxes_match_single_boxes(jtp):
         assert np.all(np.isfinite(ref[-1]))
     ens = jtp.Ensemble([_params(jtp, w) for w in ws], [w.m_plus for w in ws], [w.m_minus_abs for w in ws],
                        n_streams=4)
-    for mode in ("concurrent", "sequential", "graph", "graph"):
+    for mode in ("concurrent", "sequential", "graph", "graph", "per_box", "per_box graph", "per_box graph"):
         print(f"  mode {mode}", flush=True)
         for d in dacc:
             d.upload(np.full((d.shape[0], 3), np.nan, np.float32))
-        ens.compute_forces_device(n, dpos, dsp, dacc, sequential=(mode == "sequential"), graph=(mode == "graph"))
+        ens.compute_forces_device(n, dpos, dsp, dacc, sequential="sequential" in mode, graph="graph" in mode,
+                                  per_box_walk="per_box" in mode)
         for b, d in enumerate(dacc):
             assert np.array_equal(d.download(), ref[b]), f"mode {mode} box {b}"
     ens.close()

@@ -1,6 +1,8 @@
 """GPU ensembles (jtp_ensemble_*): every box's forces equal jtp_compute_forces on that box alone,
-bit for bit, whether the boxes run concurrently over internal streams, sequentially, or as a
-replayed graph of the whole batch; empty boxes and uneven parameters included."""
+bit for bit, whether the boxes are prepared concurrently over internal streams or sequentially, walked
+by one launch over all boxes' groups (the default; also with boxes of different group sizes, three
+launches, and a box without a tree) or box by box (JTP_ENS_WALK_PER_BOX), eagerly or as a replayed
+graph of the whole batch; empty boxes and uneven parameters included."""
 import numpy as np
 import pytest
 
@@ -40,10 +42,12 @@ def test_ensemble_matches_single_boxes(jtp, precision):
     dsp = [jtp.DeviceArray.from_host(w.species) for w in ws]
     dacc = [jtp.DeviceArray((w.n, 3), rt) for w in ws]
     n = [w.n for w in ws]
-    for mode in ["concurrent", "sequential", "graph", "graph", "concurrent", "graph"]:
+    for mode in ["concurrent", "sequential", "graph", "graph", "per_box", "per_box sequential",
+                 "per_box graph", "per_box graph", "concurrent", "graph"]:
         for d in dacc:
             d.upload(np.full((d.shape[0], 3), np.nan, rt))
-        ens.compute_forces_device(n, dpos, dsp, dacc, sequential=(mode == "sequential"), graph=(mode == "graph"))
+        ens.compute_forces_device(n, dpos, dsp, dacc, sequential="sequential" in mode, graph="graph" in mode,
+                                  per_box_walk="per_box" in mode)
         for b, d in enumerate(dacc):
             assert np.array_equal(d.download(), ref[b]), f"mode {mode} box {b}"
         st = ens.stats()
@@ -54,6 +58,39 @@ def test_ensemble_matches_single_boxes(jtp, precision):
     ens.compute_forces_device(n2, dpos, dsp, dacc, graph=True)
     for b in (0, 2, 3, 4):
         assert np.array_equal(dacc[b].download(), ref[b])
+    ens.compute_forces_device(n2, dpos, dsp, dacc, per_box_walk=True)
+    for b in (0, 2, 3, 4):
+        assert np.array_equal(dacc[b].download(), ref[b])
+    ens.close()
+
+
+def test_one_walk_mixed_boxes(jtp):
+    """Boxes with groups of 16, 32 and 64 (three walk launches) and a box without a tree (PM only)."""
+    print("case: one-walk ensemble with mixed group sizes and a PM-only box", flush=True)
+    ws = _boxes()
+    params = [_params(jtp, w, 0) for w in ws]
+    params[0].group_size, params[1].group_size, params[3].group_size = 16, 64, 16
+    params[2].tree_enabled = 0
+    ref = []
+    for w, p in zip(ws, params):
+        ctx = jtp.ForceContext(p, w.m_plus, w.m_minus_abs)
+        ref.append(ctx.compute_forces(jtp.to_fixed(w.pos, w.box_size), w.species))
+        ctx.close()
+    ens = jtp.Ensemble(params, [w.m_plus for w in ws], [w.m_minus_abs for w in ws], n_streams=2)
+    dpos = [jtp.DeviceArray.from_host(jtp.to_fixed(w.pos, w.box_size)) for w in ws]
+    dsp = [jtp.DeviceArray.from_host(w.species) for w in ws]
+    dacc = [jtp.DeviceArray((w.n, 3), np.float32) for w in ws]
+    n = [w.n for w in ws]
+    for graph in (False, True, True):
+        for d in dacc:
+            d.upload(np.full((d.shape[0], 3), np.nan, np.float32))
+        ens.compute_forces_device(n, dpos, dsp, dacc, graph=graph)
+        for b, d in enumerate(dacc):
+            assert np.array_equal(d.download(), ref[b]), f"graph {graph} box {b}"
+        st = ens.stats()
+        assert st["interactions"] > 0
+        for b in (0, 1, 3, 4):
+            assert ens.stats(b)["interactions"] > 0
     ens.close()
 
 
