@@ -463,19 +463,35 @@ class Ensemble:
         if rc != JTP_OK:
             raise JtpError(rc, "jtp_ensemble_create failed")
         self.h, self.n_boxes = h, nb
+        self.real = [np.float64 if p.precision == FP64 else np.float32 for p in params]
 
     def _check(self, rc):
         if rc != JTP_OK:
             raise JtpError(rc, self.lib.jtp_ensemble_last_error(self.h).decode())
 
+    @staticmethod
+    def _arg(d, dtype, count, name):
+        """Raw pointer of a device argument; a DeviceArray must hold `count` elements of `dtype`."""
+        if not isinstance(d, DeviceArray):
+            return d
+        if d.dtype != np.dtype(dtype):
+            raise ValueError(f"{name}: dtype {d.dtype}, expected {np.dtype(dtype)}")
+        if d.nbytes < count * np.dtype(dtype).itemsize:
+            raise ValueError(f"{name}: {d.nbytes} bytes, need {count * np.dtype(dtype).itemsize}")
+        return d.ptr
+
     def compute_forces_device(self, n, d_pos, d_species, d_acc, sequential=False, graph=False, per_box_walk=False,
                               stream=None):
         nb = self.n_boxes
-        ptr = lambda d: d.ptr if isinstance(d, DeviceArray) else d
-        nn = (C.c_int64 * nb)(*[int(x) for x in n])
-        pp = (C.c_void_p * nb)(*[ptr(d) for d in d_pos])
-        ss = (C.c_void_p * nb)(*[ptr(d) for d in d_species])
-        aa = (C.c_void_p * nb)(*[ptr(d) for d in d_acc])
+        if not (len(n) == len(d_pos) == len(d_species) == len(d_acc) == nb):
+            raise ValueError(f"expected {nb} entries per argument")
+        n = [int(x) for x in n]
+        nn = (C.c_int64 * nb)(*n)
+        pp = (C.c_void_p * nb)(*[self._arg(d, np.uint32, 3 * max(k, 0), f"pos[{b}]") for b, (d, k) in enumerate(zip(d_pos, n))])
+        ss = (C.c_void_p * nb)(*[self._arg(d, np.int8, max(k, 0), f"species[{b}]")
+                                 for b, (d, k) in enumerate(zip(d_species, n))])
+        aa = (C.c_void_p * nb)(*[self._arg(d, self.real[b], 3 * max(k, 0), f"acc[{b}]")
+                                 for b, (d, k) in enumerate(zip(d_acc, n))])
         flags = ((ENS_SEQUENTIAL if sequential else 0) | (ENS_GRAPH if graph else 0)
                  | (ENS_WALK_PER_BOX if per_box_walk else 0))
         self._check(self.lib.jtp_ensemble_compute_forces(self.h, nn, pp, ss, aa, flags, stream))

@@ -102,6 +102,20 @@ def test_ensemble_rejects_bad_arguments(jtp):
         ens.compute_forces_device([-1, 5], [0, 0], [0, 0], [0, 0])
     with pytest.raises(jtp.JtpError):
         ens.stats(5)
+    # device arrays of the wrong type or size are refused before any kernel runs
+    dpos = [jtp.DeviceArray.from_host(jtp.to_fixed(w.pos, w.box_size)) for w in ws]
+    dsp = [jtp.DeviceArray.from_host(w.species) for w in ws]
+    n = [w.n for w in ws]
+    wrong_acc = [jtp.DeviceArray((w.n, 3), np.float64) for w in ws]  # FP32 boxes
+    short_acc = [jtp.DeviceArray((w.n - 1, 3), np.float32) for w in ws]
+    with pytest.raises(ValueError):
+        ens.compute_forces_device(n, dpos, dsp, wrong_acc)
+    with pytest.raises(ValueError):
+        ens.compute_forces_device(n, dpos, dsp, short_acc)
+    with pytest.raises(ValueError):
+        ens.compute_forces_device(n[:1], dpos, dsp, short_acc)
+    for a in dpos + dsp + wrong_acc + short_acc:
+        a.free()
     ens.close()
     bad = _params(jtp, ws[0], 0)
     bad.n_cells = 48
