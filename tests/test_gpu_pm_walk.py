@@ -233,6 +233,25 @@ def test_walk_theta0_direct(jtp, orc, precision, mode, gs):
         assert np.median(e) < 3e-6 and np.percentile(e, 99) < 1e-4, (np.median(e), np.percentile(e, 99))
 
 
+@pytest.mark.parametrize("precision,leaf", [(0, 1), (0, 16), (1, 16)])
+def test_walk_theta0_leaf_sizes(jtp, orc, precision, leaf):
+    """theta = 0 with buckets of up to 1 and 16 particles: a traversal step then lists up to 64 x 16
+    bucket particles over several rounds of the lane-per-particle append, one of them deferred to the
+    next step -- against the direct sum."""
+    print(f"case: walk theta=0 leaf={leaf} precision={precision}", flush=True)
+    w = synth.clustered(n_total=40_000, box_size=100.0, n_cells=32, n_halos=30, seed=13)
+    pos = jtp.to_fixed(w.pos, w.box_size)
+    ctx = _ctx(jtp, w, precision=precision, pm_enabled=0, theta=0.0, leaf_size=leaf)
+    acc = ctx.compute_forces(pos, w.species).astype(np.float64)
+    ref = orc.short_direct(pos, w.species, w.m_plus, w.m_minus_abs, w.box_size, w.softening, w.r_cut, w.r_s, 0)
+    e = _rel(acc, ref)
+    print("leaf %d: median %.2e p99 %.2e max %.2e" % (leaf, np.median(e), np.percentile(e, 99), e.max()))
+    if precision == 1:
+        assert np.median(e) < 1e-13 and np.max(e) < 1e-9, (np.median(e), np.max(e))
+    else:
+        assert np.median(e) < 3e-6 and np.percentile(e, 99) < 1e-4, (np.median(e), np.percentile(e, 99))
+
+
 @pytest.mark.parametrize("level,precision,gs", [(4, 1, 32), (4, 0, 32), (4, 1, 64), (4, 0, 64),
                                                 (3, 1, 32), (3, 0, 32), (3, 0, 64)])
 def test_walk_theta0_seed_levels(jtp, orc, level, precision, gs):
