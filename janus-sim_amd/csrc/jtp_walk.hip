@@ -8,12 +8,15 @@
 // traversal is lane-parallel: each step pops up to 64 nodes from a per-wave LDS
 // stack, every lane classifies one node against the group's bounding box
 // (prune beyond r_cut / accept the two monopoles / take the bucket's particles /
-// open), and a wavefront prefix sum appends the accepted sources to an LDS
-// interaction list while a ballot pushes the opened nodes' children.  When the
-// list is long enough every lane evaluates its target against all entries
-// (broadcast LDS reads).  Sources are stored relative to the group centre, so
-// the pair displacement is a plain subtraction; periodic minimum image comes
-// from wrapping 32-bit fixed-point differences when the entry is built.
+// open), and a wavefront prefix sum gives every source its slot in an LDS
+// interaction list (bucket particles are then listed one lane per particle)
+// while a ballot pushes the opened nodes' children.  When the list is long
+// enough every lane evaluates its target against all entries (broadcast LDS
+// reads).  Sources are stored relative to the group centre, so the pair
+// displacement is a plain subtraction; periodic minimum image comes from
+// wrapping 32-bit fixed-point differences when the entry is built.  The groups
+// of one box (k_walk) or of several boxes (k_walk_boxes) are pulled from queues
+// by persistent workgroups.
 //
 // Pair law (per entry j with signed charge q_j = sigma_j G |m_j|):
 //   a_i += sigma_i q_j S(.) (x_j - x_i) / (r^2 + eps^2)^{3/2},  zero unless r^2 < r_cut^2
@@ -271,7 +274,7 @@ __device__ __forceinline__ void walk_group(const WalkArgs& a, int g, WalkLds<R, 
     }
     wave_lds_fence();
 
-    // particle entries grow up from L[0], monopole entries down from L[kListCap - 1]
+    // particles and cut-law monopoles grow up from L[0], tail-law monopoles down from L[kListCap - 1]
     auto drain = [&]() {  // list the pending bucket particles before the list is read
         if (pidx >= 0) L[pdst] = part_entry<R, E>(a.parts[pidx], gc, scale, qpart);
         pidx = -1;
