@@ -124,13 +124,18 @@ void make_split_poly(Ctx& c) {
         cheb_fit(Vp, kPolyDegPair, c.poly.cp,
                  [&](double v) { return cinv * cinv * cinv * split_excess_x3(std::sqrt(v) * cinv); });
         c.poly_tscale = (float)(2.0 / Vp);
-        cheb_fit(Xm, kPolyDegMono, c.poly.cm, [&](double x) { return split_scaled(x, true); });
+        // monopoles: the same H out to kTailExt r_cut
+        const double Vm = (kTailExt * kTailExt * c.p.r_cut * c.p.r_cut + c.p.softening * c.p.softening) * 1.0002 + 1e-30;
+        cheb_fit(Vm, kPolyDegTail, c.poly.cm,
+                 [&](double v) { return cinv * cinv * cinv * split_excess_x3(std::sqrt(v) * cinv); });
+        c.poly_tscale_m = (float)(2.0 / Vm);
+        c.poly_rho2_tail = (float)(Vm / 1.0002);
     } else {
         cheb_fit(Xp, kPolyDeg, c.poly.cp, [&](double x) { return split_direct(x, false); });
         c.poly_tscale = (float)(1.0 / (c.p.r_s * Xp));
         cheb_fit(Xm, kPolyDeg, c.poly.cm, [&](double x) { return split_scaled(x, false); });
+        c.poly_tscale_m = (float)(1.0 / (c.p.r_s * Xm));
     }
-    c.poly_tscale_m = (float)(1.0 / (c.p.r_s * Xm));
 }
 
 static void free_particle_buffers(Ctx& c) {

@@ -70,11 +70,15 @@ __host__ __device__ inline uint64_t morton_fixed(uint32_t ux, uint32_t uy, uint3
 //    kPolyDegPair Chebyshev interpolant Pp(t), t = 2 rho^2 / Vp - 1 on [0, Vp = rho_max^2]:
 //    no rho, no separate S, and a faster-converging fit than S(x) itself;
 //  * particle pairs under erfc only: S(x) ~= Pp(t), t = 2x/Xp - 1, degree kPolyDeg;
-//  * monopoles (no cut under the erfc laws, x up to Xm ~ r_cut (1 + sqrt(3) theta) / 2 r_s):
-//    S(x) ~= exp(-x^2) Pm(t), t = min(2x/Xm - 1, 1), Pm interpolating erfcx(x) [+ 2x/sqrt(pi)]
-//    (degree kPolyDegMono, kPolyDeg under erfc only), which stays bounded past the fitted range.
+//  * monopoles under erfc + Gaussian (the node straddles r_cut; no cut at r_cut): the same H form
+//    fitted on [0, Vm = (kTailExt r_cut)^2 + eps^2] with degree kPolyDegTail, the force dropped beyond
+//    Vm where S(kTailExt r_cut) < 2e-4;
+//  * monopoles under erfc only (x up to Xm ~ r_cut (1 + sqrt(3) theta) / 2 r_s):
+//    S(x) ~= exp(-x^2) Pm(t), t = min(2x/Xm - 1, 1), Pm interpolating erfcx(x) (degree kPolyDeg),
+//    which stays bounded past the fitted range.
 constexpr int kPolyDegPair = 8;
-constexpr int kPolyDegMono = 8;
+constexpr int kPolyDegTail = 10;
+constexpr double kTailExt = 1.4;
 struct SplitPoly {
     float cp[kPolyDeg + 1];
     float cm[kPolyDeg + 1];
@@ -159,6 +163,7 @@ struct Ctx {
     SplitPoly poly{};
     float poly_tscale = 0.0f;    // particles: t = rho^2 * poly_tscale - 1 (erfc + Gaussian), rho * poly_tscale - 1
     float poly_tscale_m = 0.0f;  // monopoles
+    float poly_rho2_tail = 0.0f; // erfc + Gaussian monopoles: rho^2 beyond which the force is dropped
     double green_norm = 0.0;
     unsigned long long* host_pinned = nullptr;  // [0] interactions, [1] (n_groups | err << 32)
 

@@ -65,7 +65,8 @@ struct WalkArgs {
     double qp, qm;  // signed particle charges sigma G |m| for species +1 / -1
     SplitPoly poly;
     float tscale;    // particles: t = rho^2 * tscale - 1 (erfc + Gaussian) or rho * tscale - 1
-    float tscale_m;  // monopoles
+    float tscale_m;  // monopoles: t = rho^2 * tscale_m - 1 (erfc + Gaussian) or rho * tscale_m - 1
+    float rho2_tail; // erfc + Gaussian monopoles: rho^2 beyond which the force is dropped
     float escale;    // exp(-x^2) = exp2(r2s * escale)
     float theta2_f, theta_cut2_f;
 };
@@ -110,15 +111,21 @@ struct Pair<float, MODE, CUT> {
             // particle pair: S / rho^3 = 1 / rho^3 + H(rho^2)
             const float t = __builtin_fmaf(r2s, a.tscale, -1.0f);
             fv = q * (inv3 + horner<kPolyDegPair>(a.poly.cp, t));
+        } else if (MODE == JTP_SPLIT_ERFC_GAUSS) {
+            // monopole: the same form with H fitted out to kTailExt r_cut, where S is below 2e-4 and
+            // the monopole's force is dropped
+            const float t = __builtin_fmaf(r2s, a.tscale_m, -1.0f);
+            fv = q * (inv3 + horner<kPolyDegTail>(a.poly.cm, t));
+            return r2s < a.rho2_tail ? fv : 0.0f;
         } else if (CUT) {
             // particle pair, erfc law: direct interpolant of S(x)
             const float t = r2s * inv * a.tscale - 1.0f;
             fv = q * horner<kPolyDeg>(a.poly.cp, t) * inv3;
         } else {
-            // monopole: exp(-x^2) times the interpolant of the scaled factor (bounded beyond its range)
+            // monopole, erfc law: exp(-x^2) times the interpolant of the scaled factor (bounded
+            // beyond its range)
             const float t = fminf(r2s * inv * a.tscale_m - 1.0f, 1.0f);
-            constexpr int DM = MODE == JTP_SPLIT_ERFC_GAUSS ? kPolyDegMono : kPolyDeg;
-            fv = q * (__builtin_amdgcn_exp2f(r2s * a.escale) * horner<DM>(a.poly.cm, t)) * inv3;
+            fv = q * (__builtin_amdgcn_exp2f(r2s * a.escale) * horner<kPolyDeg>(a.poly.cm, t)) * inv3;
         }
         return (!CUT || r2s < rcut2s) ? fv : 0.0f;
     }
@@ -652,6 +659,7 @@ static void fill_args(Ctx& c, int64_t n, void* acc_out, WalkArgs& a) {
     // t = x * 2/X - 1 with x = rho/(2 r_s) and X the fit range (make_split_poly)
     a.tscale = c.poly_tscale;
     a.tscale_m = c.poly_tscale_m;
+    a.rho2_tail = c.poly_rho2_tail;
     a.escale = (float)(c.p.r_s > 0 ? -1.4426950408889634 / (4.0 * c.p.r_s * c.p.r_s) : 0.0);
 }
 
