@@ -148,7 +148,7 @@ struct Ctx {
     // PM
     int32_t *cell_start = nullptr;   // FP64: G^3 + 1 first sorted particle of each cell
     int32_t *tile_start = nullptr;   // FP32: G^3 / 512 + 1 first sorted particle of each 8^3-cell tile
-    float *dep_tbuf = nullptr;       // FP32 deposit: the face and beyond-face node sums of each tile (1170 per tile)
+    float *dep_tbuf = nullptr;       // FP32 deposit: each tile's record, its sums as floats (1729 per tile)
     int32_t *dense_slot = nullptr;   // G^3: slot of a dense cell's corner sums (written for dense cells only)
     void *dense_corner = nullptr;    // real [n / kDenseCell + 1][8 + 27] corner sums of dense-cell chunks
     void *rho = nullptr;             // real G^3 deposited mass per cell
@@ -225,7 +225,7 @@ hipError_t launch_cell_start(const uint64_t* keys, int64_t n, int shift, int64_t
 hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s);           // grid <- sum q W
 int64_t dep_chunk_cap(int64_t n);  // FP32 deposit: chunk entries for up to n particles
 int64_t dep_chunk_nodes();         // FP32 deposit: int64 node sums per chunk entry
-int64_t dep_tile_slots();          // FP32 deposit: face and beyond-face node sums per tile
+int64_t dep_tile_slots();          // FP32 deposit: floats per tile record (own blocks, beyond nodes)
 // phi <- Green * rho; matched = 1 multiplies by the gradient-stencil match (the force's Green
 // function when deconvolving; -1 = as the context's force), 0 leaves the potential unmatched
 hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched = -1);

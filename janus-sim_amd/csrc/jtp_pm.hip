@@ -5,8 +5,9 @@
 //   cic_scatter (FP64 CAS atomics into two grids), nbody_gpu_twopass.rs:85-158,
 //     and PmGrid::assign_mass, src/treepm/pm_grid.rs:63-103 -> one net grid rho+ - rho- from the
 //     Morton-sorted particles, no float atomics: FP32 k_dep_tile (each particle read once, integer
-//     LDS sums per 8^3-cell tile, a fixed-order merge of the tile faces); FP64 k_deposit(_both)
-//     (each node gathers its source cells in a fixed order).
+//     LDS sums per 8^3-cell tile) + k_dep_assemble (each tile's own node blocks written as whole rows,
+//     the lower neighbours' overlaps added in a fixed order); FP64 k_deposit(_both) (each node
+//     gathers its source cells in a fixed order).
 //   cufft_solve_device_filtered (cuFFT D2Z / apply_green_kernel / Z2D / normalize),
 //     cuda/cufft_wrapper.cu:148-208,290-295,350-398, and the rustfft passes of
 //     PmGrid::solve_poisson_filtered, pm_grid.rs:132-271 -> five LDS Stockham passes
@@ -310,76 +311,138 @@ __global__ __launch_bounds__(256) void k_tile_mark(const int32_t* __restrict__ t
 // A workgroup takes the contiguous Morton run of one 8^3-cell tile's particles (a tile of more
 // than kInterpChunk particles is cut into chunks of that many, one workgroup each) and adds every
 // CIC contribution into LDS as 64-bit fixed point (charges pre-scaled by 2^k, |q| 2^k <= 2^22,
-// each contribution rounded to an int32 once): the node-grid nodes 0..8 per axis its cells touch
-// and the interlaced nodes at the centres of cells -1..8.  Integer addition is associative, so the
-// sums do not depend on the order in which the LDS atomics land.  Nodes only this tile reaches
-// (node grid 1..7, interlaced 1..6 per axis) are final and stored as they are; the tile's other
-// nodes -- its own nodes on the faces, which neighbours reach too, and the nodes beyond its faces,
-// which belong to neighbours -- go to a per-tile slot list, and k_dep_faces adds, for every own
-// face node, the neighbours' slots in a fixed order.  A chunked tile's chunks write their whole LDS
-// grid as int64 and k_dep_merge adds them exactly, in chunk order, before the same split.  The
-// result is bit-reproducible and needs no cell table, no zeroing pass and no float atomics.
+// each contribution rounded to an int32 once): the node-grid nodes o .. o + 8 per axis its cells
+// touch and the interlaced nodes o .. o + 9 (LDS coordinate L is interlaced node o + L, the centre
+// of cell o + L - 1).  Integer addition is associative, so the sums do not depend on the order in
+// which the LDS atomics land.  Each tile owns the sector-aligned block of nodes o .. o + 7 per axis
+// of each grid; its LDS nodes beyond that block (coordinate 8 on some axis, interlaced 8 or 9) are
+// own nodes of its upper neighbours.  k_dep_tile converts all its sums to float into the tile's
+// record (own blocks, then the beyond nodes); k_dep_assemble writes every tile's own blocks as whole
+// rows, adding to each face node (coordinate 0 on some axis, interlaced 0 or 1) the beyond values
+// of the lower neighbours that reach it, in a fixed order.  So every 32-byte sector of the grids is
+// written once and whole.  A chunked tile's chunks write their whole LDS grid as int64 and
+// k_dep_merge adds them exactly, in chunk order, into the tile's record.  The result is
+// bit-reproducible and needs no cell table, no zeroing pass and no float atomics.
 // Each thread takes a contiguous share of the run; the particles of one (cell, half-cell octant) all
 // reach the same 8 + 8 nodes, so a thread sums consecutive ones in registers and adds one LDS atomic
 // per node and run (dep_range).
 constexpr int kDepPer = kInterpChunk / 256;        // at most 16 particles per thread
 static_assert((int64_t)kDepPer << kFixBits <= (int64_t)1 << 30, "a thread's run sum fits an int32");
-constexpr int kD1 = kTile + 1;                     // node-grid nodes 0..8 per axis
-constexpr int kD2 = kTile + 2;                     // centres of cells -1..8 per axis (index m + 1)
+constexpr int kD1 = kTile + 1;                     // node-grid LDS coordinates 0..8 per axis
+constexpr int kD2 = kTile + 2;                     // interlaced LDS coordinates 0..9 per axis
 constexpr int kDN1 = kD1 * kD1 * kD1;              // 729
 constexpr int kDN2 = kD2 * kD2 * kD2;              // 1000
-constexpr int kDepB1 = kTile * kTile * kTile - (kTile - 1) * (kTile - 1) * (kTile - 1);  // 169 own face nodes
-constexpr int kDepH1 = kDN1 - kTile * kTile * kTile;                                      // 217 beyond the faces
-constexpr int kDepB2 = kTile * kTile * kTile - (kTile - 2) * (kTile - 2) * (kTile - 2);  // 296
-constexpr int kDepH2 = kDN2 - kTile * kTile * kTile;                                      // 488
-constexpr int kDepSlots1 = kDepB1 + kDepH1;
-constexpr int kDepSlots = kDepSlots1 + kDepB2 + kDepH2;  // 1170 per tile (node grid only: kDepSlots1)
+constexpr int kDepOwn = kTile * kTile * kTile;     // 512 own nodes per grid
+constexpr int kDepH1 = kDN1 - kDepOwn;             // 217 beyond nodes, node grid
+constexpr int kDepH2 = kDN2 - kDepOwn;             // 488, interlaced grid
+static_assert(kTile == 8, "own blocks are indexed by the bits of a 9-bit index");
 
-// Slots of a tile's nodes that other tiles also reach, in three slabs per grid so that both
-// directions are arithmetic.  Node grid (LDS coordinates 0..8, interior 1..7 per axis): z in {0, 8}
-// (2 x 81), then y in {0, 8} for z in 1..7 (2 x 63), then x in {0, 8} for y, z in 1..7 (2 x 49).
-// Interlaced grid (LDS coordinates 0..9 = centres of cells -1..8, interior 2..7): z in {0, 1, 8, 9}
-// (4 x 100), then y in {0, 1, 8, 9} for z in 2..7 (4 x 60), then x in {0, 1, 8, 9} for y, z in 2..7
-// (4 x 36), after the node grid's kDepSlots1.  -1: a node only this tile reaches.
-__host__ __device__ inline int dep_slot1(int x, int y, int z) {
-    if (z == 0 || z == kD1 - 1) return (z ? 81 : 0) + y * kD1 + x;
-    if (y == 0 || y == kD1 - 1) return 162 + (y ? 63 : 0) + (z - 1) * kD1 + x;
-    if (x == 0 || x == kD1 - 1) return 288 + (x ? 49 : 0) + (z - 1) * 7 + (y - 1);
-    return -1;
+// A tile's record (floats): own node-grid block (x fastest), own interlaced block, node-grid beyond
+// nodes, interlaced beyond nodes; without interlacing the node grid's two parts only.
+template <bool BOTH> constexpr int dep_rec() { return BOTH ? kDN1 + kDN2 : kDN1; }
+template <bool BOTH> constexpr int dep_beyond1() { return BOTH ? 2 * kDepOwn : kDepOwn; }
+template <bool BOTH> constexpr int dep_beyond2() { return dep_beyond1<BOTH>() + kDepH1; }
+
+// Beyond nodes in slabs, so that both directions are arithmetic.  Node grid (coordinates 0..8, at
+// least one 8): z = 8 (81, x fastest), then y = 8 for z < 8 (72), then x = 8 for y, z < 8 (64).
+// Interlaced (0..9, at least one 8 or 9): z in {8, 9} (2 x 100), then y in {8, 9} for z < 8
+// (2 x 80), then x in {8, 9} for y, z < 8 (2 x 64).
+constexpr __host__ __device__ inline int dep_beyond_slot1(int x, int y, int z) {
+    return z == kTile ? y * kD1 + x : y == kTile ? 81 + z * kD1 + x : 153 + z * kTile + y;
 }
-__host__ __device__ inline void dep_node1(int sl, int& x, int& y, int& z) {
-    if (sl < 162) {
-        z = sl < 81 ? 0 : kD1 - 1; y = (sl % 81) / kD1; x = sl % kD1;
-    } else if (sl < 288) {
-        const int q = sl - 162;
-        y = q < 63 ? 0 : kD1 - 1; z = 1 + (q % 63) / kD1; x = q % kD1;
+constexpr __host__ __device__ inline void dep_beyond_node1(int s, int& x, int& y, int& z) {
+    if (s < 81) {
+        z = kTile; y = s / kD1; x = s % kD1;
+    } else if (s < 153) {
+        y = kTile; z = (s - 81) / kD1; x = (s - 81) % kD1;
     } else {
-        const int q = sl - 288;
-        x = q < 49 ? 0 : kD1 - 1; z = 1 + (q % 49) / 7; y = 1 + q % 7;
+        x = kTile; z = (s - 153) / kTile; y = (s - 153) % kTile;
     }
 }
-__host__ __device__ inline int dep_q2(int c) { return c < 2 ? c : c - (kD2 - 4); }  // 0, 1, 8, 9 -> 0..3
-__host__ __device__ inline int dep_c2(int q) { return q < 2 ? q : q + (kD2 - 4); }  // 0..3 -> 0, 1, 8, 9
-__host__ __device__ inline bool dep_edge2(int c) { return c < 2 || c >= kD2 - 2; }
-__host__ __device__ inline int dep_slot2(int x, int y, int z) {
-    if (dep_edge2(z)) return kDepSlots1 + dep_q2(z) * 100 + y * kD2 + x;
-    if (dep_edge2(y)) return kDepSlots1 + 400 + dep_q2(y) * 60 + (z - 2) * kD2 + x;
-    if (dep_edge2(x)) return kDepSlots1 + 640 + dep_q2(x) * 36 + (z - 2) * 6 + (y - 2);
-    return -1;
+constexpr __host__ __device__ inline int dep_beyond_slot2(int x, int y, int z) {
+    return z >= kTile   ? (z - kTile) * 100 + y * kD2 + x
+           : y >= kTile ? 200 + (y - kTile) * 80 + z * kD2 + x
+                        : 360 + (x - kTile) * 64 + z * kTile + y;
 }
-__host__ __device__ inline void dep_node2(int sl, int& x, int& y, int& z) {
-    const int q = sl - kDepSlots1;
-    if (q < 400) {
-        z = dep_c2(q / 100); y = (q % 100) / kD2; x = q % kD2;
-    } else if (q < 640) {
-        const int r = q - 400;
-        y = dep_c2(r / 60); z = 2 + (r % 60) / kD2; x = r % kD2;
+constexpr __host__ __device__ inline void dep_beyond_node2(int s, int& x, int& y, int& z) {
+    if (s < 200) {
+        z = kTile + s / 100; y = s % 100 / kD2; x = s % kD2;
+    } else if (s < 360) {
+        const int q = s - 200;
+        y = kTile + q / 80; z = q % 80 / kD2; x = q % kD2;
     } else {
-        const int r = q - 640;
-        x = dep_c2(r / 36); z = 2 + (r % 36) / 6; y = 2 + r % 6;
+        const int q = s - 360;
+        x = kTile + q / 64; z = q % 64 / kTile; y = q % kTile;
     }
 }
-static_assert(162 + 126 + 98 == kDepSlots1 && 400 + 240 + 144 == kDepB2 + kDepH2, "slab sizes");
+static_assert(81 + 72 + 64 == kDepH1 && 200 + 160 + 128 == kDepH2, "beyond slab sizes");
+
+// LDS index of each entry of the interlaced tile record (the node-grid-only record is its entries
+// 0..511 and 1024..1240).
+struct DepRecMap {
+    uint16_t lds[kDN1 + kDN2];
+};
+constexpr DepRecMap make_dep_rec_map() {
+    DepRecMap m{};
+    for (int j = 0; j < kDN1 + kDN2; ++j) {
+        int x = 0, y = 0, z = 0, g2 = 0;
+        if (j < 2 * kDepOwn) {
+            const int k = j & (kDepOwn - 1);
+            x = k & 7; y = (k >> 3) & 7; z = k >> 6; g2 = j >= kDepOwn;
+        } else if (j < 2 * kDepOwn + kDepH1) {
+            dep_beyond_node1(j - 2 * kDepOwn, x, y, z);
+        } else {
+            dep_beyond_node2(j - 2 * kDepOwn - kDepH1, x, y, z);
+            g2 = 1;
+        }
+        m.lds[j] = (uint16_t)(g2 ? kDN1 + (z * kD2 + y) * kD2 + x : (z * kD1 + y) * kD1 + x);
+    }
+    return m;
+}
+__constant__ DepRecMap kDepRecMap = make_dep_rec_map();
+constexpr int kDepRecIter = (kDN1 + kDN2 + 255) / 256;  // record entries per thread of a 256-thread block
+
+// Compile-time checks: the record map is a bijection onto the LDS nodes, the beyond numbering inverts
+// in both grids, and in k_dep_assemble's sums every beyond node of a tile is read exactly once, by the
+// upper neighbour that owns the node (the one whose own coordinate is the node's minus 8 on the axes
+// where it is 8 or more).
+constexpr bool dep_maps_ok() {
+    const DepRecMap m = make_dep_rec_map();
+    bool hit[kDN1 + kDN2] = {};
+    for (int j = 0; j < kDN1 + kDN2; ++j) {
+        if (m.lds[j] >= kDN1 + kDN2 || hit[m.lds[j]]) return false;
+        hit[m.lds[j]] = true;
+    }
+    for (int s = 0; s < kDepH1; ++s) {
+        int x = 0, y = 0, z = 0;
+        dep_beyond_node1(s, x, y, z);
+        if (dep_beyond_slot1(x, y, z) != s || x > kTile || y > kTile || z > kTile) return false;
+        if (x < kTile && y < kTile && z < kTile) return false;
+    }
+    for (int s = 0; s < kDepH2; ++s) {
+        int x = 0, y = 0, z = 0;
+        dep_beyond_node2(s, x, y, z);
+        if (dep_beyond_slot2(x, y, z) != s || x > kTile + 1 || y > kTile + 1 || z > kTile + 1) return false;
+        if (x < kTile && y < kTile && z < kTile) return false;
+    }
+    int r1[kDepH1] = {}, r2[kDepH2] = {};
+    for (int g = 0; g < 2; ++g)
+        for (int k = 0; k < kDepOwn; ++k) {
+            const int x = k & 7, y = (k >> 3) & 7, z = k >> 6, lim = g ? 2 : 1;
+            for (int c = 1; c < 8; ++c) {
+                if (((c & 1) && x >= lim) || ((c & 2) && y >= lim) || ((c & 4) && z >= lim)) continue;
+                const int ax = x + ((c & 1) ? kTile : 0), ay = y + ((c & 2) ? kTile : 0), az = z + ((c & 4) ? kTile : 0);
+                if (g) ++r2[dep_beyond_slot2(ax, ay, az)];
+                else ++r1[dep_beyond_slot1(ax, ay, az)];
+            }
+        }
+    for (int s = 0; s < kDepH1; ++s)
+        if (r1[s] != 1) return false;
+    for (int s = 0; s < kDepH2; ++s)
+        if (r2[s] != 1) return false;
+    return true;
+}
+static_assert(dep_maps_ok(), "deposit record / beyond-node maps");
 
 // tile t's origin cell, from its Morton index
 __device__ inline void tile_origin(uint32_t t, int32_t o[3]) {
@@ -391,14 +454,22 @@ __device__ inline void tile_origin(uint32_t t, int32_t o[3]) {
 
 __device__ inline float dep_value(unsigned long long v, double unit) { return (float)((double)(long long)v * unit); }
 
-// The CIC contributions of one particle as int32 fixed point, v[0..7] for the node grid (corner
-// (dx, dy, dz) = (c & 1, c >> 1 & 1, c >> 2) from its cell's lower corner) and v[8..15] for the
-// interlaced grid (same corner order from the centre of cell (c - 1 + h) per axis, h = 1 if the particle
-// is in the upper half of its cell); key = its (cell, half-cell octant), whose particles all reach the
-// same 8 + 8 nodes; n1, n2 = the LDS index of the lower corner node in each grid.
+// Rounding by the last product's FMA: for |x| <= 2^22, x + 1.5 2^23 lies in [2^23, 2^24], where floats are
+// the integers, so fma(a, b, kRoundMagic) is the nearest integer to a b (ties to even) and its bit
+// pattern minus kRoundMagicBits is that integer.
+constexpr float kRoundMagic = 12582912.0f;
+constexpr uint32_t kRoundMagicBits = 0x4B400000u;
+static_assert(kFixBits <= 22, "contributions must stay within the exact range of the rounding FMA");
+
+// The CIC contributions of one particle as int32 fixed point plus kRoundMagicBits (the bits of the
+// rounding FMA), v[0..7] for the node grid (corner (dx, dy, dz) = (c & 1, c >> 1 & 1, c >> 2) from its
+// cell's lower corner) and v[8..15] for the interlaced grid (same corner order from the centre of cell
+// (c - 1 + h) per axis, h = 1 if the particle is in the upper half of its cell); key = its (cell,
+// half-cell octant), whose particles all reach the same 8 + 8 nodes; n1, n2 = the LDS index of the
+// lower corner node in each grid.
 template <bool BOTH>
-__device__ inline void dep_contrib(const uint4 r, const int32_t o[3], int sh, int logG, float qp, float qm, int v[16],
-                                   int& key, int& n1, int& n2) {
+__device__ inline void dep_contrib(const uint4 r, const int32_t o[3], int sh, int logG, float qp, float qm,
+                                   uint32_t v[16], int& key, int& n1, int& n2) {
     const int cx = (int)(((r.x >> sh) - (uint32_t)o[0]) & (kTile - 1));
     const int cy = (int)(((r.y >> sh) - (uint32_t)o[1]) & (kTile - 1));
     const int cz = (int)(((r.z >> sh) - (uint32_t)o[2]) & (kTile - 1));
@@ -408,14 +479,16 @@ __device__ inline void dep_contrib(const uint4 r, const int32_t o[3], int sh, in
     {
         const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {q * (1.0f - fz), q * fz};
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = __float2int_rn(wz[c >> 2] * wy[(c >> 1) & 1] * wx[c & 1]);
+        for (int c = 0; c < 8; ++c)
+            v[c] = __float_as_uint(__builtin_fmaf(wz[c >> 2] * wy[(c >> 1) & 1], wx[c & 1], kRoundMagic));
     }
     if (BOTH) {
         const float vx[2] = {hx ? 1.5f - fx : 0.5f - fx, hx ? fx - 0.5f : 0.5f + fx};
         const float vy[2] = {hy ? 1.5f - fy : 0.5f - fy, hy ? fy - 0.5f : 0.5f + fy};
         const float vz[2] = {q * (hz ? 1.5f - fz : 0.5f - fz), q * (hz ? fz - 0.5f : 0.5f + fz)};
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[8 + c] = __float2int_rn(vz[c >> 2] * vy[(c >> 1) & 1] * vx[c & 1]);
+        for (int c = 0; c < 8; ++c)
+            v[8 + c] = __float_as_uint(__builtin_fmaf(vz[c >> 2] * vy[(c >> 1) & 1], vx[c & 1], kRoundMagic));
     }
     key = ((cz * kTile + cy) * kTile + cx) << 3 | (hz ? 4 : 0) | (hy ? 2 : 0) | (hx ? 1 : 0);
     n1 = (cz * kD1 + cy) * kD1 + cx;
@@ -423,19 +496,21 @@ __device__ inline void dep_contrib(const uint4 r, const int32_t o[3], int sh, in
     n2 = kDN1 + ((cz + (hz ? 1 : 0)) * kD2 + cy + (hy ? 1 : 0)) * kD2 + cx + (hx ? 1 : 0);
 }
 
-// Adds one run's summed contributions to the tile's LDS sums.
+// Adds one run's summed contributions to the tile's LDS sums: acc holds the sums of the runs' FMA bit
+// patterns, bias their count times kRoundMagicBits (both modulo 2^32; the difference, at most
+// kDepPer 2^22 in magnitude, is exact as an int32).
 template <bool BOTH>
-__device__ inline void dep_flush(unsigned long long* a, const int acc[16], int n1, int n2) {
+__device__ inline void dep_flush(unsigned long long* a, const uint32_t acc[16], uint32_t bias, int n1, int n2) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int d = ((c >> 2) * kD1 + ((c >> 1) & 1)) * kD1 + (c & 1);
-        atomicAdd(&a[n1 + d], (unsigned long long)(long long)acc[c]);
+        atomicAdd(&a[n1 + d], (unsigned long long)(long long)(int32_t)(acc[c] - bias));
     }
     if (BOTH) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int d = ((c >> 2) * kD2 + ((c >> 1) & 1)) * kD2 + (c & 1);
-            atomicAdd(&a[n2 + d], (unsigned long long)(long long)acc[8 + c]);
+            atomicAdd(&a[n2 + d], (unsigned long long)(long long)(int32_t)(acc[8 + c] - bias));
         }
     }
 }
@@ -444,69 +519,59 @@ __device__ inline void dep_flush(unsigned long long* a, const int acc[16], int n
 // consecutive particles of one (cell, octant) run are summed in registers (int32: at most
 // kDepPer x 2^22) and added to LDS once per run, so a halo core costs one LDS atomic per node and
 // kDepPer particles, and the lanes of a wavefront, kDepPer particles apart in Morton order, seldom
-// meet at one LDS address.  Loads go four at a time.
+// meet at one LDS address.  Loads go four at a time; r holds the first four (loaded by the caller).
 template <bool BOTH>
-__device__ inline void dep_range(const uint4* __restrict__ parts, int32_t p, int32_t pe, const int32_t o[3], int sh,
-                                 int logG, float qp, float qm, unsigned long long* a) {
-    int acc[16];
+__device__ inline void dep_range(const uint4* __restrict__ parts, int32_t p, int32_t pe, uint4 (&r)[4],
+                                 const int32_t o[3], int sh, int logG, float qp, float qm, unsigned long long* a) {
+    uint32_t acc[16], bias = 0;
     int cur = -1, c1 = 0, c2 = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc[k] = 0;
     for (int32_t q0 = p; q0 < pe; q0 += 4) {
-        uint4 r[4];
+        if (q0 > p) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) r[u] = parts[q0 + u < pe ? q0 + u : p];
+            for (int u = 0; u < 4; ++u) r[u] = parts[q0 + u < pe ? q0 + u : p];
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (q0 + u >= pe) break;
-            int v[16], key, n1, n2;
+            uint32_t v[16];
+            int key, n1, n2;
             dep_contrib<BOTH>(r[u], o, sh, logG, qp, qm, v, key, n1, n2);
             if (key != cur) {
-                if (cur >= 0) dep_flush<BOTH>(a, acc, c1, c2);
+                if (cur >= 0) dep_flush<BOTH>(a, acc, bias, c1, c2);
                 cur = key;
                 c1 = n1;
                 c2 = n2;
+                bias = kRoundMagicBits;
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc[k] = v[k];
             } else {
+                bias += kRoundMagicBits;
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc[k] += v[k];
             }
         }
     }
-    if (cur >= 0) dep_flush<BOTH>(a, acc, c1, c2);
+    if (cur >= 0) dep_flush<BOTH>(a, acc, bias, c1, c2);
 }
 
-// Stores a tile's sums: the nodes only it reaches straight into the grids (node grid 1..7, interlaced
-// centres 1..6 per axis), the others to its slots.
+// Stores a tile's sums, converted to float, as its record (contiguous); rm = this thread's entries'
+// LDS indices.
 template <bool BOTH>
-__device__ inline void dep_store_tile(const unsigned long long* a, const int32_t o[3],
-                                      int logG, double unit, float* __restrict__ rho, float* __restrict__ rho2,
+__device__ inline void dep_store_tile(const unsigned long long* a, const uint32_t (&rm)[kDepRecIter], double unit,
                                       float* __restrict__ tb) {
-    const int tid = threadIdx.x;
-    const int64_t G = 1ll << logG;
-    const int32_t msk = (int32_t)G - 1;
-    constexpr int I1 = kTile - 1, I2 = kTile - 2;
-    for (int k = tid; k < I1 * I1 * I1; k += 256) {
-        const int x = 1 + k % I1, y = 1 + (k / I1) % I1, z = 1 + k / (I1 * I1);
-        rho[((int64_t)(o[2] + z) * G + (o[1] + y)) * G + (o[0] + x)] = dep_value(a[(z * kD1 + y) * kD1 + x], unit);
+#pragma unroll
+    for (int i = 0; i < kDepRecIter; ++i) {
+        const int j = (int)threadIdx.x + 256 * i;
+        if (j < dep_rec<BOTH>()) tb[j] = dep_value(a[rm[i]], unit);
     }
-    if (BOTH)
-        for (int k = tid; k < I2 * I2 * I2; k += 256) {
-            const int mx = 1 + k % I2, my = 1 + (k / I2) % I2, mz = 1 + k / (I2 * I2);  // centre of cell m
-            const float val = dep_value(a[kDN1 + ((mz + 1) * kD2 + my + 1) * kD2 + mx + 1], unit);
-            rho2[((int64_t)((o[2] + mz + 1) & msk) * G + ((o[1] + my + 1) & msk)) * G + ((o[0] + mx + 1) & msk)] = val;
-        }
-    for (int sl = tid; sl < (BOTH ? kDepSlots : kDepSlots1); sl += 256) {
-        int x, y, z;
-        if (sl < kDepSlots1) {
-            dep_node1(sl, x, y, z);
-            tb[sl] = dep_value(a[(z * kD1 + y) * kD1 + x], unit);
-        } else {
-            dep_node2(sl, x, y, z);
-            tb[sl] = dep_value(a[kDN1 + (z * kD2 + y) * kD2 + x], unit);
-        }
-    }
+}
+
+// LDS index of entry j of a tile record
+template <bool BOTH>
+__device__ inline uint32_t dep_rec_lds(int j) {
+    return kDepRecMap.lds[(BOTH || j < kDepOwn) ? j : j + kDepOwn];
 }
 
 // Work items: block b < ntile takes tile (b & 7) ntile / 8 + b / 8 (contiguous Morton ranges per
@@ -515,11 +580,10 @@ template <bool BOTH>
 __global__ __launch_bounds__(256) void k_dep_tile(const uint4* __restrict__ parts, const int32_t* __restrict__ ts,
                                                   const int2* __restrict__ chunks, const int32_t* __restrict__ nchunk,
                                                   int32_t ntile, int logG, float qp, float qm, double unit,
-                                                  float* __restrict__ rho, float* __restrict__ rho2,
                                                   float* __restrict__ tbuf, long long* __restrict__ cbuf) {
     constexpr int NN = kDN1 + (BOTH ? kDN2 : 0);
-    constexpr int NS = BOTH ? kDepSlots : kDepSlots1;
-    __shared__ unsigned long long a[NN];
+    constexpr int NR = dep_rec<BOTH>();
+    __shared__ __align__(16) unsigned long long a[NN];
     const uint32_t b = blockIdx.x;
     int32_t t, pbeg, pend, e = -1;
     if (b < (uint32_t)ntile) {
@@ -536,15 +600,29 @@ __global__ __launch_bounds__(256) void k_dep_tile(const uint4* __restrict__ part
         pend = min(ts[t + 1], pbeg + kInterpChunk);
     }
     const int tid = threadIdx.x;
-    for (int i = tid; i < NN; i += 256) a[i] = 0ull;
+    // this thread's share of the run; its first four particles are loaded before the LDS is cleared
+    // and its record entries' LDS indices before the second barrier, so their latency overlaps the
+    // clearing and the barriers
+    const int32_t per = (pend - pbeg + 255) >> 8;  // <= kInterpChunk / 256
+    const int32_t p = pbeg + tid * per, pe = min(pend, p + per);
+    uint4 r[4];
+    if (p < pe) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = parts[p + u < pe ? p + u : p];
+    }
+    uint4* a4 = reinterpret_cast<uint4*>(a);
+    for (int i = tid; i < NN / 2; i += 256) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+    if ((NN & 1) && tid == 0) a[NN - 1] = 0ull;
     int32_t o[3];
     tile_origin((uint32_t)t, o);
     const int sh = 32 - logG;
     __syncthreads();
-    {
-        const int32_t per = (pend - pbeg + 255) >> 8;  // <= kInterpChunk / 256
-        const int32_t p = pbeg + tid * per;
-        dep_range<BOTH>(parts, p, min(pend, p + per), o, sh, logG, qp, qm, a);
+    dep_range<BOTH>(parts, p, pe, r, o, sh, logG, qp, qm, a);
+    uint32_t rm[kDepRecIter];
+#pragma unroll
+    for (int i = 0; i < kDepRecIter; ++i) {
+        const int j = tid + 256 * i;
+        rm[i] = j < NR ? dep_rec_lds<BOTH>(j) : 0u;
     }
     __syncthreads();
     if (e >= 0) {  // a chunk: its whole grid, exact
@@ -552,18 +630,17 @@ __global__ __launch_bounds__(256) void k_dep_tile(const uint4* __restrict__ part
         for (int i = tid; i < NN; i += 256) out[i] = (long long)a[i];
         return;
     }
-    dep_store_tile<BOTH>(a, o, logG, unit, rho, rho2, tbuf + (int64_t)t * NS);
+    dep_store_tile<BOTH>(a, rm, unit, tbuf + (int64_t)t * NR);
 }
 
-// The chunks of a chunked tile summed in chunk order (exact), then stored like a whole tile: the
-// tile's nc chunk blocks split its nodes into nc slices, block k summing slice k over all chunks.
+// The chunks of a chunked tile summed in chunk order (exact) into the tile's record: the tile's nc
+// chunk blocks split the record into nc slices, block k summing slice k over all chunks.
 template <bool BOTH>
 __global__ __launch_bounds__(256) void k_dep_merge(const int32_t* __restrict__ ts, const int2* __restrict__ chunks,
-                                                   const int32_t* __restrict__ nchunk, int logG, double unit,
-                                                   const long long* __restrict__ cbuf, float* __restrict__ rho,
-                                                   float* __restrict__ rho2, float* __restrict__ tbuf) {
+                                                   const int32_t* __restrict__ nchunk, double unit,
+                                                   const long long* __restrict__ cbuf, float* __restrict__ tbuf) {
     constexpr int NN = kDN1 + (BOTH ? kDN2 : 0);
-    constexpr int NS = BOTH ? kDepSlots : kDepSlots1;
+    constexpr int NR = dep_rec<BOTH>();
     const int32_t e = (int32_t)blockIdx.x;
     if (e >= *nchunk) return;
     const int2 w = chunks[e];
@@ -571,115 +648,65 @@ __global__ __launch_bounds__(256) void k_dep_merge(const int32_t* __restrict__ t
     const int32_t k = (w.y - beg) / kInterpChunk;  // this chunk's index in its tile
     const int32_t nc = (ts[t + 1] - beg + kInterpChunk - 1) / kInterpChunk;
     const int32_t first = e - k;  // the tile's chunks are consecutive entries
-    const int32_t per = (NN + nc - 1) / nc;
-    const int32_t n0 = k * per, n1 = min(NN, n0 + per);
-    int32_t o[3];
-    tile_origin((uint32_t)t, o);
-    const int64_t G = 1ll << logG;
-    const int32_t msk = (int32_t)G - 1;
-    float* tb = tbuf + (int64_t)t * NS;
-    for (int32_t n = n0 + (int32_t)threadIdx.x; n < n1; n += 256) {
+    const int32_t per = (NR + nc - 1) / nc;
+    const int32_t j0 = k * per, j1 = min(NR, j0 + per);
+    float* tb = tbuf + (int64_t)t * NR;
+    for (int32_t j = j0 + (int32_t)threadIdx.x; j < j1; j += 256) {
+        const uint32_t n = dep_rec_lds<BOTH>(j);
         long long acc = 0;
-        for (int32_t j = 0; j < nc; ++j) acc += cbuf[(int64_t)(first + j) * NN + n];
-        const float val = (float)((double)acc * unit);
-        if (n < kDN1) {
-            const int x = n % kD1, y = (n / kD1) % kD1, z = n / (kD1 * kD1);
-            const int sl = dep_slot1(x, y, z);
-            if (sl >= 0) tb[sl] = val;
-            else rho[((int64_t)(o[2] + z) * G + (o[1] + y)) * G + (o[0] + x)] = val;
-        } else {
-            const int jn = n - kDN1;  // interlaced node at the centre of cell m, index m + 1, stored at m + 1
-            const int x = jn % kD2, y = (jn / kD2) % kD2, z = jn / (kD2 * kD2);
-            const int sl = dep_slot2(x, y, z);
-            if (sl >= 0) tb[sl] = val;
-            else rho2[((int64_t)((o[2] + z) & msk) * G + ((o[1] + y) & msk)) * G + ((o[0] + x) & msk)] = val;
-        }
+        for (int32_t q = 0; q < nc; ++q) acc += cbuf[(int64_t)(first + q) * NN + n];
+        tb[j] = (float)((double)acc * unit);
     }
 }
 
-// Own face nodes of one category: M* per axis 0 (interior coordinate), 1 (coordinate 0, which the
-// neighbour below also reaches: its local 8) or 2 (interlaced only: centre of cell 7, which the
-// neighbour above also reaches: its centre -1).  The value is the tile's own slot plus, for every
-// non-empty set of the category's face axes in a fixed order, the slot of the neighbour across them;
-// everything but the node coordinates is fixed at compile time.
-template <bool G2, int NS, int MX, int MY, int MZ>
-__device__ inline void dep_face_cat(const float* __restrict__ tbuf, uint32_t t, const uint32_t* nbr,
-                                    const int32_t o[3], int logG, float* __restrict__ grid) {
-    constexpr int NF = G2 ? kTile - 2 : kTile - 1;  // interior coordinates per axis: 1 .. NF
-    constexpr int M[3] = {MX, MY, MZ};
-    constexpr int nfree = (MX == 0) + (MY == 0) + (MZ == 0);
-    constexpr int count = nfree == 2 ? NF * NF : nfree == 1 ? NF : 1;
-    static_assert(nfree < 3, "a face category has at least one face axis");
-    const int64_t G = 1ll << logG;
-    const int32_t msk = (int32_t)G - 1;
-    for (int k = threadIdx.x; k < count; k += 256) {
-        int r = k, l[3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            if (M[ax] == 0) {
-                l[ax] = 1 + r % NF;
-                r /= NF;
-            } else {
-                l[ax] = M[ax] == 1 ? 0 : kTile - 1;
-            }
-        }
-        const int own = G2 ? dep_slot2(l[0] + 1, l[1] + 1, l[2] + 1) : dep_slot1(l[0], l[1], l[2]);
-        float v[8];
-        v[0] = tbuf[(int64_t)t * NS + own];
-#pragma unroll
-        for (int c = 1; c < 8; ++c) {
-            v[c] = 0.0f;
-            if (((c & 1) && MX == 0) || ((c & 2) && MY == 0) || ((c & 4) && MZ == 0)) continue;
-            const int dx = (c & 1) ? (MX == 1 ? -1 : 1) : 0;
-            const int dy = (c & 2) ? (MY == 1 ? -1 : 1) : 0;
-            const int dz = (c & 4) ? (MZ == 1 ? -1 : 1) : 0;
-            const uint32_t nt = nbr[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
-            const int ax = l[0] - kTile * dx, ay = l[1] - kTile * dy, az = l[2] - kTile * dz;
-            const int sl = G2 ? dep_slot2(ax + 1, ay + 1, az + 1) : dep_slot1(ax, ay, az);
-            v[c] = tbuf[(int64_t)nt * NS + sl];
-        }
-        const float sum = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-        const int64_t at = G2 ? ((int64_t)((o[2] + l[2] + 1) & msk) * G + ((o[1] + l[1] + 1) & msk)) * G +
-                                    ((o[0] + l[0] + 1) & msk)
-                              : ((int64_t)(o[2] + l[2]) * G + (o[1] + l[1])) * G + (o[0] + l[0]);
-        grid[at] = sum;
-    }
-}
-
-// tbuf holds NS slots per tile (both grids' slots when interlacing, else the node grid's)
-template <bool G2, int NS, int C>
-__device__ inline void dep_face_cats(const float* __restrict__ tbuf, uint32_t t, const uint32_t* nbr,
-                                     const int32_t o[3], int logG, float* __restrict__ grid) {
-    // categories C = MX + 3 MY + 9 MZ (base 3; the node grid uses digits 0 and 1 only)
-    if constexpr (C < 27) {
-        constexpr int MX = C % 3, MY = (C / 3) % 3, MZ = C / 9;
-        if constexpr (C != 0 && (G2 || (MX < 2 && MY < 2 && MZ < 2)))
-            dep_face_cat<G2, NS, MX, MY, MZ>(tbuf, t, nbr, o, logG, grid);
-        dep_face_cats<G2, NS, C + 1>(tbuf, t, nbr, o, logG, grid);
-    }
-}
-
-// The own face nodes of every tile (k_dep_faces): node grid 169, interlaced 296 per tile.
+// Every tile's own blocks (nodes o .. o + 7 per axis of each grid), written as whole rows: own node
+// (x, y, z) is the tile's own value plus, for every non-empty set c of its face axes (coordinate 0;
+// interlaced 0 or 1), the beyond value of the lower neighbour across those axes (at coordinates
+// + 8 there), summed in a fixed order (the eight terms pairwise, absent terms 0).
 template <bool BOTH>
-__global__ __launch_bounds__(256) void k_dep_faces(const float* __restrict__ tbuf, int logG, float* __restrict__ rho,
-                                                   float* __restrict__ rho2) {
-    __shared__ uint32_t nbr[27];  // tile index at offset (dx, dy, dz), index (dz + 1) 9 + (dy + 1) 3 + dx + 1
+__global__ __launch_bounds__(256) void k_dep_assemble(const float* __restrict__ tbuf, int logG,
+                                                      float* __restrict__ rho, float* __restrict__ rho2) {
+    constexpr int NR = dep_rec<BOTH>();
+    constexpr int NQ = BOTH ? 4 : 2;  // own nodes per thread: two per grid
+    __shared__ uint32_t nbr[8];      // tile at offset -(c & 1, c >> 1 & 1, c >> 2)
     const uint32_t ntile = gridDim.x;
     const uint32_t b = blockIdx.x;
     const uint32_t t = (b & 7u) * (ntile >> 3) + (b >> 3);
     const int32_t tm = (1 << (logG - 3)) - 1;  // tiles per axis - 1
+    const int tid = threadIdx.x;
     int32_t o[3];
     tile_origin(t, o);
-    if (threadIdx.x < 27) {
-        const int dx = (int)threadIdx.x % 3 - 1, dy = ((int)threadIdx.x / 3) % 3 - 1, dz = (int)threadIdx.x / 9 - 1;
-        nbr[threadIdx.x] = (uint32_t)(spread3_21((uint32_t)(((o[0] >> 3) + dx) & tm)) |
-                                      (spread3_21((uint32_t)(((o[1] >> 3) + dy) & tm)) << 1) |
-                                      (spread3_21((uint32_t)(((o[2] >> 3) + dz) & tm)) << 2));
-    }
+    float v[NQ][8];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q][0] = tbuf[(int64_t)t * NR + tid + 256 * q];  // own blocks: entries 0 .. NQ 256 - 1
+    if (tid < 8)
+        nbr[tid] = (uint32_t)(spread3_21((uint32_t)(((o[0] >> 3) - (tid & 1)) & tm)) |
+                              (spread3_21((uint32_t)(((o[1] >> 3) - ((tid >> 1) & 1)) & tm)) << 1) |
+                              (spread3_21((uint32_t)(((o[2] >> 3) - (tid >> 2)) & tm)) << 2));
     __syncthreads();
-    constexpr int NS = BOTH ? kDepSlots : kDepSlots1;
-    dep_face_cats<false, NS, 0>(tbuf, t, nbr, o, logG, rho);
-    if constexpr (BOTH) dep_face_cats<true, NS, 0>(tbuf, t, nbr, o, logG, rho2);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int g = q >> 1, k = tid + 256 * (q & 1);
+        const int x = k & 7, y = (k >> 3) & 7, z = k >> 6, lim = g ? 2 : 1;
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {
+            v[q][c] = 0.0f;
+            if (((c & 1) && x >= lim) || ((c & 2) && y >= lim) || ((c & 4) && z >= lim)) continue;
+            const int ax = x + ((c & 1) ? kTile : 0), ay = y + ((c & 2) ? kTile : 0), az = z + ((c & 4) ? kTile : 0);
+            const int sl = g ? dep_beyond2<BOTH>() + dep_beyond_slot2(ax, ay, az)
+                             : dep_beyond1<BOTH>() + dep_beyond_slot1(ax, ay, az);
+            v[q][c] = tbuf[(int64_t)nbr[c] * NR + sl];
+        }
+    }
+    const int64_t G = 1ll << logG;
+    const int64_t base = ((int64_t)o[2] * G + o[1]) * G + o[0];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int k = tid + 256 * (q & 1);
+        const int32_t off = ((k >> 6) << (2 * logG)) + (((k >> 3) & 7) << logG) + (k & 7);
+        const float sum = ((v[q][0] + v[q][1]) + (v[q][2] + v[q][3])) + ((v[q][4] + v[q][5]) + (v[q][6] + v[q][7]));
+        ((q >> 1) ? rho2 : rho)[base + off] = sum;
+    }
 }
 
 // Thread per grid node, nodes enumerated in Morton order so that the eight
@@ -1188,7 +1215,7 @@ hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int m
 // ceil(np / kInterpChunk) <= 2 n / kInterpChunk
 int64_t dep_chunk_cap(int64_t n) { return 2 * n / kInterpChunk + 2; }
 int64_t dep_chunk_nodes() { return kDN1 + kDN2; }
-int64_t dep_tile_slots() { return kDepSlots; }
+int64_t dep_tile_slots() { return dep_rec<true>(); }
 
 static hipError_t deposit_fp32(Ctx& c, int64_t n, hipStream_t s) {
     const int64_t G3 = c.G3();
@@ -1211,17 +1238,16 @@ static hipError_t deposit_fp32(Ctx& c, int64_t n, hipStream_t s) {
     float* r2 = (float*)c.rho2;
     if (c.p.interlace) {
         k_dep_tile<true><<<dim3((unsigned)ntile + nd), dim3(256), 0, s>>>(
-            c.parts, c.tile_start, c.dep_chunks, count + 2, ntile, c.logG, qp, qm, unit, r1, r2, c.dep_tbuf, c.dep_cbuf);
-        k_dep_merge<true><<<dim3(nd), dim3(256), 0, s>>>(c.tile_start, c.dep_chunks, count + 2, c.logG, unit,
-                                                         c.dep_cbuf, r1, r2, c.dep_tbuf);
-        k_dep_faces<true><<<dim3((unsigned)ntile), dim3(256), 0, s>>>(c.dep_tbuf, c.logG, r1, r2);
+            c.parts, c.tile_start, c.dep_chunks, count + 2, ntile, c.logG, qp, qm, unit, c.dep_tbuf, c.dep_cbuf);
+        k_dep_merge<true><<<dim3(nd), dim3(256), 0, s>>>(c.tile_start, c.dep_chunks, count + 2, unit, c.dep_cbuf,
+                                                         c.dep_tbuf);
+        k_dep_assemble<true><<<dim3((unsigned)ntile), dim3(256), 0, s>>>(c.dep_tbuf, c.logG, r1, r2);
     } else {
         k_dep_tile<false><<<dim3((unsigned)ntile + nd), dim3(256), 0, s>>>(
-            c.parts, c.tile_start, c.dep_chunks, count + 2, ntile, c.logG, qp, qm, unit, r1, nullptr, c.dep_tbuf,
-            c.dep_cbuf);
-        k_dep_merge<false><<<dim3(nd), dim3(256), 0, s>>>(c.tile_start, c.dep_chunks, count + 2, c.logG, unit,
-                                                          c.dep_cbuf, r1, nullptr, c.dep_tbuf);
-        k_dep_faces<false><<<dim3((unsigned)ntile), dim3(256), 0, s>>>(c.dep_tbuf, c.logG, r1, nullptr);
+            c.parts, c.tile_start, c.dep_chunks, count + 2, ntile, c.logG, qp, qm, unit, c.dep_tbuf, c.dep_cbuf);
+        k_dep_merge<false><<<dim3(nd), dim3(256), 0, s>>>(c.tile_start, c.dep_chunks, count + 2, unit, c.dep_cbuf,
+                                                          c.dep_tbuf);
+        k_dep_assemble<false><<<dim3((unsigned)ntile), dim3(256), 0, s>>>(c.dep_tbuf, c.logG, r1, nullptr);
     }
     return hipGetLastError();
 }
