@@ -96,7 +96,7 @@ typedef struct jtp_stats {
 } jtp_stats;
 
 /* Stages readable with jtp_stage_copy after a jtp_compute_forces (parity tests). */
-#define JTP_STAGE_KEYS      1 /* uint64[n] sorted Morton keys                                   */
+#define JTP_STAGE_KEYS      1 /* uint64[n] sorted Morton keys, low 16 bits cleared              */
 #define JTP_STAGE_PERM      2 /* int32[n]  sorted slot -> original particle index               */
 #define JTP_STAGE_NODES     3 /* int32[n-1][4] internal nodes: first, last, left, right
                                  (child >= 0: internal node; child < 0: leaf ~sorted_index)     */

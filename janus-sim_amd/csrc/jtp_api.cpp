@@ -428,7 +428,7 @@ static int enqueue_prep(Ctx& c, int64_t n, const uint32_t* d_pos, const int8_t* 
     // the packed records live in the node-sum buffer, which the tree build (after the gather) overwrites
     uint4* rec = (uint4*)c.node_sum;
     JTP_HIP(&c, launch_morton_keys(d_pos, d_species, c.keys0, c.vals0, rec, n, s));
-    JTP_HIP(&c, radix_sort_pairs(c, n, s));
+    JTP_HIP(&c, radix_sort_pairs(c, n, s, kKeyLowBits));
     const bool tiles = c.p.pm_enabled && c.p.precision == JTP_FP32;  // tile starts for the FP32 PM
     JTP_HIP(&c, launch_gather_parts(rec, c.vals0, c.parts, n, tiles ? c.tile_start : nullptr, c.logG, s));
     if (timed) JTP_HIP(&c, hipEventRecord(c.ev[1], s));

@@ -63,6 +63,14 @@ __host__ __device__ inline uint32_t compact3_21(uint64_t v) {
 __host__ __device__ inline uint64_t morton_fixed(uint32_t ux, uint32_t uy, uint32_t uz) {
     return spread3_21(ux >> 11) | (spread3_21(uy >> 11) << 1) | (spread3_21(uz >> 11) << 2);
 }
+// The particle sort orders by the Morton key with its low kKeyLowBits bits cleared (15-16 bits per
+// axis, L / 2^16 and finer left in input order): every user of the order reads key prefixes (PM
+// cells and tiles, walk seed cells, the radix tree, whose boxes are exact whatever the order inside a
+// key), so the two radix passes over those bits are skipped.
+constexpr int kKeyLowBits = 16;
+constexpr uint64_t kKeySortMask = ~((1ull << kKeyLowBits) - 1);
+static_assert(kKeyLowBits % 8 == 0 && (64 - kKeyLowBits) / 8 % 2 == 0,
+              "whole 8-bit radix passes, an even number of them (the result lands in the first buffer)");
 
 // Split-function coefficients for the FP32 walk, rho^2 = r^2 + eps^2, x = rho / (2 r_s):
 //  * particle pairs under erfc + Gaussian (always cut at r_cut): S(x) - 1 = x^3 h(x^2) with h entire,
@@ -210,7 +218,8 @@ hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, f
                                   float inv_cell, hipStream_t s);
 hipError_t launch_morton30_keys_f32(const float* pos, uint64_t* keys, int32_t* idx, int64_t n, float box_half,
                                     hipStream_t s);
-hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s);   // keys0/vals0 -> keys0/vals0 (sorted)
+// keys0/vals0 -> keys0/vals0 sorted by the key bits from first_bit (a multiple of 16) up, stable
+hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s, int first_bit = 0);
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // parts[i] = rec[perm[i]]; with tile_start (FP32 PM), also the first sorted particle of every
 // Morton-aligned 8^3-cell tile of the 2^logG mesh, tile_start[ntile] = n

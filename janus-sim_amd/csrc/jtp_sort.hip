@@ -44,7 +44,8 @@ hipError_t launch_fixed_from_f32(const float* pos, uint32_t* out, int64_t n3, do
 }
 
 // ------------------------------------------------------------------ Morton keys
-// Native keys: the top 21 bits of each fixed-point axis, interleaved (x in bit 0).  Each particle
+// Native keys: the top 21 bits of each fixed-point axis, interleaved (x in bit 0), with the low
+// kKeyLowBits bits cleared (the sort skips their passes).  Each particle
 // is also packed into one 16-byte record (x, y, z, species) in input order, so that the gather
 // after the sort reads one aligned 16-byte word per particle instead of 12 bytes that may straddle
 // two segments plus a byte in a third.
@@ -54,7 +55,7 @@ __global__ void k_morton_keys(const uint32_t* __restrict__ pos, const int8_t* __
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    keys[i] = morton_fixed(x, y, z);
+    keys[i] = morton_fixed(x, y, z) & kKeySortMask;
     vals[i] = (int32_t)i;
     rec[i] = make_uint4(x, y, z, (uint32_t)(int32_t)species[i]);
 }
@@ -117,7 +118,7 @@ hipError_t launch_morton_keys_f32(const float* pos, uint64_t* keys, int64_t n, f
 }
 
 // ------------------------------------------------------------------ radix sort
-// Pass structure (8 passes x 8 bits, stable):
+// Pass structure (8-bit digits from first_bit up, stable; 6 passes for the particle keys):
 //   k_radix_hist    per-tile digit counts, stored digit-major: hist[d * nblk + b]
 //   k_scan_chunks   exclusive scan of 2048-entry chunks of hist, chunk totals -> part[]
 //   k_scan_part     exclusive scan of part[] (one workgroup)
@@ -276,8 +277,9 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t* __restric
     }
 }
 
-hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s) {
+hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s, int first_bit) {
     if (n <= 1) return hipGetLastError();
+    if (first_bit < 0 || first_bit > 48 || first_bit % 16 != 0) return hipErrorInvalidValue;
     const int nblk = (int)((n + kSortTile - 1) / kSortTile);
     const int64_t m = (int64_t)nblk * 256;
     const int64_t nchunk = (m + kScanChunk - 1) / kScanChunk;
@@ -285,8 +287,7 @@ hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s) {
     uint64_t* kb = c.keys1;
     int32_t* va = c.vals0;
     int32_t* vb = c.vals1;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = pass * 8;
+    for (int shift = first_bit; shift < 64; shift += 8) {
         k_radix_hist<<<dim3(nblk), dim3(256), 0, s>>>(ka, n, shift, c.hist, nblk);
         k_scan_chunks<<<dim3((unsigned)nchunk), dim3(256), 0, s>>>(c.hist, m, c.hist_blk);
         k_scan_part<<<dim3(1), dim3(256), 0, s>>>(c.hist_blk, nchunk);
@@ -294,7 +295,7 @@ hipError_t radix_sort_pairs(Ctx& c, int64_t n, hipStream_t s) {
         uint64_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
     }
-    // 8 passes: the result is back in keys0/vals0
+    // an even number of passes: the result is back in keys0/vals0
     return hipGetLastError();
 }
 
