@@ -346,7 +346,7 @@ int jtp_create(const jtp_params* p, jtp_ctx** out) {
         if (e == hipSuccess) e = dalloc(c.dense_slot, (size_t)G3 + 1);
     } else {
         if (e == hipSuccess) e = dalloc(c.tile_start, (size_t)(G3 >> 9) + 1);
-        if (e == hipSuccess) e = dalloc(c.dep_tbuf, (size_t)(G3 >> 9) * dep_tile_slots());
+        if (e == hipSuccess) e = dalloc(c.dep_tbuf, (size_t)(G3 >> 9) * dep_tile_record());
     }
     if (e == hipSuccess) e = dalloc(c.cell4, (size_t)2 * 4097 + 64);
     if (e == hipSuccess) e = dalloc(c.cellnode, (size_t)4096);

@@ -234,7 +234,7 @@ hipError_t launch_cell_start(const uint64_t* keys, int64_t n, int shift, int64_t
 hipError_t pm_deposit(Ctx& c, int64_t n, hipStream_t s);           // grid <- sum q W
 int64_t dep_chunk_cap(int64_t n);  // FP32 deposit: chunk entries for up to n particles
 int64_t dep_chunk_nodes();         // FP32 deposit: int64 node sums per chunk entry
-int64_t dep_tile_slots();          // FP32 deposit: floats per tile record (own blocks, beyond nodes)
+int64_t dep_tile_record();         // FP32 deposit: floats per tile record (own blocks, beyond nodes)
 // phi <- Green * rho; matched = 1 multiplies by the gradient-stencil match (the force's Green
 // function when deconvolving; -1 = as the context's force), 0 leaves the potential unmatched
 hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int matched = -1);

@@ -1215,7 +1215,7 @@ hipError_t pm_fft_solve(Ctx& c, const void* rho, void* phi, hipStream_t s, int m
 // ceil(np / kInterpChunk) <= 2 n / kInterpChunk
 int64_t dep_chunk_cap(int64_t n) { return 2 * n / kInterpChunk + 2; }
 int64_t dep_chunk_nodes() { return kDN1 + kDN2; }
-int64_t dep_tile_slots() { return dep_rec<true>(); }
+int64_t dep_tile_record() { return dep_rec<true>(); }
 
 static hipError_t deposit_fp32(Ctx& c, int64_t n, hipStream_t s) {
     const int64_t G3 = c.G3();
